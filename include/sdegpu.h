@@ -1,0 +1,190 @@
+/*
+ * sdegpu.h - C ABI of libsdegpu.so, the MI355X (gfx950) path-integration
+ * library behind sdepy_amd.
+ *
+ * sdepy has no native code and no FFI: its hot path is the Python/NumPy loop
+ * paths_generator._generate_paths -> integrator.euler_next -> SDE.A / SDE.dZ
+ * -> wiener_source / poisson_source / cpoisson_source -> process.pmean/pvar.
+ * Each entry point below replaces one piece of that loop; the citation on
+ * each declaration names the reference code (sdepy/<file>:<line>) it stands
+ * in for.  The ctypes binding that calls them is sdepy_amd/_lib.py.
+ *
+ * Conventions
+ *   - every function returns an int status: SDEGPU_OK or one of the error
+ *     codes below; sdegpu_last_error() returns the message of the last failure
+ *     on the calling thread.  The host layer maps the codes onto the Python
+ *     exception types sdepy raises for the same conditions.
+ *   - all array arguments are caller-owned device pointers (HBM) unless
+ *     stated otherwise; nothing here allocates device memory.  Functions that
+ *     need scratch space take a workspace pointer and size, and a
+ *     *_workspace() query returns the size they need.
+ *   - `stream` is a hipStream_t (0 = the null stream).  Every launch is
+ *     asynchronous on that stream.
+ *   - arrays are C-ordered with the paths axis last and contiguous, as in
+ *     sdepy's process layout (N_t,) + vshape + (paths,) (infrastructure.py:423-456).
+ */
+#ifndef SDEGPU_H
+#define SDEGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped to Python exceptions by the host layer) ---- */
+#define SDEGPU_OK 0
+#define SDEGPU_EINVAL 1       /* ValueError */
+#define SDEGPU_EINDEX 2       /* IndexError */
+#define SDEGPU_ERUNTIME 3     /* RuntimeError (HIP failure) */
+#define SDEGPU_ENOMEM 4       /* MemoryError (workspace too small) */
+#define SDEGPU_EUNSUPPORTED 5 /* NotImplementedError */
+
+/* ---- element types ---- */
+#define SDEGPU_F32 0
+#define SDEGPU_F64 1
+
+/* ---- preset SDEs (integration.py:2053-2731) ---- */
+#define SDEGPU_MODEL_WIENER 0   /* wiener_SDE.sde        integration.py:2068-2069 */
+#define SDEGPU_MODEL_LOGNORM 1  /* lognorm_SDE.sde       integration.py:2127-2130 (log=True) */
+#define SDEGPU_MODEL_OU 2       /* ornstein_uhlenbeck_SDE / hull_white_SDE  :2198-2199, :2257-2272 */
+#define SDEGPU_MODEL_CIR 3      /* cox_ingersoll_ross_SDE.sde  integration.py:2351-2355 */
+#define SDEGPU_MODEL_HESTON 4   /* full_heston_SDE.sde   integration.py:2425-2433 (full truncation) */
+#define SDEGPU_MODEL_JUMPDIFF 5 /* jumpdiff_SDE.sde      integration.py:2594-2608 (log=True) */
+
+/* ---- how the working state is stored (SDE.let, integration.py:1495-1528) ---- */
+#define SDEGPU_OUT_IDENTITY 0   /* out[v] = x[v] (SDE.let default) */
+#define SDEGPU_OUT_SUM 1        /* out[o] = sum_f x[o*group+f]  (hull_white_SDE.let :2271-2272) */
+#define SDEGPU_OUT_FIRST 2      /* out[o] = x[o], o < nout      (heston_SDE.let :2533-2537) */
+
+#define SDEGPU_MAX_VARS 8
+
+/*
+ * One Euler-Maruyama pass over a consolidated step timeline.
+ *
+ * Replaces paths_generator._generate_paths (integration.py:392-474) with the
+ * integrator.euler_next update x_new = x + sum_id A[id]*dZ[id]
+ * (integration.py:707-723), SDE.A / SDE.dZ (integration.py:1216-1234), the
+ * source draws (wiener_source.__call__ infrastructure.py:1503-1560,
+ * poisson_source.__call__ :1617-1633, cpoisson_source.__call__ :2017-2040),
+ * SDE.store/let (integration.py:1175-1185) and the exp of SDE.exit for log
+ * SDEs (integration.py:1193-1197).
+ *
+ * Per-step table `coef` (dtype, row width ncoef), row s:
+ *   [0] ds, [1] sqrt(|ds|),
+ *   [2 .. 2+M(M+1)/2) packed lower Cholesky factor of corr (rows) if has_corr,
+ *   then the model parameters, each a block of M (or N = M/2 for Heston):
+ *     WIENER   mu[M], sigma[M]
+ *     LOGNORM  mu - sigma^2/2 [M], sigma[M]
+ *     OU       theta[M], k[M], sigma[M]
+ *     CIR      theta[M], k[M], xi[M]
+ *     HESTON   mu[N], sigma[N], theta[N], k[N], xi[N]   (state = [log x[N], y[N]])
+ *     JUMPDIFF mu - sigma^2/2 [M], sigma[M], a[M], b[M]  (jumps a + b*N(0,1))
+ * SDE parameters are evaluated at the step start, correlation and intensity at
+ * the step midpoint, as sdepy does (integration.py:1218-1221,
+ * infrastructure.py:1540, :1629).
+ */
+typedef struct sdegpu_euler_problem {
+  int32_t model;      /* SDEGPU_MODEL_* */
+  int32_t dtype;      /* SDEGPU_F32 or SDEGPU_F64: state, table and output type */
+  int32_t nvars;      /* M = prod(wshape), 1..SDEGPU_MAX_VARS */
+  int32_t nout;       /* values stored per path and output row = prod(xshape) */
+  int32_t out_mode;   /* SDEGPU_OUT_* */
+  int32_t out_group;  /* factor count for SDEGPU_OUT_SUM */
+  uint32_t exp_mask;  /* bit o set: store exp(value o) (log-integrated components) */
+  int32_t has_corr;   /* table rows carry a Cholesky factor */
+  int64_t paths;      /* 1 .. 2^32-1 */
+  int32_t nsteps;     /* Euler intervals in this pass (>= 0) */
+  int32_t ncoef;      /* table row width */
+  const void* coef;   /* device [nsteps][ncoef] */
+  const double* pois; /* device [nsteps][nvars][2] = (lam*|ds|, exp(-lam*|ds|)), JUMPDIFF only */
+  const int32_t* store_row; /* device [nsteps]: output row reached after step s, or -1 */
+  int32_t row0;       /* output row of the initial state, or -1 */
+  int32_t stream_tag; /* 0 for the forward pass, 16 for the backward pass (i0 != 0) */
+  double x0[SDEGPU_MAX_VARS]; /* initial working state (log space for log SDEs) */
+  const void* x0_paths;       /* optional device [nvars][paths] initial state, overrides x0 */
+  void* out;                  /* device [nrows][nout][paths] */
+  const void* replay_dw;      /* optional device [nsteps][nvars][paths]: recorded dw increments */
+  const void* replay_dj;      /* optional device [nsteps][nvars][paths]: recorded dj increments */
+  int32_t* info_paths;        /* optional device [nvars][paths]: Heston y<0 counts (first N
+                                 rows used) or jump counts (full_heston_SDE.info_next :2435-2439,
+                                 jumpdiff_SDE.info_next :2610-2619) */
+  long long* info_rows;       /* optional device [nrows]: jump totals per output interval */
+  uint32_t key_w[2];          /* Philox key of the dw source */
+  uint32_t epoch_w;           /* Philox epoch of the dw source */
+  uint32_t key_j[2];          /* Philox key of the dj source (JUMPDIFF) */
+  uint32_t epoch_j;           /* Philox epoch of the dj source */
+} sdegpu_euler_problem;
+
+/* Library and device queries. */
+int sdegpu_version(void);
+int sdegpu_last_error(char* buf, size_t len);
+int sdegpu_device_count(int* count);
+
+/* Euler pass, see sdegpu_euler_problem. */
+int sdegpu_euler(const sdegpu_euler_problem* prob, void* stream);
+
+/*
+ * Cross-path moments of each row of a [rows][paths] array:
+ * mean[r] = sum_p x / n and m2[r] = sum_p (x - mean)^2, accumulated in FP64
+ * with a per-row shift and merged in a fixed order (deterministic).
+ * Replaces process.pmean / pvar / psum / pstd, i.e. ndarray.mean/var/sum/std
+ * over the paths axis (infrastructure.py:851-889).  `mean` and `m2` are device
+ * [rows] doubles.
+ */
+size_t sdegpu_moments_workspace(int64_t rows, int64_t paths);
+int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths, double* mean,
+                        double* m2, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * FP64 sums of a European payoff of the values x[paths]:
+ *   kind 0: scale*max(x-strike, 0), 1: scale*max(strike-x, 0), 2: scale*x.
+ * out (device double[2]) = (sum, sum of squares).  Replaces the
+ * np.maximum(X[-1]-K, 0)*exp(-r*T) + montecarlo step of sdepy's Black-Scholes
+ * test (tests/test_quant.py:608-611).
+ */
+size_t sdegpu_payoff_workspace(int64_t paths);
+int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t kind, double strike,
+                          double scale, double* out, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/*
+ * Stand-alone source draws (the source protocol `dz(t, dt)`,
+ * infrastructure.py:1291-1333).  Slot s of the output plays the role of step
+ * s of an Euler pass with the same key, epoch and stream tag.
+ *
+ * sdegpu_wiener_fill: wiener_source.__call__ (infrastructure.py:1503-1560).
+ *   coef [nslots][ncoef] dtype with row layout [ds, sqrt|ds|, L...] as above;
+ *   out [nslots][nvars][paths] = (L z) * sqrt|ds|.
+ * sdegpu_poisson_fill: poisson_source.__call__ (infrastructure.py:1617-1633).
+ *   pois [nslots][nvars][2] = (lam*|ds|, exp(-lam*|ds|)); sign [nslots] = sign(ds);
+ *   out int64 [nslots][nvars][paths].
+ * sdegpu_cpoisson_fill: cpoisson_source.__call__ with y = norm_rv(a, b)
+ *   (infrastructure.py:2017-2040, :1653-1664).  ab [nslots][nvars][2] dtype;
+ *   out [nslots][nvars][paths] dtype; dn_out optional int64 like poisson_fill.
+ */
+int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
+                       int32_t nvars, int64_t paths, int64_t nslots, const void* coef,
+                       int32_t ncoef, int32_t has_corr, void* out, void* stream);
+int sdegpu_poisson_fill(const uint32_t key[2], uint32_t epoch, uint32_t stream_tag, int32_t nvars,
+                        int64_t paths, int64_t nslots, const double* pois, const int32_t* sign,
+                        long long* out, void* stream);
+int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
+                         uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
+                         const double* pois, const int32_t* sign, const void* ab, void* out,
+                         long long* dn_out, void* stream);
+
+/*
+ * Philox4x32-10 evaluation, for known-answer tests.
+ * sdegpu_philox_host runs on the CPU (no GPU needed); sdegpu_philox_fill runs
+ * the device twin over n counters (device uint32 [n][4] in, [n][4] out).
+ */
+int sdegpu_philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+int sdegpu_philox_fill(const uint32_t key[2], const uint32_t* ctr, uint32_t* out, int64_t n,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDEGPU_H */

@@ -1,0 +1,189 @@
+"""ctypes binding of libsdegpu.so and the device plumbing around it.
+
+The library is the product: every simulation and every cross-path reduction
+of sdepy_amd is a call into it.  There is no CPU fallback; when the library or
+a ROCm GPU is missing, the calls raise.
+
+PyTorch is used only as the allocator and stream provider for device buffers
+(``torch.empty(..., device='cuda')`` and ``torch.cuda.current_stream()``);
+no torch kernel computes anything on the hot path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libsdegpu.so')
+
+# status codes (include/sdegpu.h)
+OK, EINVAL, EINDEX, ERUNTIME, ENOMEM, EUNSUPPORTED = range(6)
+F32, F64 = 0, 1
+MODEL_WIENER, MODEL_LOGNORM, MODEL_OU, MODEL_CIR, MODEL_HESTON, \
+    MODEL_JUMPDIFF = range(6)
+OUT_IDENTITY, OUT_SUM, OUT_FIRST = range(3)
+MAX_VARS = 8
+STREAM_BACKWARD = 16
+
+_EXC = {EINVAL: ValueError, EINDEX: IndexError, ERUNTIME: RuntimeError,
+        ENOMEM: MemoryError, EUNSUPPORTED: NotImplementedError}
+
+c_int32, c_int64, c_uint32, c_double, c_void_p, c_size_t = (
+    ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_double,
+    ctypes.c_void_p, ctypes.c_size_t)
+
+
+class EulerProblem(ctypes.Structure):
+    """Mirror of sdegpu_euler_problem (include/sdegpu.h)."""
+    _fields_ = [
+        ('model', c_int32), ('dtype', c_int32), ('nvars', c_int32),
+        ('nout', c_int32), ('out_mode', c_int32), ('out_group', c_int32),
+        ('exp_mask', c_uint32), ('has_corr', c_int32),
+        ('paths', c_int64), ('nsteps', c_int32), ('ncoef', c_int32),
+        ('coef', c_void_p), ('pois', c_void_p), ('store_row', c_void_p),
+        ('row0', c_int32), ('stream_tag', c_int32),
+        ('x0', c_double * MAX_VARS), ('x0_paths', c_void_p),
+        ('out', c_void_p), ('replay_dw', c_void_p), ('replay_dj', c_void_p),
+        ('info_paths', c_void_p), ('info_rows', c_void_p),
+        ('key_w', c_uint32 * 2), ('epoch_w', c_uint32),
+        ('key_j', c_uint32 * 2), ('epoch_j', c_uint32),
+    ]
+
+
+_U32x2 = c_uint32 * 2
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ('sdegpu_version', ctypes.c_int, []),
+    ('sdegpu_last_error', ctypes.c_int, [ctypes.c_char_p, c_size_t]),
+    ('sdegpu_device_count', ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    ('sdegpu_euler', ctypes.c_int, [ctypes.POINTER(EulerProblem), c_void_p]),
+    ('sdegpu_moments_workspace', c_size_t, [c_int64, c_int64]),
+    ('sdegpu_path_moments', ctypes.c_int,
+     [c_void_p, c_int32, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+      c_size_t, c_void_p]),
+    ('sdegpu_payoff_workspace', c_size_t, [c_int64]),
+    ('sdegpu_payoff_moments', ctypes.c_int,
+     [c_void_p, c_int32, c_int64, c_int32, c_double, c_double, c_void_p,
+      c_void_p, c_size_t, c_void_p]),
+    ('sdegpu_wiener_fill', ctypes.c_int,
+     [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
+      c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
+    ('sdegpu_poisson_fill', ctypes.c_int,
+     [_U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64, c_void_p,
+      c_void_p, c_void_p, c_void_p]),
+    ('sdegpu_cpoisson_fill', ctypes.c_int,
+     [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ('sdegpu_philox_host', ctypes.c_int,
+     [ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
+      ctypes.POINTER(c_uint32)]),
+    ('sdegpu_philox_fill', ctypes.c_int,
+     [_U32x2, c_void_p, c_void_p, c_int64, c_void_p]),
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded library (loaded once); raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                'libsdegpu.so is not built ({}): run '
+                '`python -c "import __graft_entry__ as g; g.build()"` or '
+                '`python sdepy_amd/_build.py`'.format(LIB_PATH))
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(handle, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(512)
+    lib().sdegpu_last_error(buf, 512)
+    return buf.value.decode(errors='replace')
+
+
+def check(status):
+    """Raise the Python exception that matches a library status code."""
+    if status != OK:
+        raise _EXC.get(status, RuntimeError)(last_error())
+
+
+def key2(key):
+    return _U32x2(int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF)
+
+
+# ---------------------------------------------------------------------------
+# device plumbing (torch as allocator / stream provider only)
+# ---------------------------------------------------------------------------
+
+_torch = None
+
+
+def torch_mod():
+    global _torch
+    if _torch is None:
+        import torch
+        _torch = torch
+    return _torch
+
+
+def require_gpu():
+    """Raise unless a ROCm GPU is usable (no CPU fallback exists)."""
+    torch = torch_mod()
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            'sdepy_amd runs its path integration on a ROCm GPU (MI355X, '
+            'gfx950) and none is available')
+    lib()
+    return torch
+
+
+def stream():
+    return ctypes.c_void_p(torch_mod().cuda.current_stream().cuda_stream)
+
+
+_NP2TORCH = None
+
+
+def to_device(a):
+    """Copy a NumPy array to the current GPU; returns the torch tensor."""
+    torch = require_gpu()
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a).to('cuda', non_blocking=False)
+
+
+def empty(shape, dtype):
+    torch = require_gpu()
+    tdt = {np.dtype(np.float32): torch.float32,
+           np.dtype(np.float64): torch.float64,
+           np.dtype(np.int32): torch.int32,
+           np.dtype(np.int64): torch.int64,
+           np.dtype(np.uint32): torch.int32,
+           np.dtype(np.uint8): torch.uint8}[np.dtype(dtype)]
+    return torch.empty(tuple(int(s) for s in shape), dtype=tdt, device='cuda')
+
+
+def zeros(shape, dtype):
+    t = empty(shape, dtype)
+    t.zero_()
+    return t
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def to_host(t):
+    return t.cpu().numpy()
+
+
+def workspace(nbytes):
+    """A device scratch buffer of at least nbytes (kept by the caller)."""
+    return empty((max(int(nbytes), 8),), np.uint8)

@@ -1,0 +1,393 @@
+// Euler-Maruyama path kernels: one thread per path, the whole step loop of
+// sdepy's paths_generator._generate_paths (integration.py:392-474) inside the
+// thread, the working state in registers, Philox normals generated in
+// registers, and coalesced path-major stores of the output rows.
+//
+// Per step the thread
+//   1. draws M standard normals from its own Philox stream (or reads the
+//      recorded increments in replay mode), correlates them with the packed
+//      Cholesky factor of the step and scales them by sqrt(|ds|)
+//      (wiener_source.__call__, infrastructure.py:1503-1560);
+//   2. evaluates the preset's coefficients and applies
+//      x_new = x + (A_dt*ds + A_dw*dw [+ dj])  (integrator.euler_next,
+//      integration.py:707-723, with the increments summed first, then added);
+//   3. stores the state (SDE.let, exp for log SDEs) when the step lands on a
+//      point of the output timeline (paths_generator._store_if_reached,
+//      integration.py:377-390).
+// The per-step table is the same for every path, so its loads are scalar
+// (SMEM) loads served by the scalar cache, not per-lane traffic.
+#include "common.hpp"
+#include "philox.hpp"
+
+namespace sdegpu {
+
+struct EulerArgs {
+  const void* coef;
+  const double* pois;
+  const int32_t* store_row;
+  const void* x0_paths;
+  void* out;
+  const void* replay_dw;
+  const void* replay_dj;
+  int32_t* info_paths;
+  long long* info_rows;
+  double x0[SDEGPU_MAX_VARS];
+  int64_t paths;
+  int32_t nsteps, ncoef, poff, nout, out_group, has_corr, row0;
+  uint32_t exp_mask;
+  uint32_t c2w, c2p, c2j;  // counter word c2 of the normal, Poisson and jump streams
+  uint32_t kw0, kw1, ew, kj0, kj1, ej;
+};
+
+// Steps per chunk so that a chunk consumes a whole number of Philox blocks:
+// the normal with index g = s*M + v of a path lives in block g/4, lane g%4.
+template <int M>
+struct Chunking {
+  static constexpr int CH = (M % 4 == 0) ? 1 : ((M % 2 == 0) ? 2 : 4);
+  static constexpr int NZ = CH * M;
+};
+
+constexpr int kPoissonCap = 4096;
+
+template <typename T>
+__device__ __forceinline__ T fast_sqrt(T v);
+template <>
+__device__ __forceinline__ float fast_sqrt<float>(float v) {
+  return __builtin_amdgcn_sqrtf(v);
+}
+template <>
+__device__ __forceinline__ double fast_sqrt<double>(double v) {
+  return sqrt(v);
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void store_state(const EulerArgs& a, int row, const T (&x)[M],
+                                            int64_t gid) {
+  const int64_t P = a.paths;
+  T* out = (T*)a.out + (int64_t)row * a.nout * P + gid;
+  if (a.out_group == 1) {
+#pragma unroll
+    for (int o = 0; o < M; ++o) {
+      if (o < a.nout) {
+        T val = x[o];
+        if ((a.exp_mask >> o) & 1u) val = exp(val);
+        __builtin_nontemporal_store(val, out + o * P);
+      }
+    }
+  } else {
+    // hull_white_SDE.let: out_x[...] = x.sum(axis=-2) over each factor group.
+#pragma unroll
+    for (int o = 0; o < M; ++o) {
+      if (o < a.nout) {
+        const int lo = o * a.out_group, hi = lo + a.out_group;
+        T val = (T)0;
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+          if (v >= lo && v < hi) val = val + x[v];
+        if ((a.exp_mask >> o) & 1u) val = exp(val);
+        __builtin_nontemporal_store(val, out + o * P);
+      }
+    }
+  }
+}
+
+template <int MODEL, typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a) {
+  constexpr int CH = Chunking<M>::CH;
+  constexpr int NZ = Chunking<M>::NZ;
+  constexpr bool kJump = MODEL == SDEGPU_MODEL_JUMPDIFF;
+  constexpr bool kHeston = MODEL == SDEGPU_MODEL_HESTON;
+  constexpr int N = kHeston ? M / 2 : M;
+
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= a.paths) return;
+  const int64_t P = a.paths;
+  const uint32_t path = (uint32_t)gid;
+
+  T x[M];
+  if (a.x0_paths) {
+#pragma unroll
+    for (int v = 0; v < M; ++v) x[v] = ((const T*)a.x0_paths)[v * P + gid];
+  } else {
+#pragma unroll
+    for (int v = 0; v < M; ++v) x[v] = (T)a.x0[v];
+  }
+  if (a.row0 >= 0) store_state<T, M>(a, a.row0, x, gid);
+
+  int cnt[M];
+#pragma unroll
+  for (int v = 0; v < M; ++v) cnt[v] = 0;
+  int interval_jumps = 0;
+
+  const T* coef = (const T*)a.coef;
+  const bool replay = a.replay_dw != nullptr;
+  const bool replay_j = a.replay_dj != nullptr;
+
+  for (int s0 = 0; s0 < a.nsteps; s0 += CH) {
+    T z[NZ];
+    uint32_t upois[kJump ? NZ : 1];
+    if (!replay) {
+      const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
+#pragma unroll
+      for (int j = 0; j < NZ / 4; ++j)
+        normals4<T>(b0 + j, path, a.c2w, a.ew, a.kw0, a.kw1, z + 4 * j);
+    }
+    if constexpr (kJump) {
+      if (!replay_j) {
+        const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
+#pragma unroll
+        for (int j = 0; j < NZ / 4; ++j) {
+          const ctr4 r = philox4x32_10(ctr4{b0 + j, path, a.c2p, a.ej}, a.kj0, a.kj1);
+          upois[4 * j + 0] = r.x;
+          upois[4 * j + 1] = r.y;
+          upois[4 * j + 2] = r.z;
+          upois[4 * j + 3] = r.w;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int s = s0 + k;
+      if (s >= a.nsteps) break;
+      const T* c = coef + (int64_t)s * a.ncoef;
+      const T ds = c[0];
+
+      T dw[M];
+      if (replay) {
+        const T* rp = (const T*)a.replay_dw + (int64_t)s * M * P + gid;
+#pragma unroll
+        for (int v = 0; v < M; ++v) dw[v] = rp[v * P];
+      } else {
+        const T sq = c[1];
+        if (a.has_corr) {
+          const T* L = c + 2;
+          int idx = 0;
+#pragma unroll
+          for (int i = 0; i < M; ++i) {
+            T acc = (T)0;
+#pragma unroll
+            for (int j = 0; j <= i; ++j) acc = acc + L[idx++] * z[k * M + j];
+            dw[i] = acc * sq;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < M; ++v) dw[v] = z[k * M + v] * sq;
+        }
+      }
+
+      const T* p = c + a.poff;
+      if constexpr (MODEL == SDEGPU_MODEL_WIENER || MODEL == SDEGPU_MODEL_LOGNORM) {
+        // wiener_SDE: {'dt': mu, 'dw': sigma}; lognorm_SDE: {'dt': mu - sigma^2/2, 'dw': sigma}
+#pragma unroll
+        for (int v = 0; v < M; ++v) x[v] = x[v] + (p[v] * ds + p[M + v] * dw[v]);
+      } else if constexpr (MODEL == SDEGPU_MODEL_OU) {
+        // ornstein_uhlenbeck_SDE: {'dt': k*(theta - x), 'dw': sigma}
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+          const T adt = p[M + v] * (p[v] - x[v]);
+          x[v] = x[v] + (adt * ds + p[2 * M + v] * dw[v]);
+        }
+      } else if constexpr (MODEL == SDEGPU_MODEL_CIR) {
+        // cox_ingersoll_ross_SDE: x+ = max(x, 0); {'dt': k*(theta - x+), 'dw': xi*sqrt(x+)}
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+          const T xp = x[v] > (T)0 ? x[v] : (T)0;
+          const T adt = p[M + v] * (p[v] - xp);
+          const T adw = p[2 * M + v] * fast_sqrt<T>(xp);
+          x[v] = x[v] + (adt * ds + adw * dw[v]);
+        }
+      } else if constexpr (kHeston) {
+        // full_heston_SDE (full truncation): y+ = max(y, 0);
+        //   log x: {'dt': mu - sigma*sigma*y+/2, 'dw': sigma*sqrt(y+)}
+        //   y:     {'dt': k*(theta - y+),        'dw': xi*sqrt(y+)}
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const T y = x[N + i];
+          cnt[i] += (y < (T)0) ? 1 : 0;
+          const T yp = y > (T)0 ? y : (T)0;
+          const T sy = fast_sqrt<T>(yp);
+          const T mu = p[i], sig = p[N + i], th = p[2 * N + i], kk = p[3 * N + i],
+                  xi = p[4 * N + i];
+          const T axt = mu - sig * sig * yp * (T)0.5;
+          const T axw = sig * sy;
+          const T ayt = kk * (th - yp);
+          const T ayw = xi * sy;
+          x[i] = x[i] + (axt * ds + axw * dw[i]);
+          x[N + i] = y + (ayt * ds + ayw * dw[N + i]);
+        }
+      } else if constexpr (kJump) {
+        // jumpdiff_SDE: {'dt': mu - sigma^2/2, 'dw': sigma, 'dj': 1}, no martingale correction;
+        // dj = sign(ds) * sum of n ~ Poisson(lam*|ds|) variates a + b*N(0,1)
+        // (cpoisson_source.__call__ with y = norm_rv(a, b)).
+        T dj[M];
+        if (replay_j) {
+          const T* rp = (const T*)a.replay_dj + (int64_t)s * M * P + gid;
+#pragma unroll
+          for (int v = 0; v < M; ++v) dj[v] = rp[v * P];
+        } else {
+          const double* pq = a.pois + (int64_t)s * M * 2;
+#pragma unroll
+          for (int v = 0; v < M; ++v) {
+            const int n =
+                poisson_inv(u01_f64(upois[k * M + v]), pq[2 * v], pq[2 * v + 1], kPoissonCap);
+            T acc = (T)0;
+            if (n > 0) {
+              const T ja = p[2 * M + v], jb = p[3 * M + v];
+              const uint32_t slot = (uint32_t)(s * M + v);
+              T zz[4];
+              for (int j = 0; j < n; ++j) {
+                if ((j & 3) == 0)
+                  normals4<T>(slot, path, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0,
+                              a.kj1, zz);
+                const int lane = j & 3;
+                const T zj = lane == 0 ? zz[0] : lane == 1 ? zz[1] : lane == 2 ? zz[2] : zz[3];
+                acc = acc + (ja + jb * zj);
+              }
+            }
+            const bool neg = ds < (T)0;
+            dj[v] = neg ? -acc : acc;
+            cnt[v] += neg ? -n : n;
+            interval_jumps += neg ? -n : n;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+          x[v] = x[v] + ((p[v] * ds + p[M + v] * dw[v]) + dj[v]);
+      }
+
+      const int row = a.store_row[s];
+      if (row >= 0) {
+        store_state<T, M>(a, row, x, gid);
+        if constexpr (kJump) {
+          if (a.info_rows && interval_jumps != 0)
+            atomicAdd((unsigned long long*)(a.info_rows + row), (unsigned long long)(long long)interval_jumps);
+          interval_jumps = 0;
+        }
+      }
+    }
+  }
+
+  if (a.info_paths && (kHeston || kJump)) {
+#pragma unroll
+    for (int v = 0; v < (kHeston ? N : M); ++v) a.info_paths[v * P + gid] = cnt[v];
+  }
+}
+
+using EulerKernel = void (*)(const EulerArgs);
+
+template <int MODEL, typename T>
+static EulerKernel pick_m(int m) {
+  switch (m) {
+    case 1: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 1>;
+    case 2: return k_euler<MODEL, T, 2>;
+    case 3: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 3>;
+    case 4: return k_euler<MODEL, T, 4>;
+    case 5: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 5>;
+    case 6: return k_euler<MODEL, T, 6>;
+    case 7: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 7>;
+    case 8: return k_euler<MODEL, T, 8>;
+    default: return nullptr;
+  }
+}
+
+template <typename T>
+static EulerKernel pick(int model, int m) {
+  switch (model) {
+    case SDEGPU_MODEL_WIENER: return pick_m<SDEGPU_MODEL_WIENER, T>(m);
+    case SDEGPU_MODEL_LOGNORM: return pick_m<SDEGPU_MODEL_LOGNORM, T>(m);
+    case SDEGPU_MODEL_OU: return pick_m<SDEGPU_MODEL_OU, T>(m);
+    case SDEGPU_MODEL_CIR: return pick_m<SDEGPU_MODEL_CIR, T>(m);
+    case SDEGPU_MODEL_HESTON: return pick_m<SDEGPU_MODEL_HESTON, T>(m);
+    case SDEGPU_MODEL_JUMPDIFF: return pick_m<SDEGPU_MODEL_JUMPDIFF, T>(m);
+    default: return nullptr;
+  }
+}
+
+static int param_blocks(int model, int m) {
+  switch (model) {
+    case SDEGPU_MODEL_WIENER:
+    case SDEGPU_MODEL_LOGNORM: return 2 * m;
+    case SDEGPU_MODEL_OU:
+    case SDEGPU_MODEL_CIR: return 3 * m;
+    case SDEGPU_MODEL_HESTON: return 5 * (m / 2);
+    case SDEGPU_MODEL_JUMPDIFF: return 4 * m;
+    default: return -1;
+  }
+}
+
+}  // namespace sdegpu
+
+using namespace sdegpu;
+
+extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
+  if (!pr) return fail(SDEGPU_EINVAL, "sdegpu_euler: null problem");
+  const int m = pr->nvars;
+  if (m < 1 || m > SDEGPU_MAX_VARS)
+    return fail(SDEGPU_EUNSUPPORTED,
+                "the GPU integrator supports 1..%d working-state components per path, not %d",
+                SDEGPU_MAX_VARS, m);
+  if (pr->dtype != SDEGPU_F32 && pr->dtype != SDEGPU_F64)
+    return fail(SDEGPU_EUNSUPPORTED, "unsupported dtype code %d", pr->dtype);
+  if (pr->paths < 1 || pr->paths > kMaxPaths)
+    return fail(SDEGPU_EINVAL, "paths must be in [1, %lld], not %lld", (long long)kMaxPaths,
+                (long long)pr->paths);
+  if (pr->nsteps < 0) return fail(SDEGPU_EINVAL, "negative step count %d", pr->nsteps);
+  if (pr->model == SDEGPU_MODEL_HESTON && (m % 2) != 0)
+    return fail(SDEGPU_EINVAL, "the Heston working state has an even size, not %d", m);
+  const int nparam = param_blocks(pr->model, m);
+  if (nparam < 0) return fail(SDEGPU_EUNSUPPORTED, "unknown model code %d", pr->model);
+  const int ncorr = pr->has_corr ? m * (m + 1) / 2 : 0;
+  const int poff = 2 + ncorr;
+  if (pr->ncoef < poff + nparam)
+    return fail(SDEGPU_EINVAL, "table row width %d < %d required by the model", pr->ncoef,
+                poff + nparam);
+  const int group = pr->out_mode == SDEGPU_OUT_SUM ? pr->out_group : 1;
+  if (group < 1 || pr->nout < 1 || (int64_t)pr->nout * group > m)
+    return fail(SDEGPU_EINVAL, "invalid output map: nout=%d group=%d for %d components",
+                pr->nout, group, m);
+  if (pr->nsteps > 0 && (!pr->coef || !pr->store_row))
+    return fail(SDEGPU_EINVAL, "missing step table");
+  if (!pr->out) return fail(SDEGPU_EINVAL, "missing output buffer");
+  const bool jump = pr->model == SDEGPU_MODEL_JUMPDIFF;
+  if (jump && pr->nsteps > 0 && !pr->replay_dj && !pr->pois)
+    return fail(SDEGPU_EINVAL, "jump model without Poisson table");
+  if ((int64_t)pr->nsteps * m > 0xFFFFFFFFll)
+    return fail(SDEGPU_EINVAL, "too many steps for the 32-bit Philox block counter");
+
+  EulerKernel k = pr->dtype == SDEGPU_F32 ? pick<float>(pr->model, m) : pick<double>(pr->model, m);
+  if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
+
+  EulerArgs a = {};
+  a.coef = pr->coef;
+  a.pois = pr->pois;
+  a.store_row = pr->store_row;
+  a.x0_paths = pr->x0_paths;
+  a.out = pr->out;
+  a.replay_dw = pr->replay_dw;
+  a.replay_dj = pr->replay_dj;
+  a.info_paths = pr->info_paths;
+  a.info_rows = pr->info_rows;
+  for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
+  a.paths = pr->paths;
+  a.nsteps = pr->nsteps;
+  a.ncoef = pr->ncoef;
+  a.poff = poff;
+  a.nout = pr->nout;
+  a.out_group = group;
+  a.has_corr = pr->has_corr ? 1 : 0;
+  a.row0 = pr->row0;
+  a.exp_mask = pr->exp_mask;
+  const uint32_t tag = (uint32_t)pr->stream_tag & 0xF0u;
+  a.c2w = tag | kStreamNormal;
+  a.c2p = tag | kStreamPoisson;
+  a.c2j = tag | kStreamJump;
+  a.kw0 = pr->key_w[0];
+  a.kw1 = pr->key_w[1];
+  a.ew = pr->epoch_w;
+  a.kj0 = pr->key_j[0];
+  a.kj1 = pr->key_j[1];
+  a.ej = pr->epoch_j;
+
+  hipLaunchKernelGGL(k, dim3(blocks_for(pr->paths)), dim3(kBlock), 0, (hipStream_t)stream, a);
+  return launch_status("k_euler");
+}

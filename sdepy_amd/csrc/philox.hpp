@@ -1,0 +1,135 @@
+// Counter-based random numbers for the Euler path kernels.
+//
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11): a 128-bit counter is mixed with a 64-bit key by ten
+// multiply/xor rounds.  Every draw of the simulation is addressed by a counter,
+// so a path can be regenerated at any step (the adjoint kernels rely on this)
+// and the host-side twin in the oracle reproduces any subset of paths.
+//
+// Counter layout used by every kernel in this library:
+//   c0 = block index inside the stream (4 draws per block)
+//   c1 = path index
+//   c2 = stream id in the low byte (0 Wiener normals, 1 Poisson uniforms,
+//        2 jump-size normals; bit 4 marks the backward pass of an i0 != 0
+//        simulation) | sub-index << 8 (jump blocks)
+//   c3 = epoch: the number of earlier simulations drawn from the same source
+// key = the 64-bit seed drawn once from the caller's numpy rng when the
+// source is created (sdepy's `rng` argument, infrastructure.py:1334-1341).
+//
+// Uniforms and normals:
+//   f64: u = (x + 0.5) * 2^-32, exact in double, in (0, 1)
+//   f32: u = fma(float(x), 2^-32, 2^-33), in (0, 1]
+//   Box-Muller on the lane pairs (0,1) and (2,3):
+//     z0 = sqrt(-2 ln u_a) cos(2 pi u_b),  z1 = sqrt(-2 ln u_a) sin(2 pi u_b)
+// The normal with index g of a path's stream lives in block g / 4, lane g % 4.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdegpu {
+
+struct ctr4 {
+  uint32_t x, y, z, w;
+};
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+enum StreamId : uint32_t {
+  kStreamNormal = 0u,
+  kStreamPoisson = 1u,
+  kStreamJump = 2u,
+  kStreamBackward = 16u,
+};
+
+__host__ __device__ __forceinline__ ctr4 philox_round(ctr4 c, uint32_t k0, uint32_t k1) {
+  const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
+  const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
+  ctr4 r;
+  r.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+  r.y = (uint32_t)p1;
+  r.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+  r.w = (uint32_t)p0;
+  return r;
+}
+
+__host__ __device__ __forceinline__ ctr4 philox4x32_10(ctr4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    c = philox_round(c, k0, k1);
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+  }
+  return philox_round(c, k0, k1);
+}
+
+// ---- uniforms -------------------------------------------------------------
+
+__host__ __device__ __forceinline__ double u01_f64(uint32_t x) {
+  return ((double)x + 0.5) * 0x1p-32;
+}
+
+__device__ __forceinline__ float u01_f32(uint32_t x) {
+  return fmaf((float)x, 0x1p-32f, 0x1p-33f);
+}
+
+// ---- Box-Muller pairs ------------------------------------------------------
+
+template <typename T>
+struct Gauss;
+
+template <>
+struct Gauss<float> {
+  // v_log_f32 is log2 and v_sin/v_cos_f32 take their argument in revolutions,
+  // so the pair costs one log, one sqrt, one sin and one cos instruction.
+  static __device__ __forceinline__ void pair(uint32_t a, uint32_t b, float& z0, float& z1) {
+    const float ua = u01_f32(a);
+    const float ub = u01_f32(b);
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(ua));
+    z0 = r * __builtin_amdgcn_cosf(ub);
+    z1 = r * __builtin_amdgcn_sinf(ub);
+  }
+};
+
+template <>
+struct Gauss<double> {
+  static __device__ __forceinline__ void pair(uint32_t a, uint32_t b, double& z0, double& z1) {
+    const double ua = u01_f64(a);
+    const double ub = u01_f64(b);
+    const double r = sqrt(-2.0 * log(ua));
+    double s, c;
+    sincos(6.283185307179586 * ub, &s, &c);
+    z0 = r * c;
+    z1 = r * s;
+  }
+};
+
+// Four normals from one Philox block.
+template <typename T>
+__device__ __forceinline__ void normals4(uint32_t blk, uint32_t path, uint32_t c2, uint32_t epoch,
+                                         uint32_t k0, uint32_t k1, T* z) {
+  const ctr4 r = philox4x32_10(ctr4{blk, path, c2, epoch}, k0, k1);
+  Gauss<T>::pair(r.x, r.y, z[0], z[1]);
+  Gauss<T>::pair(r.z, r.w, z[2], z[3]);
+}
+
+// Poisson variate by sequential inversion of the CDF, in double precision on
+// every target so that counts are bit-identical to the host twin.
+// lam = intensity * |ds|, p0 = exp(-lam) (precomputed on the host in double).
+// The search is bounded: `cap` stops it once the pmf underflows.
+__host__ __device__ __forceinline__ int poisson_inv(double u, double lam, double p0, int cap) {
+#pragma clang fp contract(off)
+  int n = 0;
+  double p = p0;
+  double f = p0;
+  while (u > f && n < cap) {
+    n += 1;
+    p = p * (lam / (double)n);
+    f = f + p;
+  }
+  return n;
+}
+
+}  // namespace sdegpu

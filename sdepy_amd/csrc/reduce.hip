@@ -1,0 +1,193 @@
+// Cross-path reductions in FP64.
+//
+// process.pmean / pvar / psum / pstd (infrastructure.py:851-889) reduce the
+// paths axis of a (N_t,)+vshape+(paths,) array, i.e. every row of a
+// [rows][paths] matrix.  Here each row is split into chunks of kChunk paths;
+// one block reduces one (row, chunk) pair to FP64 sums of the values shifted
+// by the row's first element, and a second kernel adds the chunk partials of
+// each row in chunk order.  No atomics: the result does not depend on
+// scheduling.
+#include "common.hpp"
+
+namespace sdegpu {
+
+constexpr int64_t kChunk = 65536;  // paths reduced by one block
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of (s1, s2) over the 256 threads of a block; the result is valid in
+// thread 0.  Every thread of the block must call it.
+__device__ __forceinline__ void block_sum2(double& s1, double& s2) {
+  __shared__ double sh[2][kBlock / 64];
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[0][wave] = s1;
+    sh[1][wave] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+      a += sh[0][w];
+      b += sh[1][w];
+    }
+    s1 = a;
+    s2 = b;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_moments_partial(const T* __restrict__ x, int64_t paths,
+                                                            int64_t nchunks,
+                                                            double* __restrict__ part) {
+  const int64_t row = blockIdx.x / nchunks;
+  const int64_t chunk = blockIdx.x % nchunks;
+  const T* xr = x + row * paths;
+  const double shift = (double)xr[0];
+  const int64_t lo = chunk * kChunk;
+  const int64_t hi = lo + kChunk < paths ? lo + kChunk : paths;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double d = (double)xr[i] - shift;
+    s1 += d;
+    s2 += d * d;
+  }
+  block_sum2(s1, s2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+template <typename T>
+__global__ void k_moments_final(const T* __restrict__ x, int64_t rows, int64_t paths,
+                                int64_t nchunks, const double* __restrict__ part,
+                                double* __restrict__ mean, double* __restrict__ m2) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  const double shift = (double)x[row * paths];
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t c = 0; c < nchunks; ++c) {
+    s1 += part[2 * (row * nchunks + c)];
+    s2 += part[2 * (row * nchunks + c) + 1];
+  }
+  const double n = (double)paths;
+  const double d = s1 / n;
+  mean[row] = shift + d;
+  const double v = s2 - s1 * d;
+  m2[row] = v > 0.0 ? v : 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_payoff_partial(const T* __restrict__ x, int64_t paths,
+                                                           int kind, double strike, double scale,
+                                                           double* __restrict__ part) {
+  const int64_t lo = (int64_t)blockIdx.x * kChunk;
+  const int64_t hi = lo + kChunk < paths ? lo + kChunk : paths;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double v = (double)x[i];
+    double f = kind == 0 ? v - strike : kind == 1 ? strike - v : v;
+    if (kind != 2) f = f > 0.0 ? f : 0.0;
+    f *= scale;
+    s1 += f;
+    s2 += f * f;
+  }
+  block_sum2(s1, s2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ part, int64_t n, double* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      a += part[2 * i];
+      b += part[2 * i + 1];
+    }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+}  // namespace sdegpu
+
+using namespace sdegpu;
+
+extern "C" size_t sdegpu_moments_workspace(int64_t rows, int64_t paths) {
+  if (rows < 1 || paths < 1) return 0;
+  const int64_t nchunks = (paths + kChunk - 1) / kChunk;
+  return (size_t)(rows * nchunks * 2 * sizeof(double));
+}
+
+extern "C" int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths,
+                                   double* mean, double* m2, void* ws, size_t ws_bytes,
+                                   void* stream) {
+  if (rows < 0 || paths < 1) return fail(SDEGPU_EINVAL, "path_moments: rows=%lld paths=%lld",
+                                         (long long)rows, (long long)paths);
+  if (rows == 0) return SDEGPU_OK;
+  if (!x || !mean || !m2) return fail(SDEGPU_EINVAL, "path_moments: null pointer");
+  const int64_t nchunks = (paths + kChunk - 1) / kChunk;
+  const int64_t nblocks = rows * nchunks;
+  if (nblocks > 0x7FFFFFFFll) return fail(SDEGPU_EINVAL, "path_moments: too many rows");
+  if (ws_bytes < sdegpu_moments_workspace(rows, paths) || !ws)
+    return fail(SDEGPU_ENOMEM, "path_moments: workspace of %zu bytes, %zu needed", ws_bytes,
+                sdegpu_moments_workspace(rows, paths));
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)ws;
+  const unsigned gfin = blocks_for(rows, 64);
+  if (dtype == SDEGPU_F32) {
+    hipLaunchKernelGGL(k_moments_partial<float>, dim3((unsigned)nblocks), dim3(kBlock), 0, st,
+                       (const float*)x, paths, nchunks, part);
+    if (int e = launch_status("k_moments_partial")) return e;
+    hipLaunchKernelGGL(k_moments_final<float>, dim3(gfin), dim3(64), 0, st, (const float*)x, rows,
+                       paths, nchunks, (const double*)part, mean, m2);
+  } else if (dtype == SDEGPU_F64) {
+    hipLaunchKernelGGL(k_moments_partial<double>, dim3((unsigned)nblocks), dim3(kBlock), 0, st,
+                       (const double*)x, paths, nchunks, part);
+    if (int e = launch_status("k_moments_partial")) return e;
+    hipLaunchKernelGGL(k_moments_final<double>, dim3(gfin), dim3(64), 0, st, (const double*)x,
+                       rows, paths, nchunks, (const double*)part, mean, m2);
+  } else {
+    return fail(SDEGPU_EUNSUPPORTED, "path_moments: dtype code %d", dtype);
+  }
+  return launch_status("k_moments_final");
+}
+
+extern "C" size_t sdegpu_payoff_workspace(int64_t paths) {
+  if (paths < 1) return 0;
+  return (size_t)(((paths + kChunk - 1) / kChunk) * 2 * sizeof(double));
+}
+
+extern "C" int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t kind,
+                                     double strike, double scale, double* out, void* ws,
+                                     size_t ws_bytes, void* stream) {
+  if (paths < 1 || !x || !out) return fail(SDEGPU_EINVAL, "payoff_moments: bad arguments");
+  if (kind < 0 || kind > 2) return fail(SDEGPU_EINVAL, "payoff_moments: payoff kind %d", kind);
+  if (ws_bytes < sdegpu_payoff_workspace(paths) || !ws)
+    return fail(SDEGPU_ENOMEM, "payoff_moments: workspace of %zu bytes, %zu needed", ws_bytes,
+                sdegpu_payoff_workspace(paths));
+  const int64_t nchunks = (paths + kChunk - 1) / kChunk;
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)ws;
+  if (dtype == SDEGPU_F32)
+    hipLaunchKernelGGL(k_payoff_partial<float>, dim3((unsigned)nchunks), dim3(kBlock), 0, st,
+                       (const float*)x, paths, kind, strike, scale, part);
+  else if (dtype == SDEGPU_F64)
+    hipLaunchKernelGGL(k_payoff_partial<double>, dim3((unsigned)nchunks), dim3(kBlock), 0, st,
+                       (const double*)x, paths, kind, strike, scale, part);
+  else
+    return fail(SDEGPU_EUNSUPPORTED, "payoff_moments: dtype code %d", dtype);
+  if (int e = launch_status("k_payoff_partial")) return e;
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, (const double*)part, nchunks, out);
+  return launch_status("k_sum_partials");
+}

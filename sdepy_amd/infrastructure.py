@@ -1,0 +1,835 @@
+"""The ``process`` array type and the stochasticity sources of sdepy_amd.
+
+Drop-in counterparts of sdepy's ``process`` (sdepy/infrastructure.py:272-1209)
+and memoryless sources ``wiener_source`` (:1392-1560), ``poisson_source``
+(:1566-1633) and ``cpoisson_source`` (:1846-2040), with the same constructor
+keywords, shapes and call protocol.  What differs is where the numbers come
+from: the sources draw from a counter-based Philox generator on the GPU,
+keyed once from the caller's NumPy ``rng``, and ``process.pmean/pvar/psum/
+pstd`` reduce the paths axis on the GPU in FP64.
+"""
+import numbers
+
+import numpy as np
+import scipy.interpolate
+import scipy.stats
+
+from . import _lib
+
+# --------------------------------------------------------------------------
+# default random number generator (sdepy/infrastructure.py:26-51)
+# --------------------------------------------------------------------------
+
+default_rng = np.random.default_rng()
+
+
+def _get_default_rng():
+    return default_rng
+
+
+def philox_key(rng):
+    """The 64-bit Philox key of a new source: one draw from the caller's
+    NumPy generator (a Generator, a legacy RandomState, or the np.random
+    module), so that seeding ``rng`` makes every simulation reproducible."""
+    if isinstance(rng, np.random.Generator):
+        k = rng.integers(0, 2 ** 32, size=2, dtype=np.uint64)
+    elif isinstance(rng, np.random.RandomState) or rng is np.random:
+        k = rng.randint(0, 2 ** 32, size=2, dtype=np.uint64)
+    elif hasattr(rng, 'integers'):
+        k = rng.integers(0, 2 ** 32, size=2)
+    else:
+        raise TypeError('`rng` must be a numpy random number generator, '
+                        'not {}'.format(type(rng)))
+    return (int(k[0]), int(k[1]))
+
+
+def _shape_setup(shape):
+    return (shape,) if isinstance(shape, numbers.Integral) else tuple(shape)
+
+
+def _const_rho_to_corr(rho):
+    """Correlation matrix from a constant ``rho`` (sdepy/infrastructure.py:
+    128-150): a scalar gives [[1, rho], [rho, 1]], a K-vector gives
+    [[I, diag(rho)], [diag(rho), I]]."""
+    if rho is None:
+        return None
+    rho = np.asarray(rho)
+    n = rho.size
+    if rho.shape not in {(), (n,), (n, 1)}:
+        raise ValueError(
+            'correlation ``rho`` should be a vector, possibly with a '
+            'trailing 1-dimensional axis matching the paths axis, not an '
+            'array with shape {}'.format(rho.shape))
+    if n == 1:
+        r = rho.reshape(())
+        return np.array(((1, r), (r, 1)))
+    r = rho.reshape(n)
+    eye, diag = np.eye(n), np.diag(r)
+    return np.concatenate((np.concatenate((eye, diag)),
+                           np.concatenate((diag, eye))), axis=1)
+
+
+def _corr_setup(corr, rho):
+    """sdepy's _get_corr_matrix (infrastructure.py:153-206): ``corr``
+    overrides ``rho``; either may be constant or callable in t."""
+    if corr is None and rho is None:
+        return None
+    if corr is not None:
+        if not callable(corr):
+            corr = np.asarray(corr, dtype=float)
+            if corr.ndim not in (2, 3) or corr.shape[0] != corr.shape[1] or \
+                    (corr.ndim == 3 and corr.shape[2] != 1):
+                raise ValueError(
+                    'the correlation matrix ``corr`` should be square, '
+                    'possibly with a trailing 1-dimensional axis matching '
+                    'the paths axis, not an array with shape {}'
+                    .format(corr.shape))
+        return corr
+    if callable(rho):
+        return lambda t: _const_rho_to_corr(rho(t))
+    return _const_rho_to_corr(np.asarray(rho))
+
+
+def _corr_at(corr, t):
+    c = corr(t) if callable(corr) else corr
+    c = np.asarray(c, dtype=float)
+    if c.ndim == 3:
+        if c.shape[2] != 1:
+            raise ValueError('invalid correlation matrix shape {}'
+                             .format(c.shape))
+        c = c[..., 0]
+    return c
+
+
+def cholesky_packed(corr, vshape):
+    """Packed lower Cholesky factor (row-major, j <= i) of the block-diagonal
+    correlation that ``corr`` induces on the flattened vshape: the last axis
+    is correlated, the leading axes are independent copies
+    (wiener_source, infrastructure.py:1524-1553).  FP64 on the host."""
+    M = int(np.prod(vshape, dtype=int))
+    Mc = corr.shape[0]
+    if vshape[-1:] != (Mc,):
+        raise ValueError(
+            'cannot instantiate a Wiener source with values shape {} and '
+            'correlation matrix shape {}'.format(vshape, corr.shape))
+    try:
+        L = np.linalg.cholesky(corr)
+    except np.linalg.LinAlgError:
+        raise ValueError('the correlation matrix is not positive definite')
+    full = np.kron(np.eye(M // Mc), L)
+    return full[np.tril_indices(M)]
+
+
+# --------------------------------------------------------------------------
+# sources
+# --------------------------------------------------------------------------
+
+class source:
+    """Base class of the stochasticity sources (sdepy/infrastructure.py:
+    1291-1385): ``dz(t, dt)`` returns increments shaped
+    ``(t + dt).shape + vshape + (paths,)``."""
+
+    def __init__(self, *, paths=1, vshape=(), dtype=None, rng=None):
+        self.paths, self.dtype = paths, dtype
+        self.vshape = _shape_setup(vshape)
+        if isinstance(rng, (int, np.integer, list, np.ndarray)):
+            raise TypeError(
+                '`rng` must be an instance of a random number generator, '
+                'not {}.'.format(type(rng)))
+        self._rng = default_rng if rng is None else rng
+
+    def __call__(self, t, dt=None):
+        dt = 0 if dt is None else dt
+        t, dt = np.asarray(t), np.asarray(dt)
+        return t + dt + np.nan
+
+    @property
+    def rng(self):
+        return self._rng
+
+    @property
+    def size(self):
+        return 0
+
+    @property
+    def t(self):
+        return np.array((), dtype=float)
+
+
+class _philox_source(source):
+    """A source drawing from the GPU Philox generator.  ``key`` is drawn once
+    from ``rng``; ``epoch`` counts the calls and simulations made so far, so
+    that every call yields fresh, reproducible draws."""
+
+    def __init__(self, *, paths=1, vshape=(), dtype=None, rng=None):
+        super().__init__(paths=paths, vshape=vshape, dtype=dtype, rng=rng)
+        if not isinstance(paths, numbers.Integral) or not 1 <= paths < 2 ** 32:
+            raise ValueError('paths should be an integer in [1, 2**32), not '
+                             '{}'.format(paths))
+        self.key = philox_key(self._rng)
+        self.epoch = 0
+
+    def next_epoch(self):
+        e = self.epoch
+        self.epoch = (self.epoch + 1) & 0xFFFFFFFF
+        return e
+
+
+def _gpu_dtype(dtype):
+    """(library dtype code, compute numpy dtype, output numpy dtype)."""
+    dt = np.dtype(float if dtype is None else dtype)
+    if dt == np.float64:
+        return _lib.F64, np.float64, dt
+    if dt in (np.float32, np.float16):
+        return _lib.F32, np.float32, dt
+    raise NotImplementedError(
+        'the GPU path computes in float32 or float64, not {}'.format(dt))
+
+
+class wiener_source(_philox_source):
+    """dw, standard Wiener process increments (sdepy/infrastructure.py:
+    1392-1560), drawn on the GPU: ``dw(t, dt) = L z sqrt(|dt|)`` with z
+    Philox normals and L the FP64 Cholesky factor of ``corr`` (or of the
+    matrix built from ``rho``), evaluated at ``t + dt/2`` when callable."""
+
+    def __init__(self, *, paths=1, vshape=(), dtype=None, rng=None,
+                 corr=None, rho=None):
+        super().__init__(paths=paths, vshape=vshape, dtype=dtype, rng=rng)
+        self.corr = corr = _corr_setup(corr, rho)
+        if corr is not None:
+            if self.vshape == ():
+                raise ValueError(
+                    'if vshape is (), no correlations apply, but corr={}, '
+                    'rho={} were given'.format(corr, rho))
+            if not callable(corr) and corr.shape[0] != self.vshape[-1]:
+                raise ValueError(
+                    'cannot instantiate a Wiener source with values shape {} '
+                    'and correlation matrix shape {}'
+                    .format(self.vshape, corr.shape))
+
+    def step_table(self, tt, dtt):
+        """Rows [ds, sqrt|ds|, L...] (FP64) for slots at times tt, steps dtt."""
+        M = int(np.prod(self.vshape, dtype=int))
+        has_corr = self.corr is not None
+        ncoef = 2 + (M * (M + 1) // 2 if has_corr else 0)
+        tab = np.zeros((tt.size, ncoef))
+        tab[:, 0] = dtt
+        tab[:, 1] = np.sqrt(np.abs(dtt))
+        if has_corr:
+            if callable(self.corr):
+                for i in range(tt.size):
+                    tab[i, 2:] = cholesky_packed(
+                        _corr_at(self.corr, tt[i] + dtt[i] / 2), self.vshape)
+            else:
+                tab[:, 2:] = cholesky_packed(_corr_at(self.corr, 0.),
+                                             self.vshape)
+        return tab, has_corr
+
+    def __call__(self, t, dt):
+        t, dt = np.broadcast_arrays(np.asarray(t, dtype=float),
+                                    np.asarray(dt, dtype=float))
+        tshape = dt.shape
+        code, cdt, odt = _gpu_dtype(self.dtype)
+        M = int(np.prod(self.vshape, dtype=int))
+        n = int(np.prod(tshape, dtype=int))
+        tab, has_corr = self.step_table(t.reshape(-1), dt.reshape(-1))
+        torch = _lib.require_gpu()
+        coef = _lib.to_device(tab.astype(cdt))
+        out = _lib.empty((n, M, self.paths), cdt)
+        _lib.check(_lib.lib().sdegpu_wiener_fill(
+            code, _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
+            _lib.ptr(coef), tab.shape[1], int(has_corr), _lib.ptr(out),
+            _lib.stream()))
+        res = _lib.to_host(out).astype(odt, copy=False)
+        del torch
+        return res.reshape(tshape + self.vshape + (self.paths,))
+
+
+class poisson_source(_philox_source):
+    """dn, Poisson process increments (sdepy/infrastructure.py:1566-1633):
+    ``dn(t, dt) = sign(dt) Poisson(lam(t + dt/2) |dt|)``, drawn on the GPU by
+    inversion of Philox uniforms."""
+
+    def __init__(self, *, paths=1, vshape=(), dtype=int, rng=None, lam=1.):
+        super().__init__(paths=paths, vshape=vshape, dtype=dtype, rng=rng)
+        self.lam = lam
+        if not callable(lam):
+            try:
+                np.broadcast_to(np.asarray(lam), self.vshape + (paths,))
+            except ValueError:
+                raise ValueError(
+                    'cannot broadcast lambda parameter shaped {} to requested '
+                    'poisson source shape = vshape + (paths,) = {}'
+                    .format(np.shape(lam), self.vshape + (paths,)))
+
+    def lam_table(self, tt, dtt):
+        """[slots][M][2] = (lam |dt|, exp(-lam |dt|)) in FP64."""
+        M = int(np.prod(self.vshape, dtype=int))
+        tab = np.zeros((tt.size, M, 2))
+        for i in range(tt.size):
+            lam = self.lam(tt[i] + dtt[i] / 2) if callable(self.lam) else self.lam
+            lam = per_component(lam, self.vshape, 'lam')
+            if (lam < 0).any():
+                raise ValueError('negative Poisson intensity')
+            ld = lam * abs(dtt[i])
+            if (ld > 700).any():
+                raise NotImplementedError(
+                    'Poisson means above 700 per step are not supported by '
+                    'the GPU sampler')
+            tab[i, :, 0] = ld
+            tab[i, :, 1] = np.exp(-ld)
+        return tab
+
+    def __call__(self, t, dt):
+        t, dt = np.broadcast_arrays(np.asarray(t, dtype=float),
+                                    np.asarray(dt, dtype=float))
+        tshape = dt.shape
+        M = int(np.prod(self.vshape, dtype=int))
+        n = int(np.prod(tshape, dtype=int))
+        pois = self.lam_table(t.reshape(-1), dt.reshape(-1))
+        sign = np.sign(dt.reshape(-1)).astype(np.int32)
+        _lib.require_gpu()
+        dpois, dsign = _lib.to_device(pois), _lib.to_device(sign)
+        out = _lib.empty((n, M, self.paths), np.int64)
+        _lib.check(_lib.lib().sdegpu_poisson_fill(
+            _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
+            _lib.ptr(dpois), _lib.ptr(dsign), _lib.ptr(out), _lib.stream()))
+        res = _lib.to_host(out).astype(self.dtype, copy=False)
+        return res.reshape(tshape + self.vshape + (self.paths,))
+
+
+def per_component(value, vshape, name):
+    """A parameter broadcast to the flattened vshape, rejecting per-path
+    arrays (a trailing paths axis of size > 1)."""
+    a = np.asarray(value, dtype=float)
+    try:
+        b = np.broadcast_to(a, tuple(vshape) + (1,))
+    except ValueError:
+        raise NotImplementedError(
+            'parameter {} shaped {} varies across paths or does not '
+            'broadcast to values shaped {}: the GPU integrator takes '
+            'parameters that are constant across paths'
+            .format(name, a.shape, tuple(vshape)))
+    return np.ascontiguousarray(b[..., 0]).reshape(-1)
+
+
+# random variables for compound Poisson sources --------------------------
+
+class _norm_rv:
+    """Normal distribution of mean a and standard deviation b
+    (sdepy/infrastructure.py:1653-1664, :1715-1720); the GPU compound
+    Poisson sampler draws it in-kernel."""
+
+    def __init__(self, a=0., b=1.):
+        self.a, self.b = a, b
+
+    def _ab(self, t=None):
+        a = self.a(t) if callable(self.a) else self.a
+        b = self.b(t) if callable(self.b) else self.b
+        return a, b
+
+    def rvs(self, size, random_state=None):
+        a, b = self._ab()
+        return scipy.stats.norm(loc=a, scale=b).rvs(size=size,
+                                                    random_state=random_state)
+
+    def mean(self):
+        return np.asarray(self.a) + 0.
+
+    def std(self):
+        return np.asarray(self.b) + 0.
+
+    def var(self):
+        return np.asarray(self.b) ** 2 + 0.
+
+    def exp_mean(self):
+        a, b = np.asarray(self.a), np.asarray(self.b)
+        return np.exp(a + b * b / 2) + 0
+
+
+def norm_rv(a=0., b=1.):
+    """Normal jump-size distribution with mean a and standard deviation b,
+    either possibly callable in t (sdepy/infrastructure.py:1653-1664)."""
+    return _norm_rv(a, b)
+
+
+class cpoisson_source(_philox_source):
+    """dj, compound Poisson increments (sdepy/infrastructure.py:1846-2040).
+
+    With ``y = norm_rv(a, b)`` (the Merton case) both the Poisson counts and
+    the jump sizes are drawn on the GPU.  Any other ``y`` exposing
+    ``rvs(size, random_state)`` keeps sdepy's semantics: counts from the GPU,
+    jump sizes from ``y.rvs`` with this source's ``rng``.  ``dn_value`` holds
+    the signed counts of the last call.
+    """
+
+    def __init__(self, *, paths=1, vshape=(), dtype=None, rng=None,
+                 dn=None, ptype=int, lam=1., y=None):
+        super().__init__(paths=paths, vshape=vshape, dtype=dtype, rng=rng)
+        if dn is None:
+            self.dn = poisson_source(paths=paths, vshape=vshape, dtype=ptype,
+                                     rng=rng, lam=lam)
+            # the counts share this source's key so the Euler kernel, which
+            # draws counts and sizes from one key, reproduces this source
+            self.dn.key = self.key
+        elif isinstance(dn, type):
+            self.dn = dn(paths=paths, vshape=vshape, dtype=ptype, rng=rng,
+                         lam=lam)
+        else:
+            self.dn = dn
+        self.ptype = getattr(self.dn, 'dtype', ptype)
+        self.lam = getattr(self.dn, 'lam', lam)
+        self.y = _uniform_default() if y is None else y
+        self.dn_value = None
+        self.y_value = None
+
+    @property
+    def gpu_jumps(self):
+        """True when counts and sizes can be drawn inside the Euler kernel."""
+        return isinstance(self.y, _norm_rv) and isinstance(self.dn, poisson_source) \
+            and self.dn.key == self.key
+
+    def ab_table(self, tt, dtt):
+        M = int(np.prod(self.vshape, dtype=int))
+        tab = np.zeros((tt.size, M, 2))
+        for i in range(tt.size):
+            a, b = self.y._ab(tt[i] + dtt[i] / 2)
+            tab[i, :, 0] = per_component(a, self.vshape, 'a')
+            tab[i, :, 1] = per_component(b, self.vshape, 'b')
+        return tab
+
+    def __call__(self, t, dt):
+        t, dt = np.broadcast_arrays(np.asarray(t, dtype=float),
+                                    np.asarray(dt, dtype=float))
+        tshape = dt.shape
+        shape = self.vshape + (self.paths,)
+        if self.gpu_jumps:
+            code, cdt, odt = _gpu_dtype(self.dtype)
+            M = int(np.prod(self.vshape, dtype=int))
+            n = int(np.prod(tshape, dtype=int))
+            tf, dtf = t.reshape(-1), dt.reshape(-1)
+            pois = self.dn.lam_table(tf, dtf)
+            ab = self.ab_table(tf, dtf).astype(cdt)
+            sign = np.sign(dtf).astype(np.int32)
+            _lib.require_gpu()
+            dpois, dab, dsign = (_lib.to_device(pois), _lib.to_device(ab),
+                                 _lib.to_device(sign))
+            out = _lib.empty((n, M, self.paths), cdt)
+            dn = _lib.empty((n, M, self.paths), np.int64)
+            epoch = self.next_epoch()
+            self.dn.epoch = self.epoch
+            _lib.check(_lib.lib().sdegpu_cpoisson_fill(
+                code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
+                _lib.ptr(dpois), _lib.ptr(dsign), _lib.ptr(dab),
+                _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
+            self.dn_value = _lib.to_host(dn).astype(self.ptype).reshape(
+                tshape + shape)
+            self.y_value = None
+            return _lib.to_host(out).astype(odt, copy=False).reshape(
+                tshape + shape)
+        # generic y: counts from dn, sizes from y.rvs (sdepy's grouping by
+        # count, infrastructure.py:2026-2037)
+        sign_dt = np.sign(dt).astype(int)
+        dn = np.asarray(self.dn(t, dt))
+        dz = np.zeros(dt.shape + shape, dtype=self.dtype)
+        y_value = []
+        for i in np.ndindex(dt.shape):
+            dnp = sign_dt[i] * dn[i]
+            rv = self.y(t[i] + dt[i] / 2) if (callable(self.y) and
+                                             not hasattr(self.y, 'rvs')) else self.y
+            for j in range(1, int(dnp.max(initial=0)) + 1):
+                index = dnp == j
+                if index.any():
+                    ys = rv.rvs(size=(index.sum(), j), random_state=self.rng)
+                    dz[i][index] = sign_dt[i] * ys.sum(axis=-1)
+                    y_value.append(ys)
+        self.dn_value = dn
+        self.y_value = y_value
+        return dz
+
+
+class _uniform_rv:
+    def __init__(self, a=0., b=1.):
+        self.a, self.b = a, b
+
+    def rvs(self, size, random_state=None):
+        return scipy.stats.uniform(loc=self.a, scale=self.b - self.a).rvs(
+            size=size, random_state=random_state)
+
+    def exp_mean(self):
+        a, b = np.asarray(self.a), np.asarray(self.b)
+        return (np.exp(b) - np.exp(a)) / (b - a) + 0
+
+
+def _uniform_default():
+    return _uniform_rv(0., 1.)
+
+
+def uniform_rv(a=0., b=1.):
+    """Uniform jump sizes on [a, b] (sdepy/infrastructure.py:1667-1677);
+    sampled on the host through ``rvs``."""
+    return _uniform_rv(a, b)
+
+
+# --------------------------------------------------------------------------
+# process
+# --------------------------------------------------------------------------
+
+class process(np.ndarray):
+    """process(t=0., *, x=None, v=None, c=None, dtype=None)
+
+    Array representation of a process (sdepy/infrastructure.py:272-1209):
+    ``p[i, ..., k]`` is the value of path ``k`` at time ``p.t[i]``; shape
+    ``(N,) + vshape + (paths,)``.  Ufuncs on processes sharing a timeline
+    (or constant ones) return processes; ndarray methods return arrays.
+
+    ``pmean``, ``pvar``, ``psum`` and ``pstd`` reduce the paths axis on the
+    GPU with FP64 accumulation.
+    """
+
+    __array_priority__ = 1.0
+    interp_kind = 'linear'
+
+    def __new__(cls, t=0., *, x=None, v=None, c=None, dtype=None):
+        t = np.asarray(t)
+        if t.ndim > 1 or t.size == 0:
+            raise ValueError('the shape of a process timeline should be () '
+                             'or (n,), not {}'.format(t.shape))
+        if t.ndim == 0:
+            t = t.reshape(1)
+        if sum(z is not None for z in (x, v, c)) != 1:
+            raise ValueError('when creating a process instance, one and only '
+                             'one of x or v or c should be provided')
+        if x is not None:
+            x = np.asarray(x, dtype=dtype)
+        elif v is not None:
+            x = np.asarray(v, dtype=dtype)[..., np.newaxis]
+        else:
+            c = np.asarray(c, dtype=dtype)
+            if t.size == 1:
+                x = c[np.newaxis, ..., np.newaxis]
+            else:
+                x = np.empty(shape=t.shape + c.shape + (1,), dtype=c.dtype)
+                x[...] = c[np.newaxis, ..., np.newaxis]
+        if t.shape != x.shape[:1]:
+            raise ValueError('process could not be created from timeline t '
+                             'shaped {} and body shaped {}'
+                             .format(t.shape, x.shape))
+        obj = x.view(cls)
+        obj.t = t
+        return obj
+
+    def __array_finalize__(self, obj):
+        if isinstance(obj, process) and getattr(obj, 't', None) is not None \
+                and obj.t.shape == self.shape[:1]:
+            self.t = obj.t
+        else:
+            self.t = None
+
+    def _is_compatible(self, other):
+        t_ok = np.array_equal(self.t, other.t) or \
+            self.t.size == 1 or other.t.size == 1
+        s1, s2 = self.shape[1:], other.shape[1:]
+        vp_ok = len(s1) == len(s2) and all(
+            a == b or a == 1 or b == 1 for a, b in zip(s1, s2))
+        return t_ok and vp_ok
+
+    def __array_ufunc__(self, ufunc, method, *inputs, out=None, **kwargs):
+        procs = [a for a in inputs if isinstance(a, process)]
+        if out is not None:
+            procs += [a for a in out if isinstance(a, process)]
+        for a in procs:
+            if a.t is None:
+                raise ValueError(
+                    'cannot operate on a process without a timeline; try '
+                    'using its array view instead (x attribute)')
+        if method == '__call__':
+            for a in procs[1:]:
+                if not procs[0]._is_compatible(a):
+                    raise ValueError(
+                        'processes could not be broadcast together due to '
+                        'incompatible shapes {}, {} and/or timelines'
+                        .format(procs[0].shape, a.shape))
+        args = [a.view(np.ndarray) if isinstance(a, process) else a
+                for a in inputs]
+        if out is not None:
+            kwargs['out'] = tuple(o.view(np.ndarray) if isinstance(o, process)
+                                  else o for o in out)
+        res = getattr(ufunc, method)(*args, **kwargs)
+        if method != '__call__' or res is NotImplemented:
+            return res
+        t = procs[0].t
+        for a in procs[1:]:
+            if a.t.size > 1:
+                t = a.t
+                break
+
+        def wrap(r, o=None):
+            if o is not None and isinstance(o, process):
+                return o
+            if isinstance(r, np.ndarray) and r.ndim >= 1 and r.shape[0] == t.size:
+                return process(t=t, x=r)
+            return r
+        if isinstance(res, tuple):
+            outs = out or (None,) * len(res)
+            return tuple(wrap(r, o) for r, o in zip(res, outs))
+        return wrap(res, out[0] if out else None)
+
+    def __array_wrap__(self, out_array, context=None, return_scalar=False):
+        # ndarray methods (sum, mean, ...) return plain arrays, as in sdepy's
+        # no-override commitment (infrastructure.py:507-538)
+        arr = out_array.view(np.ndarray) if isinstance(out_array, np.ndarray) \
+            else out_array
+        if return_scalar and isinstance(arr, np.ndarray) and arr.ndim == 0:
+            return arr[()]
+        return arr
+
+    # interpolation (infrastructure.py:544-647) ---------------------------
+
+    def interp(self, *, kind=None):
+        t, x = self.t, self.view(np.ndarray)
+        kind = self.interp_kind if kind is None else kind
+        if t.size == 1:
+            def f(s):
+                s = np.asarray(s)
+                return np.ones(s.shape + x.shape[1:], dtype=x.dtype) * \
+                    x.reshape(tuple(1 for i in s.shape) + x.shape[1:])
+            return f
+        g = scipy.interpolate.interp1d(t, x, axis=0, kind=kind,
+                                       assume_sorted=True, copy=False,
+                                       bounds_error=False,
+                                       fill_value=(x[0], x[-1]))
+
+        def f(s):
+            return g(s).astype(x.dtype, copy=False)
+        return f
+
+    def __call__(self, s, ds=None, *, kind=None):
+        s = np.asarray(s)
+        f = self.interp(kind=kind)
+        if ds is None:
+            return f(s)
+        ds = np.asarray(ds)
+        return f(s + ds) - f(s)
+
+    def rebase(self, t, *, kind=None):
+        t = np.asarray(t)
+        if t.ndim == 0:
+            t = t.reshape(1)
+        return process(t, x=self(t, kind=kind))
+
+    # indexing (infrastructure.py:653-700) -------------------------------
+
+    def __getitem__(self, key):
+        x = self.view(np.ndarray)
+        cls = type(self)
+        colon = (slice(None),)
+        if not isinstance(key, (tuple, str)):
+            return x[key]
+        if isinstance(key, str):
+            key = (key,)
+        elif len(key) == 0:
+            return x[()]
+        elif not isinstance(key[0], str):
+            return x[key]
+        a, key = key[0], key[1:]
+        if len(key) == 1 and isinstance(key[0], tuple):
+            key = key[0]
+        if a not in ('v', 't', 'p'):
+            raise IndexError('process indexing error - unsupported indexing '
+                             'mode ' + repr(a))
+        if a == 'v':
+            return cls(self.t, x=x[colon + key + colon])
+        assert len(key) <= 1
+        if len(key) == 1 and isinstance(key[0], numbers.Integral):
+            i = key[0]
+            key = (slice(i, None, None),) if i == -1 else (slice(i, i + 1, None),)
+        if a == 't':
+            return cls(self.t[key], x=x[key])
+        return cls(self.t, x=x[(Ellipsis,) + key])
+
+    # properties (infrastructure.py:706-767) ------------------------------
+
+    @property
+    def x(self):
+        return self.view(np.ndarray)
+
+    @property
+    def paths(self):
+        return self.shape[-1]
+
+    @property
+    def vshape(self):
+        return self.shape[1:-1]
+
+    @property
+    def tx(self):
+        t = self.t
+        return t.reshape(t.shape + tuple(1 for i in self.shape[1:]))
+
+    @property
+    def dt(self):
+        return np.diff(self.t)
+
+    @property
+    def dtx(self):
+        dt = self.dt
+        return dt.reshape(dt.shape + (1,) * (len(self.shape) - 1))
+
+    def shapeas(self, vshape_or_process):
+        vshape = (vshape_or_process.vshape
+                  if isinstance(vshape_or_process, process)
+                  else vshape_or_process)
+        cls = type(self)
+        k = self.ndim - 2
+        h = len(vshape)
+        if h > k:
+            newshape = self.shape[:1] + (1,) * (h - k) + self.shape[1:]
+        else:
+            if h < k and set(self.shape[1:k - h + 1]) != {1}:
+                raise ValueError('could not reshape {} process values as {}'
+                                 .format(self.vshape, vshape))
+            newshape = self.shape[:1] + self.shape[k - h + 1:]
+        return cls(self.t, x=self.view(np.ndarray).reshape(newshape))
+
+    def pcopy(self, **args):
+        return type(self)(t=self.t.copy(**args),
+                          x=self.view(np.ndarray).copy(**args))
+
+    def xcopy(self, **args):
+        return type(self)(t=self.t, x=self.view(np.ndarray).copy(**args))
+
+    def tcopy(self, **args):
+        return type(self)(t=self.t.copy(**args), x=self.view(np.ndarray))
+
+    # summaries across paths (infrastructure.py:833-889) ------------------
+
+    def _path_moments(self):
+        """(n, mean, m2) per row from the GPU FP64 reduction."""
+        return path_moments(self.view(np.ndarray))
+
+    def _summary_out(self, val, dtype, out):
+        shape = self.shape[:-1] + (1,)
+        val = val.reshape(shape)
+        if out is not None:
+            out[...] = val
+            return process(t=self.t, x=out)
+        return process(t=self.t, x=val.astype(dtype))
+
+    def pmin(self, out=None):
+        return process(t=self.t, x=self.view(np.ndarray).min(
+            axis=-1, out=out, keepdims=True))
+
+    def pmax(self, out=None):
+        return process(t=self.t, x=self.view(np.ndarray).max(
+            axis=-1, out=out, keepdims=True))
+
+    def psum(self, dtype=None, out=None):
+        """Sum across paths (GPU, FP64 accumulation)."""
+        dtype = self.dtype if dtype is None else dtype
+        n, mean, _ = self._path_moments()
+        return self._summary_out(mean * n, dtype, out)
+
+    def pmean(self, dtype=None, out=None):
+        """Mean across paths (GPU, FP64 accumulation)."""
+        dtype = self.dtype if dtype is None else dtype
+        _, mean, _ = self._path_moments()
+        return self._summary_out(mean, dtype, out)
+
+    def pvar(self, dtype=None, out=None, ddof=0):
+        """Variance across paths (GPU, FP64 accumulation)."""
+        dtype = self.dtype if dtype is None else dtype
+        n, _, m2 = self._path_moments()
+        return self._summary_out(m2 / (n - ddof), dtype, out)
+
+    def pstd(self, dtype=None, out=None, ddof=0):
+        """Standard deviation across paths (GPU, FP64 accumulation)."""
+        dtype = self.dtype if dtype is None else dtype
+        n, _, m2 = self._path_moments()
+        return self._summary_out(np.sqrt(m2 / (n - ddof)), dtype, out)
+
+    # summaries across values and along time (host NumPy) -----------------
+
+    def _vaxes(self):
+        return tuple(range(1, len(self.vshape) + 1))
+
+    def vmin(self, out=None):
+        return process(t=self.t, x=self.view(np.ndarray).min(axis=self._vaxes(), out=out))
+
+    def vmax(self, out=None):
+        return process(t=self.t, x=self.view(np.ndarray).max(axis=self._vaxes(), out=out))
+
+    def vsum(self, dtype=None, out=None):
+        return process(t=self.t, x=self.view(np.ndarray).sum(
+            axis=self._vaxes(), dtype=dtype, out=out))
+
+    def vmean(self, dtype=None, out=None):
+        return process(t=self.t, x=self.view(np.ndarray).mean(
+            axis=self._vaxes(), dtype=dtype, out=out))
+
+    def vvar(self, dtype=None, out=None, ddof=0):
+        return process(t=self.t, x=self.view(np.ndarray).var(
+            axis=self._vaxes(), dtype=dtype, out=out, ddof=ddof))
+
+    def vstd(self, dtype=None, out=None, ddof=0):
+        return process(t=self.t, x=self.view(np.ndarray).std(
+            axis=self._vaxes(), dtype=dtype, out=out, ddof=ddof))
+
+    def tmin(self, out=None):
+        return process(t=self.t[:1], x=self.view(np.ndarray).min(
+            axis=0, out=out, keepdims=True))
+
+    def tmax(self, out=None):
+        return process(t=self.t[:1], x=self.view(np.ndarray).max(
+            axis=0, out=out, keepdims=True))
+
+    def tsum(self, dtype=None, out=None):
+        dtype = self.dtype if dtype is None else dtype
+        return process(t=self.t[:1], x=self.view(np.ndarray).sum(
+            axis=0, dtype=dtype, out=out, keepdims=True))
+
+    def tcumsum(self, dtype=None, out=None):
+        dtype = self.dtype if dtype is None else dtype
+        return process(t=self.t, x=self.view(np.ndarray).cumsum(
+            axis=0, dtype=dtype, out=out))
+
+    def tmean(self, dtype=None, out=None):
+        dtype = self.dtype if dtype is None else dtype
+        return process(t=self.t[:1], x=self.view(np.ndarray).mean(
+            axis=0, dtype=dtype, out=out, keepdims=True))
+
+    def tvar(self, dtype=None, out=None, ddof=0):
+        dtype = self.dtype if dtype is None else dtype
+        return process(t=self.t[:1], x=self.view(np.ndarray).var(
+            axis=0, dtype=dtype, out=out, ddof=ddof, keepdims=True))
+
+    def tstd(self, dtype=None, out=None, ddof=0):
+        dtype = self.dtype if dtype is None else dtype
+        return process(t=self.t[:1], x=self.view(np.ndarray).std(
+            axis=0, dtype=dtype, out=out, ddof=ddof, keepdims=True))
+
+
+def path_moments(x, device_tensor=None):
+    """FP64 (n, mean, m2) over the last axis of ``x``, reduced on the GPU
+    (``sdegpu_path_moments``).  ``device_tensor``, when given, is a device
+    copy of ``x`` and avoids the upload."""
+    x = np.asarray(x)
+    if x.dtype.kind != 'f':
+        raise NotImplementedError(
+            'the GPU path reductions take floating point processes, not {}'
+            .format(x.dtype))
+    paths = x.shape[-1]
+    rows = int(np.prod(x.shape[:-1], dtype=int))
+    code, cdt, _ = _gpu_dtype(x.dtype)
+    _lib.require_gpu()
+    dx = device_tensor if device_tensor is not None else \
+        _lib.to_device(x.astype(cdt, copy=False).reshape(rows, paths))
+    mean = _lib.empty((rows,), np.float64)
+    m2 = _lib.empty((rows,), np.float64)
+    nbytes = _lib.lib().sdegpu_moments_workspace(rows, paths)
+    ws = _lib.workspace(nbytes)
+    _lib.check(_lib.lib().sdegpu_path_moments(
+        _lib.ptr(dx), code, rows, paths, _lib.ptr(mean), _lib.ptr(m2),
+        _lib.ptr(ws), nbytes, _lib.stream()))
+    return (float(paths), _lib.to_host(mean).reshape(x.shape[:-1]),
+            _lib.to_host(m2).reshape(x.shape[:-1]))

@@ -1,0 +1,729 @@
+"""GPU-backed preset processes with sdepy's API.
+
+Each class here is a drop-in for the preset of the same name in
+sdepy/integration.py: same constructor keywords and defaults, same timeline
+semantics (``steps``, ``i0``, output timeline kept by reference), same
+``info`` items, and the same return type (a ``process``, or a tuple of them
+for ``full_heston_process``).  Where sdepy runs ``paths_generator.
+_generate_paths`` (integration.py:392-474) as a Python loop over NumPy arrays,
+``__call__`` builds the per-step coefficient tables on the host and makes one
+``sdegpu_euler`` call per integration pass.
+
+Host-side work, as in sdepy: argument consolidation (SDE.__init__,
+integration.py:1081-1148), timeline validation and step consolidation
+(paths_generator.__call__, :476-584), parameter evaluation at each step
+start (SDE.A, :1216-1225) and at mid-step for correlations and intensities
+(infrastructure.py:1540, :1629).  Device work: everything per path and step.
+
+Sources: the default ``wiener_source`` / ``cpoisson_source(y=norm_rv)``
+draw inside the kernel.  Any other object complying with sdepy's source
+protocol (a ``process``, a user callable with ``paths`` and ``vshape``, a
+compound Poisson source with another jump law) is honoured by calling it on
+the host once per step, in sorted source-id order (SDE.dZ, :1227-1234), and
+replaying the recorded increments in the kernel.
+"""
+import inspect
+
+import numpy as np
+
+from . import _lib
+from .infrastructure import (process, wiener_source, cpoisson_source,
+                             poisson_source, norm_rv, per_component,
+                             _gpu_dtype, _shape_setup, _get_default_rng)
+
+__all__ = ['wiener_process', 'lognorm_process', 'ornstein_uhlenbeck_process',
+           'hull_white_process', 'hull_white_1factor_process',
+           'cox_ingersoll_ross_process', 'full_heston_process',
+           'heston_process', 'jumpdiff_process', 'merton_jumpdiff_process',
+           'kou_jumpdiff_process']
+
+_empty = inspect.Signature.empty
+
+
+# --------------------------------------------------------------------------
+# timeline (paths_generator.pace / __call__, integration.py:189-218, 504-544)
+# --------------------------------------------------------------------------
+
+def _pace(timeline, steps):
+    ttype = timeline.dtype
+    if timeline.size == 1:
+        return timeline
+    if steps is None:
+        return np.array((), dtype=ttype)
+    if np.isscalar(steps):
+        return np.linspace(timeline[0], timeline[-1], steps, dtype=ttype)
+    return np.fromiter(steps, dtype=ttype)
+
+
+def consolidate_timeline(timeline, steps, i0):
+    """Validated float timeline, the timeline as given, and the consolidated
+    integration points ``steps_tt``; raises sdepy's ValueError/IndexError."""
+    tt = np.asarray(timeline)
+    if tt.shape != (tt.size,):
+        raise ValueError(
+            'the integration timeline should be a one-dimensional array, '
+            'not an array of shape {}'.format(tt.shape))
+    if not np.array_equal(tt, np.unique(tt)):
+        raise ValueError(
+            'the integration timeline sholud be an array of strictly '
+            'increasing numbers, but {} was given'.format(tt))
+    try:
+        if not np.isscalar(tt[i0]):
+            raise IndexError()
+    except (IndexError, TypeError):
+        raise IndexError(
+            'i0 should be an integer indexing an element of the integration '
+            'timeline, but {} was given'.format(i0))
+    ttype = float if tt.dtype.kind in 'iu' else tt.dtype
+    tt, tt_asis = tt.astype(ttype), tt
+    tmin, tmax = tt[0], tt[-1]
+    target = _pace(tt, steps)
+    target = target[np.logical_and(target >= tmin, target <= tmax)]
+    steps_tt = np.unique(np.concatenate((target, tt)))
+    return tt, tt_asis, steps_tt
+
+
+def _store_rows(steps_pass, tt_pass, row_of):
+    """Output row reached after each step of a pass, or -1
+    (paths_generator._store_if_reached, integration.py:377-390).  A pass runs
+    forward (ascending times) or backward (descending)."""
+    sign = 1. if tt_pass[0] <= tt_pass[-1] else -1.
+    tp, sp = sign * tt_pass, sign * steps_pass[1:]
+    p = np.searchsorted(tp, sp)
+    pc = np.minimum(p, tp.size - 1)
+    hit = (p < tp.size) & (tp[pc] == sp)
+    rows = np.full(sp.size, -1, dtype=np.int32)
+    rows[hit] = np.asarray(row_of(p[hit]), dtype=np.int32)
+    return rows
+
+
+def _check_source(src, paths, vshape):
+    """sdepy's _check_source (infrastructure.py:209-232)."""
+    if callable(src) and hasattr(src, 'paths') and hasattr(src, 'vshape'):
+        paths_ok = src.paths == paths
+        try:
+            np.broadcast_to(np.empty(src.vshape), vshape)
+            vshape_ok = True
+        except ValueError:
+            vshape_ok = False
+        if not paths_ok or not vshape_ok:
+            raise ValueError(
+                'invalid stochasticity source: expecting soruce paths={} and '
+                'vshape broadcastable to {}, but paths={}, vshape={} were '
+                'found'.format(paths, vshape, src.paths, src.vshape))
+        return
+    raise ValueError(
+        "stochasticity source of type '{}', not compliant with the source "
+        'protocol (should be callable with properly defined paths and '
+        'vshape attributes)'.format(type(src).__name__))
+
+
+def _source_setup(dz, source_type, paths, vshape, **args):
+    """sdepy's _source_setup (infrastructure.py:235-243)."""
+    if dz is None:
+        return source_type(paths=paths, vshape=vshape, **args)
+    if inspect.isclass(dz):
+        return dz(paths=paths, vshape=vshape, **args)
+    _check_source(dz, paths, vshape)
+    return dz
+
+
+def _dt_source(paths, vshape):
+    """SDE.source_dt (integration.py:1294-1309): dt(s, ds) = ds."""
+    def dt(s, ds):
+        return ds
+    dt.paths, dt.vshape = paths, vshape
+    return dt
+
+
+# --------------------------------------------------------------------------
+# common machinery
+# --------------------------------------------------------------------------
+
+class _gpu_sde:
+    """Keyword handling and one-call-per-pass integration shared by the
+    presets.  Subclasses declare the model code, their parameters and how the
+    working state maps to the stored values."""
+
+    model = None
+    log = False
+    q = 1
+    sources = ('dt', 'dw')
+    init_defaults = {'x0': 1.}
+    sde_defaults = {'mu': 0., 'sigma': 1.}
+    more_defaults = {}
+
+    def __init__(self, *, paths=1, vshape=(), dtype=None, rng=None,
+                 steps=None, i0=0, info=None, getinfo=True, method='euler',
+                 **args):
+        if method != 'euler':
+            raise ValueError(
+                "unrecognized integration method {}: use 'euler'"
+                .format(method))
+        self.method = method
+        src_defaults = {'dw': None, 'corr': None, 'rho': None}
+        if 'dj' in self.sources:
+            src_defaults.update(self.dj_defaults)
+        expected = {**self.init_defaults, **self.sde_defaults,
+                    **self.more_defaults, **src_defaults}
+        unexpected = set(args) - set(expected)
+        if unexpected:
+            raise TypeError('unexpected keyword(s): {}'.format(unexpected))
+        self._args = {**expected, **args}
+        self.paths = paths
+        if not isinstance(paths, (int, np.integer)) or paths < 1:
+            raise ValueError('paths should be a positive integer, not {}'
+                             .format(paths))
+        self.dtype = dtype
+        self.steps, self.i0 = steps, i0
+        self.info = {} if info is None else info
+        self.getinfo = getinfo
+        self.vshape, self.xshape, self.wshape = self.shapes(_shape_setup(vshape))
+        self._rng_asis = rng
+        self._rng = _get_default_rng() if rng is None else rng
+        for k in self.init_defaults:
+            self._args[k] = np.asarray(self._args[k])
+        a = self._args
+        self.sources = {}
+        self.sources['dw'] = _source_setup(
+            a['dw'], wiener_source, paths=paths, vshape=self.wshape,
+            dtype=dtype, rng=rng, corr=a['corr'], rho=a['rho'])
+        if 'dj' in type(self).sources:
+            self.sources['dj'] = self._dj_setup()
+        self.sources['dt'] = _dt_source(paths, self.wshape)
+
+    # subclass hooks -------------------------------------------------------
+
+    def shapes(self, vshape):
+        return vshape, vshape, vshape
+
+    @property
+    def args(self):
+        return self._args
+
+    @property
+    def rng(self):
+        return self._rng
+
+    def _nvars(self):
+        return int(np.prod(self.wshape, dtype=int))
+
+    def _init_state(self, P, cdt):
+        """Working state after SDE.init (and the log of SDE.begin for log
+        SDEs, integration.py:1154-1167), shaped (M, P)."""
+        x0 = np.asarray(self._args['x0'])
+        st = np.broadcast_to(x0, self.wshape + (P,)).astype(cdt)
+        if self.log:
+            st = np.log(st)
+        return st.reshape(self._nvars(), P)
+
+    def _init_target(self):
+        """Shape (without the paths axis) the init parameters broadcast to."""
+        return self.wshape
+
+    def _x0_state(self, cdt):
+        """Initial working state: (M,) when it is the same on every path,
+        else (M, paths)."""
+        const = True
+        for name in self.init_defaults:
+            try:
+                np.broadcast_to(np.asarray(self._args[name]),
+                                self._init_target() + (1,))
+            except ValueError:
+                const = False
+        if const:
+            return np.ascontiguousarray(self._init_state(1, cdt)[:, 0])
+        return np.ascontiguousarray(self._init_state(self.paths, cdt))
+
+    def _param_rows(self, t0s):
+        """Model parameter block of each step row (FP64), evaluated at the
+        step start times t0s."""
+        raise NotImplementedError
+
+    def _output_map(self):
+        """(nout, out_mode, out_group, exp_mask)."""
+        M = self._nvars()
+        return M, _lib.OUT_IDENTITY, 1, ((1 << M) - 1) if self.log else 0
+
+    def _result(self, tt, xx):
+        return process(t=tt, x=xx)
+
+    # per-parameter evaluation -------------------------------------------
+
+    def _eval(self, name, t0s, shape):
+        """Values of parameter `name` at each step start: (nsteps, prod(shape))."""
+        z = self._args[name]
+        n = int(np.prod(shape, dtype=int))
+        if callable(z):
+            return np.stack([per_component(z(t), shape, name) for t in t0s]) \
+                if len(t0s) else np.zeros((0, n))
+        v = per_component(z, shape, name)
+        return np.broadcast_to(v, (len(t0s), n))
+
+    # integration ----------------------------------------------------------
+
+    def __call__(self, timeline):
+        return self.prepare(timeline).run()
+
+    def prepare(self, timeline):
+        """Validate the timeline, build the step tables, upload them and
+        allocate the device output; returns a ``PreparedRun`` whose
+        ``launch()`` issues the kernels and ``result()`` returns the
+        process(es).  ``__call__`` is ``prepare(timeline).run()``."""
+        tt, tt_asis, steps_tt = consolidate_timeline(timeline, self.steps,
+                                                     self.i0)
+        i0 = self.i0
+        code, cdt, odt = _gpu_dtype(self.dtype)
+        _lib.require_gpu()
+        nout, out_mode, out_group, exp_mask = self._output_map()
+        xx_dev = _lib.empty((tt.size, nout, self.paths), cdt)
+        if self.getinfo:
+            self.info.update(t0=tt[i0], tmin=tt[0], tmax=tt[-1],
+                             computed_steps=0, stored_steps=0)
+        i0n = i0 % tt.size
+        j0 = int(np.nonzero(steps_tt == tt[i0])[0][0])
+        passes = []
+        if i0 != 0:
+            passes.append((steps_tt[j0::-1], tt[i0::-1],
+                           lambda j: i0n - j, True))
+        if i0 == 0 or i0 not in (-1, tt.size - 1):
+            passes.append((steps_tt[j0:], tt[i0:], lambda j: i0n + j, False))
+        x0 = self._x0_state(cdt)
+        prepared = [self._prepare_pass(sp, tp, row_of, reverse, k == 0, x0,
+                                       code, cdt, xx_dev,
+                                       (nout, out_mode, out_group, exp_mask))
+                    for k, (sp, tp, row_of, reverse) in enumerate(passes)]
+        return PreparedRun(self, prepared, xx_dev, tt, tt_asis, odt)
+
+    def _prepare_pass(self, steps_pass, tt_pass, row_of, reverse, first, x0,
+                      code, cdt, xx_dev, outmap):
+        nout, out_mode, out_group, exp_mask = outmap
+        M = self._nvars()
+        P = self.paths
+        t0s = steps_pass[:-1]
+        ds = np.diff(steps_pass)
+        nsteps = ds.size
+        rows = _store_rows(steps_pass, tt_pass, row_of)
+
+        prob = _lib.EulerProblem()
+        prob.model, prob.dtype, prob.nvars = self.model, code, M
+        prob.nout, prob.out_mode, prob.out_group = nout, out_mode, out_group
+        prob.exp_mask = exp_mask
+        prob.paths, prob.nsteps = P, nsteps
+        prob.row0 = int(row_of(0)) if first else -1
+        prob.stream_tag = _lib.STREAM_BACKWARD if reverse else 0
+        keep = []   # device buffers alive until the call is done
+
+        # Wiener source: in-kernel Philox draws, or recorded increments
+        dw = self.sources['dw']
+        host_sources = {}
+        if isinstance(dw, wiener_source):
+            wtab, has_corr = dw.step_table(t0s, ds)
+            prob.key_w[0], prob.key_w[1] = dw.key
+            prob.epoch_w = dw.next_epoch()
+        else:
+            wtab = np.stack((ds, np.sqrt(np.abs(ds))), axis=1) if nsteps \
+                else np.zeros((0, 2))
+            has_corr = False
+            host_sources['dw'] = dw
+        dj = self.sources.get('dj')
+        if dj is not None:
+            if isinstance(dj, cpoisson_source) and dj.gpu_jumps:
+                prob.key_j[0], prob.key_j[1] = dj.key
+                prob.epoch_j = dj.next_epoch()
+                dj.dn.epoch = dj.epoch
+                pois = dj.dn.lam_table(t0s, ds)
+                ab = dj.ab_table(t0s, ds)
+            else:
+                host_sources['dj'] = dj
+                pois = ab = None
+        self._host_dj = 'dj' in host_sources
+        prob.has_corr = int(has_corr)
+        params = self._param_rows(t0s)
+        if dj is not None:
+            extra = np.zeros((nsteps, 2 * M)) if ab is None else \
+                np.concatenate((ab[:, :, 0], ab[:, :, 1]), axis=1)
+            params = np.concatenate((params, extra), axis=1)
+        tab = np.concatenate((wtab, params), axis=1) if nsteps else \
+            np.zeros((0, wtab.shape[1] + params.shape[1]))
+        prob.ncoef = tab.shape[1]
+        if nsteps:
+            d_tab = _lib.to_device(tab.astype(cdt))
+            d_rows = _lib.to_device(rows)
+            keep += [d_tab, d_rows]
+            prob.coef, prob.store_row = d_tab.data_ptr(), d_rows.data_ptr()
+        if dj is not None and pois is not None and nsteps:
+            d_pois = _lib.to_device(pois)
+            keep.append(d_pois)
+            prob.pois = d_pois.data_ptr()
+
+        # host sources, drawn in sorted id order per step (SDE.dZ)
+        recorded = {k: [] for k in host_sources}
+        dn_host = []
+        for s, d in zip(t0s, ds):
+            for sid in sorted(host_sources):
+                z = np.asarray(host_sources[sid](s, d), dtype=float)
+                recorded[sid].append(np.broadcast_to(
+                    z, self.wshape + (P,)).reshape(M, P))
+                if sid == 'dj':
+                    dnv = getattr(host_sources[sid], 'dn_value', None)
+                    if dnv is not None:
+                        dn_host.append(np.broadcast_to(
+                            np.asarray(dnv), self.wshape + (P,)).reshape(M, P))
+        for sid, arrs in recorded.items():
+            if not nsteps:
+                continue
+            d = _lib.to_device(np.stack(arrs).astype(cdt))
+            keep.append(d)
+            setattr(prob, 'replay_' + sid, d.data_ptr())
+
+        if x0.shape == (M,):
+            for v in range(M):
+                prob.x0[v] = float(x0[v])
+        else:
+            d_x0 = _lib.to_device(np.ascontiguousarray(x0, dtype=cdt))
+            keep.append(d_x0)
+            prob.x0_paths = d_x0.data_ptr()
+        prob.out = xx_dev.data_ptr()
+
+        info_paths = info_rows = None
+        if self.getinfo and self.model in (_lib.MODEL_HESTON, _lib.MODEL_JUMPDIFF):
+            info_paths = _lib.zeros((M, P), np.int32)
+            prob.info_paths = info_paths.data_ptr()
+            if self.model == _lib.MODEL_JUMPDIFF:
+                info_rows = _lib.zeros((xx_dev.shape[0],), np.int64)
+                prob.info_rows = info_rows.data_ptr()
+
+        def post():
+            if self.getinfo:
+                self.info['computed_steps'] += nsteps
+                self.info['stored_steps'] += int((rows >= 0).sum())
+                self._pass_info(info_paths, info_rows, tt_pass, row_of,
+                                dn_host, rows)
+        return _Pass(prob, keep, post, nsteps)
+
+    def _pass_info(self, info_paths, info_rows, tt_pass, row_of, dn_host, rows):
+        pass
+
+
+class _Pass:
+    """One integration pass ready to launch: the problem struct and the
+    device buffers it points to (kept alive with it)."""
+
+    def __init__(self, prob, keep, post, nsteps):
+        self.prob, self.keep, self.post, self.nsteps = prob, keep, post, nsteps
+
+    def launch(self, stream=None):
+        _lib.check(_lib.lib().sdegpu_euler(
+            self.prob, stream if stream is not None else _lib.stream()))
+
+
+class PreparedRun:
+    """A simulation with its tables uploaded and its output allocated on the
+    device (``xx_dev``, shaped (N_t, nout, paths)).  ``launch()`` may be
+    called repeatedly (e.g. to time the kernels); with ``fresh=True`` each
+    launch advances the Philox epochs so that it draws new paths."""
+
+    def __init__(self, sde, passes, xx_dev, tt, tt_asis, odt):
+        self.sde, self.passes, self.xx_dev = sde, passes, xx_dev
+        self.tt, self.tt_asis, self.odt = tt, tt_asis, odt
+
+    def launch(self, fresh=False):
+        for p in self.passes:
+            if fresh:
+                p.prob.epoch_w = (p.prob.epoch_w + 1) & 0xFFFFFFFF
+                p.prob.epoch_j = (p.prob.epoch_j + 1) & 0xFFFFFFFF
+            p.launch()
+
+    def result(self):
+        for p in self.passes:
+            p.post()
+        sde = self.sde
+        xx = _lib.to_host(self.xx_dev).astype(self.odt, copy=False)
+        xx = xx.reshape(self.tt.shape + sde.xshape + (sde.paths,))
+        return sde._result(self.tt_asis, xx)
+
+    def run(self):
+        self.launch()
+        return self.result()
+
+    @property
+    def path_steps(self):
+        return self.sde.paths * sum(p.nsteps for p in self.passes)
+
+
+# --------------------------------------------------------------------------
+# presets (integration.py:2056-2770)
+# --------------------------------------------------------------------------
+
+class wiener_process(_gpu_sde):
+    """wiener_process(paths=1, vshape=(), dtype=None, rng=None, steps=None,
+    i0=0, info=None, getinfo=True, method='euler', x0=0., mu=0., sigma=1.,
+    dw=None, corr=None, rho=None)
+
+    Brownian motion with drift, dx = mu dt + sigma dw
+    (sdepy wiener_SDE / wiener_process, integration.py:2056-2116)."""
+    model = _lib.MODEL_WIENER
+    init_defaults = {'x0': 0.}
+    sde_defaults = {'mu': 0., 'sigma': 1.}
+
+    def _param_rows(self, t0s):
+        return np.concatenate((self._eval('mu', t0s, self.wshape),
+                               self._eval('sigma', t0s, self.wshape)), axis=1)
+
+
+class lognorm_process(_gpu_sde):
+    """lognorm_process(paths=1, vshape=(), dtype=None, rng=None, steps=None,
+    i0=0, info=None, getinfo=True, method='euler', x0=1., mu=0., sigma=1.,
+    dw=None, corr=None, rho=None)
+
+    Lognormal process, integrated in log space:
+    da = (mu - sigma^2/2) dt + sigma dw (sdepy lognorm_SDE, log=True,
+    integration.py:2119-2182)."""
+    model = _lib.MODEL_LOGNORM
+    log = True
+
+    def _param_rows(self, t0s):
+        mu = self._eval('mu', t0s, self.wshape)
+        sigma = self._eval('sigma', t0s, self.wshape)
+        return np.concatenate((mu - sigma * sigma / 2, sigma), axis=1)
+
+
+class ornstein_uhlenbeck_process(_gpu_sde):
+    """ornstein_uhlenbeck_process(paths=1, vshape=(), dtype=None, rng=None,
+    steps=None, i0=0, info=None, getinfo=True, method='euler', x0=0.,
+    theta=0., k=1., sigma=1., dw=None, corr=None, rho=None)
+
+    dx = k (theta - x) dt + sigma dw (sdepy ornstein_uhlenbeck_SDE,
+    integration.py:2185-2244)."""
+    model = _lib.MODEL_OU
+    init_defaults = {'x0': 0.}
+    sde_defaults = {'theta': 0., 'k': 1., 'sigma': 1.}
+
+    def _param_rows(self, t0s):
+        w = self.wshape
+        return np.concatenate((self._eval('theta', t0s, w),
+                               self._eval('k', t0s, w),
+                               self._eval('sigma', t0s, w)), axis=1)
+
+
+class hull_white_1factor_process(ornstein_uhlenbeck_process):
+    """Synonym of ornstein_uhlenbeck_process (integration.py:2316-2339)."""
+
+
+class hull_white_process(ornstein_uhlenbeck_process):
+    """hull_white_process(paths=1, vshape=(), dtype=None, rng=None,
+    steps=None, i0=0, info=None, getinfo=True, method='euler', factors=1,
+    x0=0., theta=0., k=1., sigma=1., dw=None, corr=None, rho=None)
+
+    Sum of F correlated Ornstein-Uhlenbeck factors; working shape
+    vshape + (factors,), stored value the sum over factors (sdepy
+    hull_white_SDE, integration.py:2247-2313)."""
+    more_defaults = {'factors': 1}
+
+    def shapes(self, vshape):
+        return vshape, vshape, vshape + (self._args['factors'],)
+
+    def _output_map(self):
+        F = self._args['factors']
+        M = self._nvars()
+        return M // F, _lib.OUT_SUM, F, 0
+
+
+class cox_ingersoll_ross_process(_gpu_sde):
+    """cox_ingersoll_ross_process(paths=1, vshape=(), dtype=None, rng=None,
+    steps=None, i0=0, info=None, getinfo=True, method='euler', x0=1.,
+    theta=1., k=1., xi=1., dw=None, corr=None, rho=None)
+
+    dx = k (theta - x+) dt + xi sqrt(x+) dw, x+ = max(x, 0) (sdepy
+    cox_ingersoll_ross_SDE, integration.py:2342-2400)."""
+    model = _lib.MODEL_CIR
+    init_defaults = {'x0': 1.}
+    sde_defaults = {'theta': 1., 'k': 1., 'xi': 1.}
+
+    def _param_rows(self, t0s):
+        w = self.wshape
+        return np.concatenate((self._eval('theta', t0s, w),
+                               self._eval('k', t0s, w),
+                               self._eval('xi', t0s, w)), axis=1)
+
+
+class full_heston_process(_gpu_sde):
+    """full_heston_process(paths=1, vshape=(), dtype=None, rng=None,
+    steps=None, i0=0, info=None, getinfo=True, method='euler', x0=1., mu=0.,
+    sigma=1., y0=1., theta=1., k=1., xi=1., dw=None, corr=None, rho=None)
+
+    Heston stochastic volatility with full truncation; returns (x, y)
+    (sdepy full_heston_SDE, integration.py:2403-2514).  Working state
+    [log x, y] with the equation axis appended when vshape == () and stacked
+    on the last axis otherwise (SDEs.shapes/pack, integration.py:1688-1792).
+    ``info['negative_y_count']`` counts, per path, the steps that started
+    with y < 0."""
+    model = _lib.MODEL_HESTON
+    q = 2
+    init_defaults = {'x0': 1., 'y0': 1.}
+    sde_defaults = {'mu': 0., 'sigma': 1., 'theta': 1., 'k': 1., 'xi': 1.}
+
+    def shapes(self, vshape):
+        self.addaxis = vshape == ()
+        if self.addaxis:
+            w = vshape + (2,)
+        else:
+            w = vshape[:-1] + (vshape[-1] * 2,)
+        return vshape, w, w
+
+    def _init_target(self):
+        return self.vshape
+
+    def _init_state(self, P, cdt):
+        # full_heston_SDE.init (integration.py:2416-2419) with SDEs.pack
+        tgt = self.vshape + (P,)
+        lx = np.broadcast_to(np.log(np.asarray(self._args['x0'], float)), tgt)
+        y0 = np.broadcast_to(np.asarray(self._args['y0'], float), tgt)
+        X = np.stack((lx, y0), axis=-2) if self.addaxis else \
+            np.concatenate((lx, y0), axis=-2)
+        return X.reshape(self._nvars(), P).astype(cdt)
+
+    def _param_rows(self, t0s):
+        v = self.vshape
+        return np.concatenate([self._eval(n, t0s, v) for n in
+                               ('mu', 'sigma', 'theta', 'k', 'xi')], axis=1)
+
+    def _output_map(self):
+        M = self._nvars()
+        return M, _lib.OUT_IDENTITY, 1, (1 << (M // 2)) - 1
+
+    def _unpack(self, xx):
+        if self.addaxis:
+            return xx[..., 0, :], xx[..., 1, :]
+        d = self.vshape[-1]
+        return xx[..., :d, :], xx[..., d:, :]
+
+    def _result(self, tt, xx):
+        x, y = self._unpack(xx)
+        return process(tt, x=x), process(tt, x=y)
+
+    def _pass_info(self, info_paths, info_rows, tt_pass, row_of, dn_host, rows):
+        if info_paths is None:
+            return
+        N = self._nvars() // 2
+        cnt = _lib.to_host(info_paths)[:N].astype(int)
+        self.info['negative_y_count'] = cnt.reshape(self.vshape + (self.paths,))
+
+
+class heston_process(full_heston_process):
+    """heston_process: as full_heston_process, storing and returning x only
+    (sdepy heston_SDE, integration.py:2517-2573)."""
+
+    def shapes(self, vshape):
+        v, x, w = super().shapes(vshape)
+        return v, v, w
+
+    def _output_map(self):
+        N = self._nvars() // 2
+        return N, _lib.OUT_FIRST, 1, (1 << N) - 1
+
+    def _result(self, tt, xx):
+        return process(tt, x=xx)
+
+
+class jumpdiff_process(_gpu_sde):
+    """jumpdiff_process(paths=1, vshape=(), dtype=None, rng=None, steps=None,
+    i0=0, info=None, getinfo=True, method='euler', x0=1., mu=0., sigma=1.,
+    dw=None, corr=None, rho=None, dj=None, dn=None, ptype=int, lam=1., y=None)
+
+    Lognormal process with compound Poisson log-jumps, no martingale
+    correction: da = (mu - sigma^2/2) dt + sigma dw + dj (sdepy jumpdiff_SDE,
+    integration.py:2576-2695).  ``info['jump_rate']`` and
+    ``info['jump_count']`` as in jumpdiff_SDE.info_next (:2610-2623)."""
+    model = _lib.MODEL_JUMPDIFF
+    log = True
+    sources = ('dt', 'dw', 'dj')
+    dj_defaults = {'dj': None, 'dn': None, 'ptype': int, 'lam': 1., 'y': None}
+
+    def _dj_setup(self):
+        a = self._args
+        return _source_setup(a['dj'], cpoisson_source, paths=self.paths,
+                             vshape=self.wshape, dtype=self.dtype,
+                             rng=self._rng_asis, dn=a['dn'], ptype=a['ptype'],
+                             lam=a['lam'], y=self._y())
+
+    def _y(self):
+        return self._args['y']
+
+    def _param_rows(self, t0s):
+        mu = self._eval('mu', t0s, self.wshape)
+        sigma = self._eval('sigma', t0s, self.wshape)
+        return np.concatenate((mu - sigma * sigma / 2, sigma), axis=1)
+
+    def _pass_info(self, info_paths, info_rows, tt_pass, row_of, dn_host, rows):
+        # jumpdiff_SDE.info_begin/info_next/info_end (integration.py:2588-2623):
+        # jump_rate[i-1] accumulates the jumps of the steps that end in
+        # (tt[i-1], tt[i]], per unit time and path; jump_count per path.
+        P = self.paths
+        n_t = tt_pass.size
+        totals = np.zeros(n_t)
+        if dn_host:
+            counts = np.sum(dn_host, axis=0)
+            j = 1
+            for s, r in enumerate(rows):
+                if j < n_t:
+                    totals[j] += dn_host[s].sum()
+                if r >= 0:
+                    j += 1
+        elif self._host_dj:
+            counts = np.zeros((self._nvars(), P), dtype=int)
+        elif info_paths is not None:
+            counts = _lib.to_host(info_paths)
+            tr = _lib.to_host(info_rows).astype(np.int64)
+            for j in range(1, n_t):
+                totals[j] = tr[int(row_of(j))]
+        else:
+            return
+        rate = np.zeros(tt_pass.shape, dtype=float)
+        for j in range(1, n_t):
+            rate[j - 1] = totals[j] / (tt_pass[j] - tt_pass[j - 1]) / P
+        if n_t > 1:
+            rate[-1] = rate[-2]
+        self.info['jump_rate'] = rate
+        self.info['jump_count'] = np.asarray(counts, dtype=int).reshape(
+            self.vshape + (P,))
+
+
+class merton_jumpdiff_process(jumpdiff_process):
+    """merton_jumpdiff_process: jumpdiff_process with y = norm_rv(a, b)
+    (sdepy merton_jumpdiff_SDE, integration.py:2698-2731); jumps are drawn
+    inside the Euler kernel."""
+    dj_defaults = {'dj': None, 'dn': None, 'ptype': int, 'lam': 1.,
+                   'a': 0., 'b': 1.}
+
+    def _y(self):
+        return norm_rv(a=self._args['a'], b=self._args['b'])
+
+
+class kou_jumpdiff_process(jumpdiff_process):
+    """kou_jumpdiff_process: jumpdiff_process with double exponential jump
+    sizes (sdepy kou_jumpdiff_SDE, integration.py:2734-2770); counts are drawn
+    on the GPU, sizes on the host through ``rvs``, and the increments are
+    replayed in the kernel."""
+    dj_defaults = {'dj': None, 'dn': None, 'ptype': int, 'lam': 1.,
+                   'a': .5, 'b': .5, 'pa': .5}
+
+    def _y(self):
+        return _double_exp_rv(self._args['a'], self._args['b'],
+                              self._args['pa'])
+
+
+class _double_exp_rv:
+    """Double exponential distribution (sdepy _double_exp_rv,
+    infrastructure.py:1756-1793), sampled on the host."""
+
+    def __init__(self, a=1., b=1., pa=.5):
+        self.a, self.b, self.pa = a, b, pa
+
+    def rvs(self, size, random_state=None):
+        import scipy.stats
+        plus = scipy.stats.expon(scale=self.a).rvs(size=size, random_state=random_state)
+        minus = scipy.stats.expon(scale=self.b).rvs(size=size, random_state=random_state)
+        u = scipy.stats.uniform(scale=1.).rvs(size=size, random_state=random_state)
+        return np.where(u <= self.pa, plus, -minus) + 0

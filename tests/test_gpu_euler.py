@@ -1,0 +1,294 @@
+"""Parity of the HIP Euler path kernels with the oracle (GPU tests).
+
+Every case runs the sdepy_amd preset on the GPU and the oracle's restatement
+of sdepy's loop on the CPU on identical inputs:
+  * replay: both consume the same recorded increments (sdepy's own replay
+    hook, a bare source-protocol callable) -- FP64 must agree to 1e-12;
+  * Philox: the oracle regenerates the kernel's counter-based draws with its
+    host twin -- FP64 to 1e-12 (up to libm ulps), FP32 to the stated
+    tolerance against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import sdepy_amd as sa
+from oracle import euler as ora
+from oracle import philox
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-12
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+class _Replay:
+    """Bare source-protocol object (sdepy's tests/test_processes.py:100-109)."""
+
+    def __init__(self, inc, paths, vshape):
+        self.inc, self.paths, self.vshape, self.k = inc, paths, vshape, 0
+
+    def __call__(self, t, dt):
+        z = self.inc[self.k]
+        self.k += 1
+        return z
+
+
+def test_philox_device_known_answers_and_twin(gpu):
+    torch = gpu
+    from sdepy_amd import _lib
+    rng = np.random.default_rng(7)
+    ctr = rng.integers(0, 2 ** 32, size=(4096, 4), dtype=np.uint64).astype(np.uint32)
+    kat = np.array([[0, 0, 0, 0], [0xffffffff] * 4,
+                    [0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344]], np.uint32)
+    keys = [(0, 0), (0xffffffff, 0xffffffff), (0xa4093822, 0x299f31d0)]
+    expect = [[0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8],
+              [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd],
+              [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]]
+    for c, k, e in zip(kat, keys, expect):
+        d_c = _lib.to_device(c[None, :])
+        out = _lib.empty((1, 4), np.uint32)
+        _lib.check(_lib.lib().sdegpu_philox_fill(_lib.key2(k), _lib.ptr(d_c),
+                                                 _lib.ptr(out), 1, _lib.stream()))
+        got = _lib.to_host(out).view(np.uint32)[0]
+        assert [int(x) for x in got] == e
+    key = (0x1234567, 0x89abcdef)
+    d_c = _lib.to_device(ctr)
+    out = _lib.empty(ctr.shape, np.uint32)
+    _lib.check(_lib.lib().sdegpu_philox_fill(_lib.key2(key), _lib.ptr(d_c),
+                                             _lib.ptr(out), ctr.shape[0],
+                                             _lib.stream()))
+    got = _lib.to_host(out).view(np.uint32)
+    twin = np.stack(philox.philox4x32_10(ctr[:, 0], ctr[:, 1], ctr[:, 2],
+                                         ctr[:, 3], *key), axis=-1)
+    assert np.array_equal(got, twin)
+
+
+def test_gbm_replay_fp64_matches_oracle_c1(gpu):
+    """Config C1: lognorm, 10k paths, 252 steps, FP64, recorded increments."""
+    P = 10_000
+    tt = np.linspace(0, 1, 253)
+    dt = np.diff(tt)
+    inc = np.random.RandomState(1234).normal(size=(252, P)) * np.sqrt(dt)[:, None]
+    args = dict(x0=1., mu=.05, sigma=.2)
+    x = sa.lognorm_process(paths=P, dtype=np.float64,
+                           dw=_Replay(inc, P, ()), **args)(tt)
+    ref = ora.integrate('lognorm', tt, paths=P,
+                        sources={'dw': ora.ReplaySource(inc, P, ())}, **args)
+    assert isinstance(x, sa.process) and x.shape == (253, P)
+    assert x.t is tt or np.array_equal(x.t, tt)
+    assert _rel(x, ref) < RTOL64
+    # terminal pmean/pvar through the GPU reduction vs NumPy on the oracle
+    m, v = x.pmean(), x.pvar()
+    assert _rel(m[:, 0], ref.mean(axis=-1)) < RTOL64
+    assert _rel(v[1:, 0], ref[1:].var(axis=-1)) < 1e-11
+    # log-Euler exactness with constant parameters
+    # (sdepy tests/test_processes.py:681-708)
+    exact = np.exp((.05 - .02) * 1. + .2 * inc.sum(axis=0))
+    assert _rel(x[-1], exact) < 1e-12
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_gbm_philox_matches_twin(gpu, dtype):
+    P, tt = 3000, np.linspace(0, 2, 41)
+    args = dict(x0=2., mu=.03, sigma=.35)
+    p = sa.lognorm_process(paths=P, dtype=dtype, rng=np.random.default_rng(5),
+                           steps=101, **args)
+    key = p.sources['dw'].key
+    x = p(tt)
+    assert x.dtype == dtype
+    src = ora.PhiloxWiener(key, 0, P, ())
+    ref = ora.integrate('lognorm', tt, paths=P, steps=101,
+                        sources={'dw': src}, **args)
+    tol = RTOL64 if dtype == np.float64 else 3e-5
+    assert _rel(x, ref) < tol
+    # a second call draws a fresh epoch
+    x2 = p(tt)
+    ref2 = ora.integrate('lognorm', tt, paths=P, steps=101,
+                         sources={'dw': ora.PhiloxWiener(key, 1, P, ())}, **args)
+    assert _rel(x2, ref2) < tol
+    assert not np.allclose(x2, x)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_ou_correlated_matches_twin(gpu, dtype):
+    """Config C2 in miniature: OU with 4 correlated components."""
+    g = np.random.default_rng(1234)
+    theta, k, sigma = (g.uniform(-1, 1, (4, 1)), g.uniform(.5, 3, (4, 1)),
+                       g.uniform(.1, .5, (4, 1)))
+    A = g.standard_normal((4, 4))
+    S = A @ A.T + 4 * np.eye(4)
+    d = 1 / np.sqrt(np.diag(S))
+    C = S * d[:, None] * d[None, :]
+    P, tt = 2048, np.linspace(0, 1, 51)
+    p = sa.ornstein_uhlenbeck_process(paths=P, vshape=4, dtype=dtype, x0=0.,
+                                      theta=theta, k=k, sigma=sigma, corr=C,
+                                      rng=np.random.default_rng(3))
+    key = p.sources['dw'].key
+    x = p(tt)
+    assert x.shape == (51, 4, P)
+    src = ora.PhiloxWiener(key, 0, P, (4,), corr=C)
+    ref = ora.integrate('ou', tt, paths=P, vshape=(4,), sources={'dw': src},
+                        x0=0., theta=theta, k=k, sigma=sigma)
+    if dtype == np.float64:
+        assert np.max(np.abs(x - ref)) < 1e-12
+    else:
+        assert np.max(np.abs(x - ref)) < 2e-5
+
+
+def test_hull_white_2f_and_cir_and_wiener(gpu):
+    P, tt = 1500, np.linspace(0, 1, 31)
+    # hull-white 2 factors: output = sum over factors
+    p = sa.hull_white_process(paths=P, factors=2, x0=np.array([[.01], [.02]]),
+                              theta=np.array([[.03], [.01]]),
+                              k=np.array([[.1], [.3]]),
+                              sigma=np.array([[.01], [.02]]), rho=.4,
+                              rng=np.random.default_rng(11))
+    key = p.sources['dw'].key
+    x = p(tt)
+    ref = ora.integrate('hull_white', tt, paths=P, factors=2,
+                        sources={'dw': ora.PhiloxWiener(key, 0, P, (2,), rho=.4)},
+                        x0=np.array([[.01], [.02]]),
+                        theta=np.array([[.03], [.01]]),
+                        k=np.array([[.1], [.3]]),
+                        sigma=np.array([[.01], [.02]]))
+    assert x.shape == (31, P)
+    assert np.max(np.abs(x - ref)) < 1e-14
+    # CIR with negative excursions (xi large)
+    p = sa.cox_ingersoll_ross_process(paths=P, x0=.05, theta=.04, k=1., xi=.8,
+                                      rng=np.random.default_rng(12))
+    key = p.sources['dw'].key
+    x = p(tt)
+    ref = ora.integrate('cir', tt, paths=P,
+                        sources={'dw': ora.PhiloxWiener(key, 0, P, ())},
+                        x0=.05, theta=.04, k=1., xi=.8)
+    assert (ref < 0).any()
+    assert np.max(np.abs(x - ref)) < 1e-12
+    # Wiener with a vshape and time-dependent drift
+    mu = lambda t: np.array([[.1], [-.2], [.3]]) * t  # noqa: E731
+    p = sa.wiener_process(paths=P, vshape=3, x0=1., mu=mu, sigma=.5,
+                          rng=np.random.default_rng(13))
+    key = p.sources['dw'].key
+    x = p(tt)
+    ref = ora.integrate('wiener', tt, paths=P, vshape=(3,),
+                        sources={'dw': ora.PhiloxWiener(key, 0, P, (3,))},
+                        x0=1., mu=mu, sigma=.5)
+    assert np.max(np.abs(x - ref)) < 1e-12
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_full_heston_matches_twin(gpu, dtype):
+    """Config C3 parameters in miniature (Feller condition fails: y < 0 occurs)."""
+    P, n = 2000, 100
+    args = dict(x0=100., mu=.02, sigma=1., y0=.04, theta=.04, k=2., xi=.5)
+    info = {}
+    p = sa.full_heston_process(paths=P, dtype=dtype, steps=n + 1, rho=-.7,
+                               info=info, rng=np.random.default_rng(21), **args)
+    key = p.sources['dw'].key
+    x, y = p((0., 1.))
+    ref_info = {}
+    rx, ry = ora.integrate('full_heston', (0., 1.), paths=P, steps=n + 1,
+                           sources={'dw': ora.PhiloxWiener(key, 0, P, (2,), rho=-.7)},
+                           info=ref_info, **args)
+    assert x.shape == y.shape == (2, P)
+    if dtype == np.float64:
+        assert _rel(x, rx) < 1e-10
+        assert np.max(np.abs(y - ry)) < 1e-12
+        assert np.array_equal(info['negative_y_count'], ref_info['negative_y_count'])
+        assert ref_info['negative_y_count'].sum() > 0
+    else:
+        assert _rel(x, rx) < 1e-4
+        assert np.max(np.abs(y - ry)) < 1e-4
+    assert info['computed_steps'] == n and info['stored_steps'] == 1
+    # heston_process stores x only
+    p2 = sa.heston_process(paths=P, dtype=dtype, steps=n + 1, rho=-.7,
+                           rng=np.random.default_rng(21), **args)
+    x2 = p2((0., 1.))
+    assert x2.shape == (2, P)
+    assert np.array_equal(x2, x)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_merton_matches_twin(gpu, dtype):
+    P, n = 400, 40
+    args = dict(x0=1., mu=.03, sigma=.2, lam=5., a=-.1, b=.15)
+    info = {}
+    p = sa.merton_jumpdiff_process(paths=P, dtype=dtype, steps=n + 1, info=info,
+                                   rng=np.random.default_rng(31), **args)
+    kw, kj = p.sources['dw'].key, p.sources['dj'].key
+    tt = np.linspace(0, 1, 5)
+    x = p(tt)
+    ref_info = {}
+    ref = ora.integrate(
+        'jumpdiff', tt, paths=P, steps=n + 1, info=ref_info,
+        sources={'dw': ora.PhiloxWiener(kw, 0, P, ()),
+                 'dj': ora.PhiloxCPoisson(kj, 0, P, (), lam=5., a=-.1, b=.15)},
+        x0=1., mu=.03, sigma=.2)
+    assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+    assert ref_info['jump_count'].sum() > 100
+    assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12)
+    tol = RTOL64 if dtype == np.float64 else 5e-5
+    assert _rel(x, ref) < tol
+
+
+def test_backward_integration_i0_and_steps(gpu):
+    """i0 != 0 runs a backward pass from tt[i0] (paths_generator.__call__,
+    integration.py:571-580)."""
+    P, tt = 700, np.array([0., .3, .5, 1., 1.7, 2.])
+    args = dict(x0=1.5, mu=.1, sigma=.3)
+    info = {}
+    p = sa.lognorm_process(paths=P, i0=3, steps=23, info=info,
+                           rng=np.random.default_rng(41), **args)
+    key = p.sources['dw'].key
+    x = p(tt)
+    ref_info = {}
+
+    class _TwoPass(ora.PhiloxWiener):
+        def begin_pass(self, tag):
+            super().begin_pass(tag)
+            self.epoch = 0 if tag else 1
+
+    ref = ora.integrate('lognorm', tt, paths=P, i0=3, steps=23, info=ref_info,
+                        sources={'dw': _TwoPass(key, 0, P, ())}, **args)
+    assert np.all(x[3] == 1.5)
+    assert _rel(x, ref) < RTOL64
+    assert info['computed_steps'] == ref_info['computed_steps']
+    assert info['stored_steps'] == ref_info['stored_steps'] == 5
+
+
+def test_path_moments_match_numpy(gpu):
+    rng = np.random.default_rng(9)
+    for dtype in (np.float64, np.float32):
+        x = (3 + rng.standard_normal((7, 5, 70001))).astype(dtype)
+        p = sa.process(t=np.arange(7.), x=x)
+        m, v = p.pmean(dtype=np.float64), p.pvar(dtype=np.float64, ddof=1)
+        s, sd = p.psum(dtype=np.float64), p.pstd(dtype=np.float64)
+        xd = x.astype(np.float64)
+        assert _rel(m[..., 0], xd.mean(axis=-1)) < 1e-13
+        assert _rel(v[..., 0], xd.var(axis=-1, ddof=1)) < 1e-12
+        assert _rel(s[..., 0], xd.sum(axis=-1)) < 1e-13
+        assert _rel(sd[..., 0], xd.std(axis=-1)) < 1e-12
+        assert p.pmean().dtype == dtype
+        out = np.full((7, 5, 1), np.nan)
+        r = p.pmean(out=out)
+        assert np.shares_memory(r, out)
+
+
+def test_gbm_statistics_within_3se(gpu):
+    """FP32 Philox paths: terminal mean and variance against the lognormal
+    closed forms (analytical.py:133-161), within 3 standard errors."""
+    from oracle import closed_forms as cf
+    P = 1_000_000
+    p = sa.lognorm_process(paths=P, dtype=np.float32, x0=1., mu=.05, sigma=.2,
+                           rng=np.random.default_rng(2024))
+    x = p(np.linspace(0, 1, 53))
+    xT = x[-1].astype(np.float64)
+    m, v = cf.lognorm_mean(1., 1., .05, .2), cf.lognorm_var(1., 1., .05, .2)
+    se_m = np.sqrt(v / P)
+    assert abs(xT.mean() - m) < 3 * se_m
+    mu4 = np.mean((xT - xT.mean()) ** 4)
+    se_v = np.sqrt((mu4 - v * v) / P)
+    assert abs(xT.var() - v) < 3 * se_v
