@@ -1,0 +1,230 @@
+"""Benchmark of the path-integration hot path on one MI355X.
+
+Workload (one "step"): Heston full-truncation Euler, 4,000,000 paths x 365
+steps over one year, FP32 state, European call K=100: one launch of the
+reverse-mode adjoint kernel, which returns the price and the 7 pathwise
+Greeks d/d(x0, r, k, theta, xi, rho, v0) with FP64 accumulation.
+
+Printed as one JSON line:
+  value       path-steps/s of the FP32 forward+adjoint (4M*365 per step)
+  fp64_fd     the same set (price + 7 Greeks) from 15 FP64 forward runs with
+              central finite differences on common random numbers, its
+              throughput and the speedup of the adjoint over it
+  accuracy    |FP32 adjoint - FP64 FD| / SE for the price and every Greek
+  roofline    the adjoint kernel: algorithmic FP32 flops / kernel time vs the
+              MI355X FP32 vector peak (the kernel is VALU-bound, not HBM or
+              MFMA); HIP-event timing on the launch stream
+  ou_fullpath the OU-4 correlated full-path kernel (1M x 1000, FP32), the
+              HBM-bound case: stored bytes / kernel time vs 8 TB/s
+  cpu_baseline the oracle's NumPy restatement of sdepy's loop (same
+              operations as the reference, NumPy draws) on the host, one core,
+              price + 7 Greeks by 15 forward runs on a bounded path sample
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ('path-steps/s, Heston 4M×365 FP32 fwd+adjoint vs FP64 fwd+FD, '
+          'at ≤3 SE')
+HPRM = dict(x0=100., r=.02, sigma=1., k=2., theta=.04, xi=.5, rho=-.7, y0=.04)
+PATHS, NSTEPS, STRIKE, MATURITY = 4_000_000, 365, 100., 1.
+
+# algorithmic FP32 flops per path-step of the adjoint kernel (FMA = 2):
+#   forward pass 26 (Euler 16 + Box-Muller arithmetic 10), segment recompute
+#   of the variance 18 (Box-Muller 10 + variance update 8), reverse sweep 30
+ADJ_FLOPS_PER_PATH_STEP = 74
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.
+
+
+def _events(torch):
+    return (torch.cuda.Event(enable_timing=True),
+            torch.cuda.Event(enable_timing=True))
+
+
+def timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def cpu_baseline(paths, steps):
+    """The oracle (NumPy restatement of sdepy's Euler loop with NumPy
+    multivariate-normal draws, as sdepy makes them) timed on one host core:
+    15 forward runs give the price and 7 central-difference Greeks."""
+    from oracle import euler as ora
+    names = ('x0', 'r', 'k', 'theta', 'xi', 'rho', 'y0')
+    base = {k: HPRM[k] for k in names}
+    runs = [dict(base)]
+    for n in names:
+        for sg in (1, -1):
+            d = dict(base)
+            d[n] = base[n] * (1 + sg * 1e-4)
+            runs.append(d)
+    t0 = time.perf_counter()
+    for d in runs:
+        rng = np.random.RandomState(1234)   # common random numbers
+        src = ora.NumpyWiener(rng, paths, (2,), rho=d['rho'])
+        x, _ = ora.integrate('full_heston', (0., MATURITY), paths=paths,
+                             steps=steps + 1, sources={'dw': src},
+                             x0=d['x0'], mu=d['r'], sigma=1., y0=d['y0'],
+                             theta=d['theta'], k=d['k'], xi=d['xi'])
+        np.maximum(x[-1] - STRIKE, 0.).mean()
+    dt = time.perf_counter() - t0
+    return {'value': paths * steps / dt, 'unit': 'path-steps/s',
+            'cores': 1, 'kind': 'port',
+            'sample': ('{} paths x {} steps, FP64, price + 7 Greeks by 15 CRN '
+                       'forward runs of the oracle, {:.1f} s').format(
+                           paths, steps, dt),
+            'host_cpu': _cpu_name(),
+            'forward_path_steps_per_s': 15 * paths * steps / dt}
+
+
+def cpu_baseline_subprocess(paths, steps):
+    """cpu_baseline in a child process pinned to one thread."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS='1', OPENBLAS_NUM_THREADS='1',
+               MKL_NUM_THREADS='1')
+    code = ('import json, bench; print(json.dumps(bench.cpu_baseline({}, {})))'
+            .format(int(paths), int(steps)))
+    res = subprocess.run([sys.executable, '-c', code], cwd=ROOT, env=env,
+                         capture_output=True, text=True, check=True)
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def _cpu_name():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--legs', default='adjoint,fd,ou,cpu',
+                    help='comma list of: adjoint, fd, ou, cpu')
+    ap.add_argument('--cpu-paths', type=int, default=30000)
+    args = ap.parse_args()
+    legs = set(args.legs.split(','))
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: this build runs the path kernels on '
+              'a single MI355X (paths shard by range across GPUs, see '
+              'DESIGN.md); --gpus {} is not supported'.format(args.gpus),
+              file=sys.stderr)
+        sys.exit(2)
+
+    from sdepy_amd import _lib, greeks as G
+    torch = _lib.require_gpu()
+    torch.cuda.set_device(0)
+
+    # ---- headline: FP32 forward + adjoint, price + 7 Greeks ----
+    if 'adjoint' not in legs:
+        args.steps, args.warmup = 1, 0
+    run = G.HestonGreeks(paths=PATHS, steps=NSTEPS, strike=STRIKE,
+                         maturity=MATURITY, dtype=np.float32,
+                         key=(0x5eed, 0xc0ffee), epoch=0, **HPRM)
+    ms, wall = timed(torch, run.launch, args.steps, args.warmup)
+    res = run.result()
+    value = run.path_steps / (ms * 1e-3)
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'path-steps/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'heston_ft_adjoint_4Mx365_fp32',
+                   'paths': PATHS, 'steps_per_path': NSTEPS,
+                   'maturity': MATURITY, 'strike': STRIKE, **HPRM,
+                   'greeks': list(G.HESTON_GREEKS)},
+        'wall_s_timed': wall,
+        'price': res.price, 'price_se': res.price_se,
+        'greeks': res.greeks, 'greeks_se': res.greeks_se,
+        'greeks_per_s': 7 / (ms * 1e-3),
+    }
+    achieved = ADJ_FLOPS_PER_PATH_STEP * run.path_steps / (ms * 1e-3) / 1e12
+    out['roofline'] = {
+        'kernel': 'k_heston_adjoint<float>', 'bound': 'valu',
+        'achieved': achieved, 'peak': FP32_VECTOR_PEAK_TFLOPS,
+        'unit': 'TFLOP/s', 'frac': achieved / FP32_VECTOR_PEAK_TFLOPS,
+        'traffic': None,
+        'flops_per_path_step': ADJ_FLOPS_PER_PATH_STEP}
+
+    # ---- FP64 forward + CRN finite differences, same paths ----
+    if 'fd' in legs:
+        fd = G.HestonFD(paths=PATHS, steps=NSTEPS, strike=STRIKE,
+                        maturity=MATURITY, dtype=np.float64,
+                        key=(0x5eed, 0xc0ffee), epoch=0, h=1e-4, **HPRM)
+        fd_ms, _ = timed(torch, fd.launch, 2, 1)
+        price64, se64, fdg = fd.result()
+        fd_value = PATHS * NSTEPS / (fd_ms * 1e-3)
+        acc = {'price': abs(res.price - price64) / res.price_se}
+        for k in G.HESTON_GREEKS:
+            acc[k] = abs(res.greeks[k] - fdg[k]) / res.greeks_se[k]
+        out['fp64_fd'] = {'value': fd_value, 'unit': 'path-steps/s',
+                          'ms_per_set': fd_ms, 'price': price64,
+                          'greeks': fdg, 'speedup_of_adjoint': value / fd_value}
+        out['accuracy_se'] = acc
+        out['accuracy_max_se'] = max(acc.values())
+        del fd
+        torch.cuda.empty_cache()
+
+    # ---- OU-4 correlated full path (HBM-bound) ----
+    if 'ou' in legs:
+        import sdepy_amd as sa
+        g = np.random.default_rng(1234)
+        th, kk, sg = (g.uniform(-1, 1, (4, 1)), g.uniform(.5, 3, (4, 1)),
+                      g.uniform(.1, .5, (4, 1)))
+        A = g.standard_normal((4, 4))
+        S = A @ A.T + 4 * np.eye(4)
+        dd = 1 / np.sqrt(np.diag(S))
+        C = S * dd[:, None] * dd[None, :]
+        p = sa.ornstein_uhlenbeck_process(
+            paths=10 ** 6, vshape=4, dtype=np.float32, x0=0., theta=th, k=kk,
+            sigma=sg, corr=C, rng=np.random.default_rng(1))
+        ou = p.prepare(np.linspace(0., 1., 1001))
+        ou_ms, _ = timed(torch, lambda: ou.launch(fresh=True), 5, 2)
+        nbytes = 1001 * 4 * 10 ** 6 * 4
+        gbs = nbytes / (ou_ms * 1e-3) / 1e9
+        out['ou_fullpath'] = {
+            'workload': 'ou4_corr_fullpath_1Mx1000_fp32',
+            'ms': ou_ms, 'path_steps_per_s': ou.path_steps / (ou_ms * 1e-3),
+            'roofline': {'kernel': 'k_euler<OU,float,4>', 'bound': 'hbm',
+                         'achieved': gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                         'frac': gbs / HBM_PEAK_GBS, 'traffic': None,
+                         'bytes_per_path_step': 16}}
+        del ou, p
+        torch.cuda.empty_cache()
+
+    if 'cpu' in legs:
+        out['cpu_baseline'] = cpu_baseline_subprocess(args.cpu_paths, NSTEPS)
+        out['cpu_baseline']['gpu_over_cpu'] = value / out['cpu_baseline']['value']
+    else:
+        out['cpu_baseline'] = None
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

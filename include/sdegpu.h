@@ -92,7 +92,9 @@ typedef struct sdegpu_euler_problem {
   int32_t nout;       /* values stored per path and output row = prod(xshape) */
   int32_t out_mode;   /* SDEGPU_OUT_* */
   int32_t out_group;  /* factor count for SDEGPU_OUT_SUM */
-  uint32_t exp_mask;  /* bit o set: store exp(value o) (log-integrated components) */
+  uint32_t exp_mask;  /* bit o set: store exp(value o); must be the model's log components:
+                         all nout for LOGNORM/JUMPDIFF (SDE.log, integration.py:1193-1197),
+                         the first M/2 for HESTON (log x, :2441-2444), none otherwise */
   int32_t has_corr;   /* table rows carry a Cholesky factor */
   int64_t paths;      /* 1 .. 2^32-1 */
   int32_t nsteps;     /* Euler intervals in this pass (>= 0) */
@@ -174,6 +176,68 @@ int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                          uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
                          const double* pois, const int32_t* sign, const void* ab, void* out,
                          long long* dn_out, void* stream);
+
+/*
+ * Pathwise Greeks by reverse-mode differentiation of the Euler scheme.
+ *
+ * sdepy has no Greeks: these entry points have no counterpart in the
+ * reference.  They differentiate exactly the discretisation of
+ * full_heston_SDE (integration.py:2425-2433) and of the log-space
+ * lognorm/jumpdiff SDEs (integration.py:2127-2130, :2594-2608) on the Philox
+ * draws of the forward kernel (same key/epoch => same paths as sdegpu_euler),
+ * so the price equals the forward estimator and the gradients are its
+ * pathwise derivatives.  Payoff: exp(-r T) max(+-(x_T - strike), 0).
+ *
+ * out (device double[2 + 2*NG]): sum of discounted payoffs, sum of their
+ * squares, then for each Greek i the sum of per-path derivatives and the sum
+ * of their squares (the host divides by paths and forms standard errors).
+ *
+ * Heston (NG = 7): d/d(x0, r, k, theta, xi, rho, y0); r is both the drift mu
+ * of log x and the discount rate.  Reverse sweep over checkpoint segments of
+ * 16 steps recomputed from the Philox counters; the workspace holds
+ * ceil(nsteps/16) variance checkpoints per path plus block partials.
+ * steps: device [nsteps][2] = (ds, sqrt(ds)) in dtype.  Scalar rho builds
+ * corr = [[1, rho], [rho, 1]] as sdepy does (infrastructure.py:128-150).
+ */
+typedef struct sdegpu_heston_greeks {
+  int32_t dtype;   /* SDEGPU_F32 / SDEGPU_F64 */
+  int32_t payoff;  /* 0 call, 1 put */
+  int64_t paths;
+  int32_t nsteps;
+  const void* steps;
+  double x0, r, sigma, k, theta, xi, rho, y0; /* full_heston_process keywords (mu = r) */
+  double strike, maturity;
+  uint32_t key[2]; /* Philox key of the dw source */
+  uint32_t epoch;
+} sdegpu_heston_greeks;
+
+size_t sdegpu_heston_greeks_workspace(const sdegpu_heston_greeks* g);
+int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* out, void* workspace,
+                             size_t workspace_bytes, void* stream);
+
+/*
+ * Lognormal / Merton jump-diffusion (NG = 5): d/d(x0, mu, sigma, a, b) of
+ * exp(-r T) max(+-(x_T - strike), 0) for lognorm_process (pois = NULL) or
+ * merton_jumpdiff_process with jumps a + b N(0,1); r is a fixed discount
+ * rate.  pois: device [nsteps][2] = (lam*ds, exp(-lam*ds)) or NULL.
+ */
+typedef struct sdegpu_loglin_greeks {
+  int32_t dtype;
+  int32_t payoff;
+  int64_t paths;
+  int32_t nsteps;
+  const void* steps;   /* device [nsteps][2] = (ds, sqrt(ds)) */
+  const double* pois;  /* device [nsteps][2] or NULL */
+  double x0, mu, sigma, a, b, r, strike, maturity;
+  uint32_t key_w[2];
+  uint32_t epoch_w;
+  uint32_t key_j[2];
+  uint32_t epoch_j;
+} sdegpu_loglin_greeks;
+
+size_t sdegpu_loglin_greeks_workspace(int64_t paths);
+int sdegpu_loglin_greeks_run(const sdegpu_loglin_greeks* g, double* out, void* workspace,
+                             size_t workspace_bytes, void* stream);
 
 /*
  * Philox4x32-10 evaluation, for known-answer tests.

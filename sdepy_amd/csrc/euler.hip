@@ -60,21 +60,26 @@ __device__ __forceinline__ double fast_sqrt<double>(double v) {
   return sqrt(v);
 }
 
-template <typename T, int M>
-__device__ __forceinline__ void store_state(const EulerArgs& a, int row, const T (&x)[M],
-                                            int64_t gid) {
+// Which stored values are exponentiated is fixed by the model, as sdepy's
+// `log` attribute is fixed by the preset: lognorm/jumpdiff integrate log x
+// (SDE.exit exps everything, integration.py:1193-1197), full_heston/heston
+// keep log x in the first N components (full_heston_SDE.result :2441-2444,
+// heston_SDE.result :2539-2541); the rest store the state as is.  A
+// compile-time rule keeps the exp out of the store path of the other models.
+template <int MODEL, int M>
+struct OutRule {
+  static constexpr bool exp_all = MODEL == SDEGPU_MODEL_LOGNORM || MODEL == SDEGPU_MODEL_JUMPDIFF;
+  static constexpr int exp_first = MODEL == SDEGPU_MODEL_HESTON ? M / 2 : (exp_all ? M : 0);
+  static constexpr uint32_t mask = exp_first >= 32 ? 0xFFFFFFFFu : ((1u << exp_first) - 1u);
+};
+
+template <int MODEL, typename T, int M>
+__device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs& a, int row,
+                                            const T (&x)[M], int64_t gid) {
+  constexpr int EF = OutRule<MODEL, M>::exp_first;
   const int64_t P = a.paths;
-  T* out = (T*)a.out + (int64_t)row * a.nout * P + gid;
-  if (a.out_group == 1) {
-#pragma unroll
-    for (int o = 0; o < M; ++o) {
-      if (o < a.nout) {
-        T val = x[o];
-        if ((a.exp_mask >> o) & 1u) val = exp(val);
-        __builtin_nontemporal_store(val, out + o * P);
-      }
-    }
-  } else {
+  T* o_row = out + (int64_t)row * a.nout * P + gid;
+  if (MODEL == SDEGPU_MODEL_OU && a.out_group != 1) {
     // hull_white_SDE.let: out_x[...] = x.sum(axis=-2) over each factor group.
 #pragma unroll
     for (int o = 0; o < M; ++o) {
@@ -84,15 +89,27 @@ __device__ __forceinline__ void store_state(const EulerArgs& a, int row, const T
 #pragma unroll
         for (int v = 0; v < M; ++v)
           if (v >= lo && v < hi) val = val + x[v];
-        if ((a.exp_mask >> o) & 1u) val = exp(val);
-        __builtin_nontemporal_store(val, out + o * P);
+        __builtin_nontemporal_store(val, o_row + o * P);
       }
+    }
+    return;
+  }
+#pragma unroll
+  for (int o = 0; o < M; ++o) {
+    if (o < a.nout) {
+      T val = x[o];
+      if constexpr (EF > 0) {
+        if (o < EF) val = exp(val);
+      }
+      __builtin_nontemporal_store(val, o_row + o * P);
     }
   }
 }
 
 template <int MODEL, typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a) {
+__global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __restrict__ coef,
+                                                  const int32_t* __restrict__ store_row,
+                                                  T* __restrict__ out) {
   constexpr int CH = Chunking<M>::CH;
   constexpr int NZ = Chunking<M>::NZ;
   constexpr bool kJump = MODEL == SDEGPU_MODEL_JUMPDIFF;
@@ -112,14 +129,13 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a) {
 #pragma unroll
     for (int v = 0; v < M; ++v) x[v] = (T)a.x0[v];
   }
-  if (a.row0 >= 0) store_state<T, M>(a, a.row0, x, gid);
+  if (a.row0 >= 0) store_state<MODEL, T, M>(out, a, a.row0, x, gid);
 
   int cnt[M];
 #pragma unroll
   for (int v = 0; v < M; ++v) cnt[v] = 0;
   int interval_jumps = 0;
 
-  const T* coef = (const T*)a.coef;
   const bool replay = a.replay_dw != nullptr;
   const bool replay_j = a.replay_dj != nullptr;
 
@@ -255,9 +271,9 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a) {
           x[v] = x[v] + ((p[v] * ds + p[M + v] * dw[v]) + dj[v]);
       }
 
-      const int row = a.store_row[s];
+      const int row = store_row[s];
       if (row >= 0) {
-        store_state<T, M>(a, row, x, gid);
+        store_state<MODEL, T, M>(out, a, row, x, gid);
         if constexpr (kJump) {
           if (a.info_rows && interval_jumps != 0)
             atomicAdd((unsigned long long*)(a.info_rows + row), (unsigned long long)(long long)interval_jumps);
@@ -273,10 +289,11 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a) {
   }
 }
 
-using EulerKernel = void (*)(const EulerArgs);
+template <typename T>
+using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, T*);
 
 template <int MODEL, typename T>
-static EulerKernel pick_m(int m) {
+static EulerKernel<T> pick_m(int m) {
   switch (m) {
     case 1: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 1>;
     case 2: return k_euler<MODEL, T, 2>;
@@ -291,7 +308,7 @@ static EulerKernel pick_m(int m) {
 }
 
 template <typename T>
-static EulerKernel pick(int model, int m) {
+static EulerKernel<T> pick(int model, int m) {
   switch (model) {
     case SDEGPU_MODEL_WIENER: return pick_m<SDEGPU_MODEL_WIENER, T>(m);
     case SDEGPU_MODEL_LOGNORM: return pick_m<SDEGPU_MODEL_LOGNORM, T>(m);
@@ -301,6 +318,14 @@ static EulerKernel pick(int model, int m) {
     case SDEGPU_MODEL_JUMPDIFF: return pick_m<SDEGPU_MODEL_JUMPDIFF, T>(m);
     default: return nullptr;
   }
+}
+
+// exp_mask the model implies for nout stored values (see OutRule).
+static uint32_t expected_exp_mask(int model, int m, int nout) {
+  int ef = 0;
+  if (model == SDEGPU_MODEL_LOGNORM || model == SDEGPU_MODEL_JUMPDIFF) ef = nout;
+  if (model == SDEGPU_MODEL_HESTON) ef = m / 2 < nout ? m / 2 : nout;
+  return ef >= 32 ? 0xFFFFFFFFu : ((1u << ef) - 1u);
 }
 
 static int param_blocks(int model, int m) {
@@ -354,8 +379,10 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   if ((int64_t)pr->nsteps * m > 0xFFFFFFFFll)
     return fail(SDEGPU_EINVAL, "too many steps for the 32-bit Philox block counter");
 
-  EulerKernel k = pr->dtype == SDEGPU_F32 ? pick<float>(pr->model, m) : pick<double>(pr->model, m);
-  if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
+  const uint32_t want_mask = expected_exp_mask(pr->model, m, pr->nout);
+  if (pr->exp_mask != want_mask)
+    return fail(SDEGPU_EINVAL, "exp_mask 0x%x does not match model %d (0x%x expected)",
+                pr->exp_mask, pr->model, want_mask);
 
   EulerArgs a = {};
   a.coef = pr->coef;
@@ -388,6 +415,18 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.kj1 = pr->key_j[1];
   a.ej = pr->epoch_j;
 
-  hipLaunchKernelGGL(k, dim3(blocks_for(pr->paths)), dim3(kBlock), 0, (hipStream_t)stream, a);
+  const dim3 grid(blocks_for(pr->paths)), block(kBlock);
+  hipStream_t st = (hipStream_t)stream;
+  if (pr->dtype == SDEGPU_F32) {
+    EulerKernel<float> k = pick<float>(pr->model, m);
+    if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
+    hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
+                       (float*)pr->out);
+  } else {
+    EulerKernel<double> k = pick<double>(pr->model, m);
+    if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
+    hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
+                       (double*)pr->out);
+  }
   return launch_status("k_euler");
 }

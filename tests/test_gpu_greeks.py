@@ -1,0 +1,120 @@
+"""Greeks kernels against the oracle (GPU tests).
+
+* FP64 adjoint kernel == oracle FP64 adjoint on the same Philox paths
+  (per-path sums; 1e-10 relative);
+* FP32 adjoint kernel vs the FP64 oracle (stated tolerance);
+* FP64 forward + CRN finite differences on the GPU == the oracle's finite
+  differences of sdepy's restated loop;
+* the price of the adjoint kernel == the forward Euler kernel's price;
+* torch.autograd surface.
+"""
+import numpy as np
+import pytest
+
+import sdepy_amd as sa
+from sdepy_amd import greeks as G
+from oracle import greeks as og
+
+pytestmark = pytest.mark.gpu
+
+KEY = (0x0badcafe, 0x12345678)
+HPRM = dict(x0=100., r=.02, k=2., theta=.04, xi=.5, rho=-.7, y0=.04)
+
+
+def _sums(pay, g):
+    return np.concatenate(([pay.sum(), (pay * pay).sum()],
+                           np.stack((g.sum(axis=1), (g * g).sum(axis=1)), axis=1).reshape(-1)))
+
+
+@pytest.mark.parametrize('K', [100., 0.])
+def test_heston_adjoint_fp64_matches_oracle(gpu, K):
+    P, n = 4096, 60
+    run = G.HestonGreeks(paths=P, steps=n, strike=K, dtype=np.float64, key=KEY,
+                         epoch=3, **HPRM)
+    run.launch()
+    res = run.result()
+    pay, g = og.heston_adjoint(KEY, 3, P, n, 1., strike=K, **HPRM)
+    got = np.concatenate(([res.price], [res.greeks[k] for k in og.HESTON_NAMES]))
+    ref = np.concatenate(([pay.mean()], g.mean(axis=1)))
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12 * max(1., abs(ref).max()))
+    assert np.isclose(res.price_se, pay.std(ddof=0) / np.sqrt(P - 1), rtol=1e-8)
+
+
+def test_heston_adjoint_fp32_vs_fp64_oracle(gpu):
+    P, n = 8192, 100
+    res = G.heston_greeks(paths=P, steps=n, strike=100., dtype=np.float32,
+                          key=KEY, epoch=0, **HPRM)
+    pay, g = og.heston_adjoint(KEY, 0, P, n, 1., strike=100., **HPRM)
+    assert abs(res.price - pay.mean()) < 2e-4 * abs(pay.mean())
+    for i, name in enumerate(og.HESTON_NAMES):
+        ref = g[i].mean()
+        se = g[i].std() / np.sqrt(P)
+        # FP32 rounding of the state is far below the sampling error
+        assert abs(res.greeks[name] - ref) < 0.05 * se + 1e-3 * abs(ref), name
+
+
+def test_heston_price_matches_forward_kernel_and_fd(gpu):
+    """Adjoint-kernel price == forward Euler kernel price on the same key, and
+    the GPU FP64 forward + CRN finite differences == the oracle's."""
+    P, n = 3000, 40
+    fd = G.HestonFD(paths=P, steps=n, strike=100., key=KEY, epoch=1,
+                    dtype=np.float64, h=1e-4, **HPRM)
+    fd.launch()
+    price, se, greeks = fd.result()
+    res = G.heston_greeks(paths=P, steps=n, strike=100., dtype=np.float64,
+                          key=KEY, epoch=1, **HPRM)
+    assert abs(price - res.price) < 1e-10 * price
+    ref_fd = og.heston_fd(KEY, 1, P, n, 1., strike=100., h=1e-4, **HPRM)
+    ref_p = og.heston_price(KEY, 1, P, n, 1., strike=100., **HPRM)
+    assert abs(price - ref_p) < 1e-10 * ref_p
+    for name in og.HESTON_NAMES:
+        assert abs(greeks[name] - ref_fd[name]) < 1e-6 * max(1., abs(ref_fd[name])), name
+
+
+def test_heston_autograd(gpu):
+    torch = gpu
+    cfg = dict(paths=4096, steps=30, strike=100., dtype=np.float32, key=KEY,
+               epoch=0)
+    p = torch.tensor([HPRM[k] for k in G.HESTON_GREEKS], dtype=torch.float64,
+                     requires_grad=True)
+    price = G.heston_price(p, **cfg)
+    price.backward()
+    res = G.heston_greeks(**cfg, **HPRM)
+    assert abs(price.item() - res.price) < 1e-9 * res.price
+    assert np.allclose(p.grad.numpy(), [res.greeks[k] for k in G.HESTON_GREEKS],
+                       rtol=1e-12)
+    with torch.no_grad():
+        p2 = p.detach()
+        price2 = G.heston_price(p2, **cfg)
+    assert abs(price2.item() - res.price) < 1e-4 * res.price
+
+
+@pytest.mark.parametrize('lam', [0., 3.])
+def test_loglin_greeks_match_oracle(gpu, lam):
+    P, n = 4096, 50
+    kw, kj = (1, 2), (3, 4)
+    prm = dict(x0=1., mu=.03, sigma=.2, strike=1., r=.03)
+    jp = dict(lam=lam, a=-.1, b=.15) if lam else {}
+    res = G.merton_greeks(paths=P, steps=n, dtype=np.float64, key_w=kw, key_j=kj,
+                          epoch=2, maturity=1., **prm, **jp)
+    pay, g = og.loglin_adjoint(kw, kj, 2, P, n, 1., **prm, **jp)
+    assert abs(res.price - pay.mean()) < 1e-10 * pay.mean()
+    for i, name in enumerate(og.LOG_NAMES):
+        ref = g[i].mean()
+        assert abs(res.greeks[name] - ref) < 1e-10 * max(1e-3, abs(ref)), name
+    # the price is the forward estimator of the process class on the same keys
+    ref_p = og.loglin_price(kw, kj, 2, P, n, 1., **prm, **jp)
+    assert abs(res.price - ref_p) < 1e-10 * ref_p
+
+
+def test_lognorm_delta_matches_black_scholes(gpu):
+    """Pathwise delta of the exact log-Euler scheme vs bscall_delta
+    (analytical.py:984-996), within 3 standard errors at 1M paths."""
+    from oracle import closed_forms as cf
+    res = G.lognorm_greeks(paths=1_000_000, steps=12, x0=1., mu=.05, sigma=.3,
+                           strike=1.1, r=.05, maturity=2., dtype=np.float32,
+                           key_w=(5, 6), key_j=(7, 8))
+    bs = cf.bscall(1.1, 2., 1., .05, 0., .3)
+    d = cf.bscall_delta(1.1, 2., 1., .05, 0., .3)
+    assert abs(res.price - bs) < 3 * res.price_se
+    assert abs(res.greeks['x0'] - d) < 3 * res.greeks_se['x0']
