@@ -50,16 +50,18 @@ def _events(torch):
             torch.cuda.Event(enable_timing=True))
 
 
-def timed(torch, fn, steps, warmup):
+def timed(torch, fn, steps, warmup, stream=None):
+    """Mean time of fn() in ms from HIP events recorded on the stream the
+    kernels are launched on (torch's current stream unless given)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     e0, e1 = _events(torch)
     t0 = time.perf_counter()
-    e0.record()
+    e0.record(stream)
     for _ in range(steps):
         fn()
-    e1.record()
+    e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return e0.elapsed_time(e1) / steps, wall
@@ -125,8 +127,10 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--legs', default='adjoint,fd,ou,cpu',
-                    help='comma list of: adjoint, fd, ou, cpu')
+    ap.add_argument('--legs', default='adjoint,fd,ou,batched,cpu',
+                    help='comma list of: adjoint, fd, ou, batched, cpu, copyref '
+                         '(a 1 GiB device copy used to calibrate rocprof byte '
+                         'counters)')
     ap.add_argument('--cpu-paths', type=int, default=30000)
     args = ap.parse_args()
     legs = set(args.legs.split(','))
@@ -140,6 +144,16 @@ def main():
     from sdepy_amd import _lib, greeks as G
     torch = _lib.require_gpu()
     torch.cuda.set_device(0)
+
+    if 'copyref' in legs:
+        # a plain copy of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE of this
+        # kernel read 1 GiB each when the counters are exact
+        src = torch.empty(1 << 28, dtype=torch.float32, device='cuda').fill_(1.)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- headline: FP32 forward + adjoint, price + 7 Greeks ----
     if 'adjoint' not in legs:
@@ -216,6 +230,26 @@ def main():
                          'frac': gbs / HBM_PEAK_GBS, 'traffic': None,
                          'bytes_per_path_step': 16}}
         del ou, p
+        torch.cuda.empty_cache()
+
+    # ---- 1024 heterogeneous GBM / 2-factor Hull-White instances ----
+    if 'batched' in legs:
+        from sdepy_amd import batched as B
+        specs = B.heterogeneous_specs(1024, seed=1234)
+        t0 = time.perf_counter()
+        bt = B.BatchedTerminalMoments(specs, autotune=True, graph=True)
+        prep = time.perf_counter() - t0
+        bms, _ = timed(torch, bt.launch, 3, 1, stream=bt.stream)
+        mean, var = bt.result()
+        out['batched'] = {
+            'workload': 'c5_1024_gbm_hw2f_heterogeneous_fp32',
+            'instances': bt.n, 'tiles': bt.n_items,
+            'path_steps': bt.path_steps, 'ms': bms,
+            'path_steps_per_s': bt.path_steps / (bms * 1e-3),
+            'blocks': bt.blocks, 'cu': bt.cu,
+            'autotune_s': {str(k): v for k, v in bt.tuning.items()},
+            'prepare_s': prep, 'graph': bt.graph is not None}
+        del bt
         torch.cuda.empty_cache()
 
     if 'cpu' in legs:

@@ -240,6 +240,65 @@ int sdegpu_loglin_greeks_run(const sdegpu_loglin_greeks* g, double* out, void* w
                              size_t workspace_bytes, void* stream);
 
 /*
+ * Many independent simulations in one persistent launch.
+ *
+ * sdepy has no batching: a user runs the presets one after the other
+ * (each a paths_generator._generate_paths loop, integration.py:392-474) and
+ * reduces each terminal value with process.pmean/pvar (infrastructure.py:
+ * 861-879).  Here every instance (a lognorm_process, wiener_process,
+ * ornstein_uhlenbeck_process or hull_white_process with vshape ()) keeps its
+ * own path count, FP32 step table (row layout of sdegpu_euler_problem.coef),
+ * parameters and Philox key/epoch -- so it draws exactly what sdegpu_euler
+ * would draw for it -- and the batch returns, per instance, the FP64 mean and
+ * sum of squared deviations of the terminal stored value (exp of the state
+ * for LOGNORM, the sum of the factors for OU).
+ *
+ * The host cuts each instance into 64-path tiles, lists them in the order
+ * they should be claimed (longest first), and gives each instance the index
+ * of its first partial slot; the persistent grid's wavefronts claim tiles
+ * through `counter` (reset by the launch itself), write per-tile partial sums
+ * to their fixed slots, and a final kernel merges each instance's slots in
+ * order.  Deterministic.
+ */
+typedef struct sdegpu_batch_instance {
+  int32_t model;       /* SDEGPU_MODEL_LOGNORM, SDEGPU_MODEL_WIENER or SDEGPU_MODEL_OU */
+  int32_t nvars;       /* 1 (LOGNORM, WIENER, OU) or 2..4 OU factors summed on output */
+  int32_t has_corr;    /* step rows carry a packed Cholesky factor */
+  int32_t nsteps;
+  int64_t paths;
+  int64_t coef_offset; /* first element of this instance's step table in coef */
+  int32_t ncoef;
+  uint32_t epoch;
+  uint32_t key[2];
+  double x0[4];        /* initial working state (log x0 for LOGNORM) */
+  double shift;        /* subtracted from terminal values before the FP64 sums */
+} sdegpu_batch_instance;
+
+typedef struct sdegpu_batch {
+  const sdegpu_batch_instance* inst; /* device [n_inst] */
+  int32_t n_inst;
+  int32_t n_items;                   /* number of tiles */
+  const int32_t* work;               /* device [n_items][2] = (instance, tile), claim order */
+  const int64_t* slot0;              /* device [n_inst]: first partial slot of each instance */
+  const float* coef;                 /* device: concatenated FP32 step tables */
+  double* part;                      /* device [n_items][2] workspace */
+  unsigned int* counter;             /* device, one word */
+  double* mean;                      /* device [n_inst] */
+  double* m2;                        /* device [n_inst]: sum of squared deviations */
+  int32_t blocks;                    /* persistent grid: blocks of 256 threads */
+} sdegpu_batch;
+
+int sdegpu_batch_run(const sdegpu_batch* batch, void* stream);
+/* Capture the batch launch (counter reset, persistent kernel, merge) into a
+ * HIP graph on `stream` (which must not be the null stream); replay it with
+ * sdegpu_batch_graph_launch. */
+int sdegpu_batch_graph_create(const sdegpu_batch* batch, void* stream, void** graph_exec);
+int sdegpu_batch_graph_launch(void* graph_exec, void* stream);
+int sdegpu_batch_graph_destroy(void* graph_exec);
+/* Compute units of the current device (sizes the persistent grid). */
+int sdegpu_cu_count(int* count);
+
+/*
  * Philox4x32-10 evaluation, for known-answer tests.
  * sdegpu_philox_host runs on the CPU (no GPU needed); sdegpu_philox_fill runs
  * the device twin over n counters (device uint32 [n][4] in, [n][4] out).

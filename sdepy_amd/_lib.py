@@ -149,13 +149,12 @@ def stream():
     return ctypes.c_void_p(torch_mod().cuda.current_stream().cuda_stream)
 
 
-_NP2TORCH = None
-
-
 def to_device(a):
     """Copy a NumPy array to the current GPU; returns the torch tensor."""
     torch = require_gpu()
     a = np.ascontiguousarray(a)
+    if not a.flags.writeable:
+        a = a.copy()
     return torch.from_numpy(a).to('cuda', non_blocking=False)
 
 

@@ -1,8 +1,8 @@
 """The ``process`` array type and the stochasticity sources of sdepy_amd.
 
 Drop-in counterparts of sdepy's ``process`` (sdepy/infrastructure.py:272-1209)
-and memoryless sources ``wiener_source`` (:1392-1560), ``poisson_source``
-(:1566-1633) and ``cpoisson_source`` (:1846-2040), with the same constructor
+and memoryless sources ``wiener_source`` (:1388-1560), ``poisson_source``
+(:1566-1633) and ``cpoisson_source`` (:1881-2040), with the same constructor
 keywords, shapes and call protocol.  What differs is where the numbers come
 from: the sources draw from a counter-based Philox generator on the GPU,
 keyed once from the caller's NumPy ``rng``, and ``process.pmean/pvar/psum/
@@ -188,7 +188,7 @@ def _gpu_dtype(dtype):
 
 class wiener_source(_philox_source):
     """dw, standard Wiener process increments (sdepy/infrastructure.py:
-    1392-1560), drawn on the GPU: ``dw(t, dt) = L z sqrt(|dt|)`` with z
+    1388-1560), drawn on the GPU: ``dw(t, dt) = L z sqrt(|dt|)`` with z
     Philox normals and L the FP64 Cholesky factor of ``corr`` (or of the
     matrix built from ``rho``), evaluated at ``t + dt/2`` when callable."""
 
@@ -354,7 +354,7 @@ def norm_rv(a=0., b=1.):
 
 
 class cpoisson_source(_philox_source):
-    """dj, compound Poisson increments (sdepy/infrastructure.py:1846-2040).
+    """dj, compound Poisson increments (sdepy/infrastructure.py:1881-2040).
 
     With ``y = norm_rv(a, b)`` (the Merton case) both the Poisson counts and
     the jump sizes are drawn on the GPU.  Any other ``y`` exposing
@@ -466,7 +466,7 @@ def _uniform_default():
 
 
 def uniform_rv(a=0., b=1.):
-    """Uniform jump sizes on [a, b] (sdepy/infrastructure.py:1667-1677);
+    """Uniform jump sizes on [a, b] (sdepy/infrastructure.py:1667-1677, 1723-1728);
     sampled on the host through ``rvs``."""
     return _uniform_rv(a, b)
 

@@ -1,0 +1,250 @@
+"""Host-side logic and the C ABI boundary (CPU tests, no GPU compute).
+
+* libsdegpu.so loads and exports every function include/sdegpu.h declares;
+* the ctypes mirrors of the C structs have the C compiler's layout;
+* argument validation of the entry points (host code, before any launch)
+  returns the documented status codes, mapped to sdepy's exception types;
+* timeline consolidation, store-row mapping, parameter tables and argument
+  handling of the preset classes agree with the oracle's restatement;
+* the process type behaves as sdepy's (construction, ufuncs, indexing,
+  host summaries);
+* without a GPU every simulation entry point fails loudly.
+"""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import sdepy_amd as sa
+from sdepy_amd import _lib, greeks as G
+from sdepy_amd import integration as integ
+from sdepy_amd.infrastructure import per_component, philox_key, cholesky_packed
+from oracle import euler as ora
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'sdegpu.h')
+
+
+def _declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
+    return sorted(set(re.findall(r'\b(sdegpu_\w+)\s*\(', src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) >= 17
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the ctypes bindings give every one of them a prototype
+    from sdepy_amd import batched
+    bound = _lib.lib()
+    G._bind()
+    batched._bind()
+    for n in names:
+        assert getattr(bound, n).argtypes is not None, n
+
+
+def _c_layout(struct, fields):
+    code = '#include <stdio.h>\n#include <stddef.h>\n#include "sdegpu.h"\nint main(){\n'
+    code += 'printf("%zu\\n", sizeof({}));\n'.format(struct)
+    for f in fields:
+        code += 'printf("%zu\\n", offsetof({}, {}));\n'.format(struct, f)
+    code += 'return 0;}\n'
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, 'l.c')
+        exe = os.path.join(td, 'l')
+        open(c, 'w').write(code)
+        subprocess.run(['gcc', '-I', os.path.dirname(HEADER), c, '-o', exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    return [int(v) for v in out.split()]
+
+
+@pytest.mark.parametrize('cstruct,pystruct', [
+    ('sdegpu_euler_problem', _lib.EulerProblem),
+    ('sdegpu_heston_greeks', G.HestonGreeksProblem),
+    ('sdegpu_loglin_greeks', G.LogGreeksProblem),
+])
+def test_ctypes_layout_matches_c(cstruct, pystruct):
+    fields = [f[0] for f in pystruct._fields_]
+    got = _c_layout(cstruct, fields)
+    assert got[0] == ctypes.sizeof(pystruct)
+    for f, off in zip(fields, got[1:]):
+        assert getattr(pystruct, f).offset == off, f
+
+
+def test_philox_host_entry_known_answer():
+    c = (ctypes.c_uint32 * 4)(0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344)
+    k = (ctypes.c_uint32 * 2)(0xa4093822, 0x299f31d0)
+    o = (ctypes.c_uint32 * 4)()
+    _lib.check(_lib.lib().sdegpu_philox_host(c, k, o))
+    assert list(o) == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_entry_point_validation_maps_to_exceptions():
+    lib = _lib.lib()
+    pr = _lib.EulerProblem()
+    pr.model, pr.dtype, pr.nvars, pr.paths = _lib.MODEL_LOGNORM, _lib.F32, 9, 10
+    with pytest.raises(NotImplementedError):
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.nvars, pr.nout, pr.ncoef = 1, 1, 1
+    with pytest.raises(ValueError):        # table too narrow
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.ncoef, pr.exp_mask = 4, 0           # lognorm must exp its output
+    pr.out = 1
+    with pytest.raises(ValueError):
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.model, pr.nvars = _lib.MODEL_HESTON, 3
+    with pytest.raises(ValueError):        # odd Heston state
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.model, pr.nvars, pr.paths = _lib.MODEL_OU, 1, 0
+    with pytest.raises(ValueError):
+        _lib.check(lib.sdegpu_euler(pr, None))
+    with pytest.raises(ValueError):
+        _lib.check(lib.sdegpu_path_moments(None, 0, -1, 10, None, None, None, 0, None))
+    assert lib.sdegpu_moments_workspace(3, 70000) == 3 * 2 * 2 * 8
+    with pytest.raises(MemoryError):
+        _lib.check(lib.sdegpu_path_moments(ctypes.c_void_p(8), 0, 3, 70000,
+                                           ctypes.c_void_p(8), ctypes.c_void_p(8),
+                                           ctypes.c_void_p(8), 8, None))
+
+
+@pytest.mark.parametrize('tt,steps,i0', [
+    (np.linspace(0, 1, 11), None, 0), (np.linspace(0, 1, 11), 37, 0),
+    ((0., .3, 1.), (.1, .2, .25, .9, 5.), 1), (np.arange(10), 37, 0),
+    ((2.,), None, 0), ((0., 1., 2., 3.), 8, -1),
+])
+def test_timeline_consolidation_matches_oracle(tt, steps, i0):
+    a = integ.consolidate_timeline(tt, steps, i0)
+    b = ora.consolidate(tt, steps, i0)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y) and x.dtype == y.dtype
+
+
+@pytest.mark.parametrize('bad,exc', [
+    (np.ones((2, 3)), ValueError), ((1, 2, 2, 3), ValueError),
+    ((3, 2, 1), ValueError)])
+def test_timeline_errors(bad, exc):
+    with pytest.raises(exc):
+        integ.consolidate_timeline(bad, None, 0)
+    with pytest.raises(IndexError):
+        integ.consolidate_timeline((1., 2.), None, 12)
+
+
+def test_store_rows_forward_and_backward():
+    tt = np.array([0., .3, .5, 1., 1.7, 2.])
+    _, _, steps = integ.consolidate_timeline(tt, 23, 3)
+    j0 = int(np.nonzero(steps == tt[3])[0][0])
+    fwd = integ._store_rows(steps[j0:], tt[3:], lambda j: 3 + j)
+    bwd = integ._store_rows(steps[j0::-1], tt[3::-1], lambda j: 3 - j)
+    # brute force
+    for sp, tp, rows, sgn in ((steps[j0:], tt[3:], fwd, 1), (steps[j0::-1], tt[3::-1], bwd, -1)):
+        for s, r in enumerate(rows):
+            hit = np.nonzero(tp == sp[s + 1])[0]
+            assert r == (3 + sgn * hit[0] if hit.size else -1)
+    assert set(fwd[fwd >= 0]) == {4, 5} and set(bwd[bwd >= 0]) == {0, 1, 2}
+
+
+def test_per_component_and_per_path_rejection():
+    assert np.array_equal(per_component(np.array([[1.], [2.]]), (2,), 'k'), [1., 2.])
+    assert np.array_equal(per_component(3., (2, 2), 'k'), [3.] * 4)
+    with pytest.raises(NotImplementedError):
+        per_component(np.arange(5.), (), 'x')       # a value per path
+
+
+def test_philox_key_reproducible():
+    assert philox_key(np.random.default_rng(3)) == philox_key(np.random.default_rng(3))
+    assert philox_key(np.random.RandomState(3)) == philox_key(np.random.RandomState(3))
+    with pytest.raises(TypeError):
+        sa.wiener_source(rng=5)
+
+
+def test_cholesky_packing_block_diagonal():
+    c = np.array([[1, .3, .1], [.3, 1, .2], [.1, .2, 1]])
+    L = np.linalg.cholesky(c)
+    packed = cholesky_packed(c, (2, 3))
+    full = np.zeros((6, 6))
+    full[np.tril_indices(6)] = packed
+    assert np.allclose(full[:3, :3], L) and np.allclose(full[3:, 3:], L)
+    assert np.allclose(full[3:, :3], 0)
+    with pytest.raises(ValueError):
+        cholesky_packed(c, (2,))
+
+
+def test_preset_argument_handling_and_tables():
+    with pytest.raises(TypeError):
+        sa.lognorm_process(paths=3, bogus=1)
+    with pytest.raises(ValueError):
+        sa.lognorm_process(paths=3, method='rk')
+    p = sa.lognorm_process(paths=4, mu=lambda t: .1 * t, sigma=.3)
+    rows = p._param_rows(np.array([0., 1., 2.]))
+    assert np.allclose(rows[:, 0], .1 * np.array([0., 1., 2.]) - .045)
+    assert np.allclose(rows[:, 1], .3)
+    h = sa.hull_white_process(paths=4, factors=3, rho=None)
+    assert h.wshape == (3,) and h.xshape == ()
+    assert h._output_map()[:3] == (1, _lib.OUT_SUM, 3)
+    fh = sa.full_heston_process(paths=4, vshape=2, rho=[.1, .2])
+    assert fh.wshape == (4,) and fh.sources['dw'].corr.shape == (4, 4)
+    st = fh._x0_state(np.float64)
+    assert st.shape == (4,) and np.allclose(st, [0, 0, 1, 1])
+    he = sa.heston_process(paths=4)
+    assert he.xshape == () and he._output_map() == (1, _lib.OUT_FIRST, 1, 1)
+    m = sa.merton_jumpdiff_process(paths=4, lam=2., a=-.1, b=.2)
+    assert m.sources['dj'].gpu_jumps
+    x0 = sa.ornstein_uhlenbeck_process(paths=3, x0=np.array([1., 2., 3.]))
+    assert x0._x0_state(np.float64).shape == (1, 3)     # per-path initial values
+
+
+def test_process_type_semantics():
+    t = np.array([0., 1., 2.])
+    x = np.arange(3 * 2 * 5, dtype=float).reshape(3, 2, 5)
+    p = sa.process(t, x=x)
+    assert p.paths == 5 and p.vshape == (2,) and p.t is not None
+    q = np.exp(p)
+    assert isinstance(q, sa.process) and np.array_equal(q.t, t)
+    r = p + p
+    assert isinstance(r, sa.process)
+    with pytest.raises(ValueError):
+        p + sa.process((0., 5., 6.), x=x)
+    s = p.sum()
+    assert not isinstance(s, sa.process)
+    assert isinstance(p.mean(axis=-1), np.ndarray) and not isinstance(p.mean(axis=-1), sa.process)
+    assert p['t', 1].shape == (1, 2, 5) and p['t', 1].t[0] == 1.
+    assert p['v', 1].shape == (3, 5)
+    assert p['p', 2].shape == (3, 2, 1)
+    assert np.array_equal(p[0], x[0])
+    c = sa.process(c=[1., 2.])
+    assert c.shape == (1, 2, 1)
+    v = sa.process(t, v=[1., 2., 3.])
+    assert v.shape == (3, 1)
+    f = p.interp()
+    assert np.allclose(f(.5), (x[0] + x[1]) / 2)
+    assert np.allclose(p.rebase((.5, 1.5))[0], (x[0] + x[1]) / 2)
+    assert np.allclose(p.tmean()[0], x.mean(axis=0))
+    assert np.allclose(p.vmean(), x.mean(axis=1))
+    assert np.allclose(p.pmin()[..., 0], x.min(axis=-1))
+    assert np.allclose(p.pcopy(), p) and p.pcopy().t is not p.t
+
+
+@pytest.mark.skipif(_lib.torch_mod().cuda.is_available(), reason='a GPU is present')
+def test_no_gpu_fails_loudly():
+    with pytest.raises(RuntimeError):
+        sa.lognorm_process(paths=3)(np.linspace(0, 1, 3))
+    with pytest.raises(RuntimeError):
+        sa.process(np.arange(2.), x=np.ones((2, 3))).pmean()
+    with pytest.raises(RuntimeError):
+        sa.wiener_source(paths=3)(0., 1.)
+    with pytest.raises(RuntimeError):
+        G.heston_greeks(paths=10, steps=3)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    res = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2'],
+                         capture_output=True, text=True)
+    assert res.returncode == 2 and 'one GPU' in res.stderr
