@@ -77,7 +77,9 @@ extern "C" {
  *   then the model parameters, each a block of M (or N = M/2 for Heston):
  *     WIENER   mu[M], sigma[M]
  *     LOGNORM  mu - sigma^2/2 [M], sigma[M]
- *     OU       theta[M], k[M], sigma[M]
+ *     OU       1 - k ds [M], k theta ds [M], sigma[M]; when has_corr, row i of the
+ *              Cholesky factor is pre-multiplied by sigma_i sqrt(|ds|), so the
+ *              update is x = (1 - k ds) x + (k theta ds + sum_j L'_ij z_j)
  *     CIR      theta[M], k[M], xi[M]
  *     HESTON   mu[N], sigma[N], theta[N], k[N], xi[N]   (state = [log x[N], y[N]])
  *     JUMPDIFF mu - sigma^2/2 [M], sigma[M], a[M], b[M]  (jumps a + b*N(0,1))

@@ -104,8 +104,7 @@ def _instance(proc, timeline):
                                   'increments (dw must be a wiener_source)')
     tt, _, steps_tt = consolidate_timeline(timeline, proc.steps, 0)
     t0s, ds = steps_tt[:-1], np.diff(steps_tt)
-    wtab, has_corr = dw.step_table(t0s, ds)
-    tab = np.concatenate((wtab, proc._param_rows(t0s)), axis=1)
+    tab, has_corr = proc._coef_table(dw, t0s, ds)
     x0 = proc._x0_state(np.float32)
     if x0.ndim != 1:
         raise NotImplementedError('batched simulations take an initial value '

@@ -68,7 +68,7 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
           float acc = 0.f;
 #pragma unroll
           for (int j = 0; j <= i; ++j) acc = acc + L[idx++] * z[k * M + j];
-          dw[i] = acc * sq;
+          dw[i] = MODEL == SDEGPU_MODEL_OU ? acc : acc * sq;
         }
       } else {
 #pragma unroll
@@ -76,10 +76,13 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
       }
       const float* p = c + poff;
       if constexpr (MODEL == SDEGPU_MODEL_OU) {
+        // rows: a = 1 - k ds, b = k theta ds, sigma; correlated draws carry
+        // sigma sqrt|ds| (folded into the Cholesky rows by the host)
+        const bool folded = M > 1 && d.has_corr;
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-          const float adt = p[M + v] * (p[v] - x[v]);
-          x[v] = x[v] + (adt * ds + p[2 * M + v] * dw[v]);
+          const float inc = folded ? dw[v] : p[2 * M + v] * dw[v];
+          x[v] = p[v] * x[v] + (p[M + v] + inc);
         }
       } else {
 #pragma unroll
@@ -139,22 +142,40 @@ __global__ void k_batch_reset(unsigned int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
 }
 
-__global__ void k_batch_finish(const sdegpu_batch_instance* __restrict__ inst,
-                               const int64_t* __restrict__ slot0, const double* __restrict__ part,
-                               int32_t n_inst, double* __restrict__ mean, double* __restrict__ m2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_inst) return;
+// One block per instance: the tile partials are summed in a fixed order
+// (strided per thread, then a shared-memory tree), so the result does not
+// depend on scheduling.
+__global__ __launch_bounds__(kBlock) void k_batch_finish(
+    const sdegpu_batch_instance* __restrict__ inst, const int64_t* __restrict__ slot0,
+    const double* __restrict__ part, int32_t n_inst, double* __restrict__ mean,
+    double* __restrict__ m2) {
+  __shared__ double sh1[kBlock], sh2[kBlock];
+  const int i = blockIdx.x;
   const int64_t ntiles = (inst[i].paths + kTile - 1) / kTile;
+  const int64_t base = slot0[i];
   double s1 = 0.0, s2 = 0.0;
-  for (int64_t t = 0; t < ntiles; ++t) {
-    s1 += part[2 * (slot0[i] + t)];
-    s2 += part[2 * (slot0[i] + t) + 1];
+  for (int64_t t = threadIdx.x; t < ntiles; t += kBlock) {
+    s1 += part[2 * (base + t)];
+    s2 += part[2 * (base + t) + 1];
   }
-  const double n = (double)inst[i].paths;
-  const double dm = s1 / n;
-  mean[i] = inst[i].shift + dm;
-  const double v = s2 - s1 * dm;
-  m2[i] = v > 0.0 ? v : 0.0;
+  sh1[threadIdx.x] = s1;
+  sh2[threadIdx.x] = s2;
+  __syncthreads();
+#pragma unroll
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sh1[threadIdx.x] += sh1[threadIdx.x + w];
+      sh2[threadIdx.x] += sh2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = (double)inst[i].paths;
+    const double dm = sh1[0] / n;
+    mean[i] = inst[i].shift + dm;
+    const double v = sh2[0] - sh1[0] * dm;
+    m2[i] = v > 0.0 ? v : 0.0;
+  }
 }
 
 static int check_batch(const sdegpu_batch* b) {
@@ -180,8 +201,8 @@ static int launch_batch(const sdegpu_batch* b, hipStream_t st) {
   a.n_items = b->n_items;
   hipLaunchKernelGGL(k_batch_persistent, dim3(b->blocks), dim3(kBlock), 0, st, a);
   if (int e = launch_status("k_batch_persistent")) return e;
-  hipLaunchKernelGGL(k_batch_finish, dim3(blocks_for(b->n_inst, 64)), dim3(64), 0, st, b->inst,
-                     b->slot0, (const double*)b->part, b->n_inst, b->mean, b->m2);
+  hipLaunchKernelGGL(k_batch_finish, dim3(b->n_inst), dim3(kBlock), 0, st, b->inst, b->slot0,
+                     (const double*)b->part, b->n_inst, b->mean, b->m2);
   return launch_status("k_batch_finish");
 }
 

@@ -168,6 +168,9 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
       const T* c = coef + (int64_t)s * a.ncoef;
       const T ds = c[0];
 
+      // OU: the host folds sigma_i * sqrt|ds| into the Cholesky rows, so the
+      // correlated draw is already the diffusion increment sigma * dw.
+      constexpr bool kFold = MODEL == SDEGPU_MODEL_OU;
       T dw[M];
       if (replay) {
         const T* rp = (const T*)a.replay_dw + (int64_t)s * M * P + gid;
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
             T acc = (T)0;
 #pragma unroll
             for (int j = 0; j <= i; ++j) acc = acc + L[idx++] * z[k * M + j];
-            dw[i] = acc * sq;
+            dw[i] = kFold ? acc : acc * sq;
           }
         } else {
 #pragma unroll
@@ -197,11 +200,14 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 #pragma unroll
         for (int v = 0; v < M; ++v) x[v] = x[v] + (p[v] * ds + p[M + v] * dw[v]);
       } else if constexpr (MODEL == SDEGPU_MODEL_OU) {
-        // ornstein_uhlenbeck_SDE: {'dt': k*(theta - x), 'dw': sigma}
+        // ornstein_uhlenbeck_SDE: {'dt': k*(theta - x), 'dw': sigma}, i.e.
+        // x + k (theta - x) ds + sigma dw = (1 - k ds) x + (k theta ds + sigma dw)
+        // with a = 1 - k ds, b = k theta ds precomputed per step on the host.
+        const bool folded = a.has_corr && !replay;
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-          const T adt = p[M + v] * (p[v] - x[v]);
-          x[v] = x[v] + (adt * ds + p[2 * M + v] * dw[v]);
+          const T inc = folded ? dw[v] : p[2 * M + v] * dw[v];
+          x[v] = p[v] * x[v] + (p[M + v] + inc);
         }
       } else if constexpr (MODEL == SDEGPU_MODEL_CIR) {
         // cox_ingersoll_ross_SDE: x+ = max(x, 0); {'dt': k*(theta - x+), 'dw': xi*sqrt(x+)}

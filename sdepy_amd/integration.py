@@ -235,10 +235,21 @@ class _gpu_sde:
             return np.ascontiguousarray(self._init_state(1, cdt)[:, 0])
         return np.ascontiguousarray(self._init_state(self.paths, cdt))
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
         """Model parameter block of each step row (FP64), evaluated at the
-        step start times t0s."""
+        step start times t0s (steps ds)."""
         raise NotImplementedError
+
+    def _coef_table(self, dw, t0s, ds):
+        """FP64 step table [ds, sqrt|ds|, packed Cholesky factor (if any),
+        model parameters] for an in-kernel Wiener source; (table, has_corr)."""
+        wtab, has_corr = dw.step_table(t0s, ds)
+        params = self._param_rows(t0s, ds)
+        wtab = self._fold(wtab, params, has_corr)
+        return np.concatenate((wtab, params), axis=1), has_corr
+
+    def _fold(self, wtab, params, has_corr):
+        return wtab
 
     def _output_map(self):
         """(nout, out_mode, out_group, exp_mask)."""
@@ -318,12 +329,13 @@ class _gpu_sde:
         dw = self.sources['dw']
         host_sources = {}
         if isinstance(dw, wiener_source):
-            wtab, has_corr = dw.step_table(t0s, ds)
+            tab, has_corr = self._coef_table(dw, t0s, ds)
             prob.key_w[0], prob.key_w[1] = dw.key
             prob.epoch_w = dw.next_epoch()
         else:
             wtab = np.stack((ds, np.sqrt(np.abs(ds))), axis=1) if nsteps \
                 else np.zeros((0, 2))
+            tab = np.concatenate((wtab, self._param_rows(t0s, ds)), axis=1)
             has_corr = False
             host_sources['dw'] = dw
         dj = self.sources.get('dj')
@@ -337,15 +349,11 @@ class _gpu_sde:
             else:
                 host_sources['dj'] = dj
                 pois = ab = None
-        self._host_dj = 'dj' in host_sources
-        prob.has_corr = int(has_corr)
-        params = self._param_rows(t0s)
-        if dj is not None:
             extra = np.zeros((nsteps, 2 * M)) if ab is None else \
                 np.concatenate((ab[:, :, 0], ab[:, :, 1]), axis=1)
-            params = np.concatenate((params, extra), axis=1)
-        tab = np.concatenate((wtab, params), axis=1) if nsteps else \
-            np.zeros((0, wtab.shape[1] + params.shape[1]))
+            tab = np.concatenate((tab, extra), axis=1)
+        self._host_dj = 'dj' in host_sources
+        prob.has_corr = int(has_corr)
         prob.ncoef = tab.shape[1]
         if nsteps:
             d_tab = _lib.to_device(tab.astype(cdt))
@@ -467,7 +475,7 @@ class wiener_process(_gpu_sde):
     init_defaults = {'x0': 0.}
     sde_defaults = {'mu': 0., 'sigma': 1.}
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
         return np.concatenate((self._eval('mu', t0s, self.wshape),
                                self._eval('sigma', t0s, self.wshape)), axis=1)
 
@@ -483,7 +491,7 @@ class lognorm_process(_gpu_sde):
     model = _lib.MODEL_LOGNORM
     log = True
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
         mu = self._eval('mu', t0s, self.wshape)
         sigma = self._eval('sigma', t0s, self.wshape)
         return np.concatenate((mu - sigma * sigma / 2, sigma), axis=1)
@@ -500,11 +508,29 @@ class ornstein_uhlenbeck_process(_gpu_sde):
     init_defaults = {'x0': 0.}
     sde_defaults = {'theta': 0., 'k': 1., 'sigma': 1.}
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
+        # rows (1 - k ds, k theta ds, sigma): x + k (theta - x) ds + sigma dw
+        # = (1 - k ds) x + (k theta ds + sigma dw)
         w = self.wshape
-        return np.concatenate((self._eval('theta', t0s, w),
-                               self._eval('k', t0s, w),
+        theta = self._eval('theta', t0s, w)
+        k = self._eval('k', t0s, w)
+        kds = k * ds[:, None]
+        return np.concatenate((1. - kds, kds * theta,
                                self._eval('sigma', t0s, w)), axis=1)
+
+    def _fold(self, wtab, params, has_corr):
+        # row i of the packed Cholesky factor times sigma_i sqrt|ds|, so the
+        # correlated draw is the diffusion increment itself
+        if not has_corr:
+            return wtab
+        M = self._nvars()
+        wtab = wtab.copy()
+        sigma = params[:, 2 * M:3 * M]
+        sq = wtab[:, 1]
+        for i in range(M):
+            lo = 2 + i * (i + 1) // 2
+            wtab[:, lo:lo + i + 1] *= (sigma[:, i] * sq)[:, None]
+        return wtab
 
 
 class hull_white_1factor_process(ornstein_uhlenbeck_process):
@@ -541,7 +567,7 @@ class cox_ingersoll_ross_process(_gpu_sde):
     init_defaults = {'x0': 1.}
     sde_defaults = {'theta': 1., 'k': 1., 'xi': 1.}
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
         w = self.wshape
         return np.concatenate((self._eval('theta', t0s, w),
                                self._eval('k', t0s, w),
@@ -584,7 +610,7 @@ class full_heston_process(_gpu_sde):
             np.concatenate((lx, y0), axis=-2)
         return X.reshape(self._nvars(), P).astype(cdt)
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
         v = self.vshape
         return np.concatenate([self._eval(n, t0s, v) for n in
                                ('mu', 'sigma', 'theta', 'k', 'xi')], axis=1)
@@ -651,7 +677,7 @@ class jumpdiff_process(_gpu_sde):
     def _y(self):
         return self._args['y']
 
-    def _param_rows(self, t0s):
+    def _param_rows(self, t0s, ds):
         mu = self._eval('mu', t0s, self.wshape)
         sigma = self._eval('sigma', t0s, self.wshape)
         return np.concatenate((mu - sigma * sigma / 2, sigma), axis=1)

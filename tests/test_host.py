@@ -70,6 +70,8 @@ def _c_layout(struct, fields):
     ('sdegpu_euler_problem', _lib.EulerProblem),
     ('sdegpu_heston_greeks', G.HestonGreeksProblem),
     ('sdegpu_loglin_greeks', G.LogGreeksProblem),
+    ('sdegpu_batch_instance', __import__('sdepy_amd.batched', fromlist=['x']).BatchInstance),
+    ('sdegpu_batch', __import__('sdepy_amd.batched', fromlist=['x']).Batch),
 ])
 def test_ctypes_layout_matches_c(cstruct, pystruct):
     fields = [f[0] for f in pystruct._fields_]
@@ -183,9 +185,18 @@ def test_preset_argument_handling_and_tables():
     with pytest.raises(ValueError):
         sa.lognorm_process(paths=3, method='rk')
     p = sa.lognorm_process(paths=4, mu=lambda t: .1 * t, sigma=.3)
-    rows = p._param_rows(np.array([0., 1., 2.]))
+    rows = p._param_rows(np.array([0., 1., 2.]), np.array([1., 1., 1.]))
     assert np.allclose(rows[:, 0], .1 * np.array([0., 1., 2.]) - .045)
     assert np.allclose(rows[:, 1], .3)
+    # OU rows (1 - k ds, k theta ds, sigma); sigma sqrt|ds| folded into L rows
+    o = sa.ornstein_uhlenbeck_process(paths=4, vshape=2, theta=np.array([[1.], [2.]]),
+                                      k=np.array([[.5], [3.]]), sigma=np.array([[.2], [.4]]),
+                                      rho=.3)
+    tab, hc = o._coef_table(o.sources['dw'], np.array([0., .25]), np.array([.25, .25]))
+    assert hc and tab.shape == (2, 2 + 3 + 6)
+    assert np.allclose(tab[0, 5:7], [1 - .125, 1 - .75]) and np.allclose(tab[0, 7:9], [.125, 1.5])
+    L = np.linalg.cholesky(np.array([[1, .3], [.3, 1]]))
+    assert np.allclose(tab[0, 2:5], [L[0, 0] * .2 * .5, L[1, 0] * .4 * .5, L[1, 1] * .4 * .5])
     h = sa.hull_white_process(paths=4, factors=3, rho=None)
     assert h.wshape == (3,) and h.xshape == ()
     assert h._output_map()[:3] == (1, _lib.OUT_SUM, 3)
