@@ -77,19 +77,147 @@ __device__ __forceinline__ double trcp<double>(double v) {
   return 1.0 / v;
 }
 
+// Per-step constants of the Heston step, built once per launch on the device
+// from the caller's (ds, sqrt|ds|) table, in FP64 and then rounded to T:
+//   [0] ds            [1] sq = sqrt|ds|   [2] r ds          [3] sigma^2/2 ds
+//   [4] k ds          [5] k theta ds      [6] sigma sq      [7] xi sq
+//   [8] xi sq / 2     [9] sigma sq / 2
+// With them the forward step is
+//   X' = X + (sigma sq sv z0 + (r ds - sigma^2/2 ds y+))
+//   V' = V + (xi sq sv w    + (k theta ds - k ds y+)),   w = rho z0 + rhoc z1
+// (sv = sqrt(y+), y+ = max(V, 0)), the scheme of full_heston_SDE.sde
+// (integration.py:2425-2433) with the increments summed before the add.
+constexpr int kHestonTab = 12;
+
 template <typename T>
+__global__ __launch_bounds__(kBlock) void k_heston_table(const T* __restrict__ steps, int n,
+                                                         double r, double sig, double k, double th,
+                                                         double xi, T* __restrict__ tab,
+                                                         double* __restrict__ sum_ds) {
+  __shared__ double sh[kBlock];
+  double part = 0.0;
+  for (int s = threadIdx.x; s < n; s += kBlock) {
+    const double ds = (double)steps[2 * s], sq = (double)steps[2 * s + 1];
+    T* row = tab + (int64_t)s * kHestonTab;
+    row[0] = (T)ds;
+    row[1] = (T)sq;
+    row[2] = (T)(r * ds);
+    row[3] = (T)(0.5 * sig * sig * ds);
+    row[4] = (T)(k * ds);
+    row[5] = (T)(k * th * ds);
+    row[6] = (T)(sig * sq);
+    row[7] = (T)(xi * sq);
+    row[8] = (T)(0.5 * xi * sq);
+    row[9] = (T)(0.5 * sig * sq);
+    row[10] = (T)0;
+    row[11] = (T)0;
+    part += ds;
+  }
+  sh[threadIdx.x] = part;
+  __syncthreads();
+#pragma unroll
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *sum_ds = sh[0];
+}
+
+// Reverse-mode adjoint of the full-truncation Heston Euler scheme.
+//
+// The log-price recursion has unit Jacobian in X, so its adjoint Xbar is the
+// payoff's dP/dX_T at every step and every X-dependent parameter adjoint is a
+// forward sum (d/dr of the drift = Xbar sum ds).  What remains reverse is the
+// variance adjoint: with Vbar the adjoint of V_{n+1}, step n contributes
+//   k:      Vbar (theta - y+) ds          theta: Vbar k ds
+//   xi:     Vbar sv sq w                  rho:   Vbar xi sv sq (z0 - rho/rhoc z1)
+//                                                = xi/rhoc^2 Vbar sv sq (z0 - rho w)
+// and, where V_n > 0 (d y+/dV = 1; 0 otherwise),
+//   Vbar += Vbar (xi sq w / (2 sv) - k ds) + Xbar (sigma sq z0 / (2 sv) - sigma^2/2 ds).
+// k and theta only need S1 = sum Vbar ds and S2 = sum Vbar y+ ds, xi and rho
+// the sums of Vbar sv sq w and Vbar sv sq (z0 - rho w); the constant factors
+// are applied once per path at the end.
+template <typename T>
+__device__ __forceinline__ T pos_part(T v) {
+  return v > (T)0 ? v : (T)0;
+}
+
+// Running reverse-sweep state of one path: the constant log-price adjoint,
+// the variance adjoint and the four parameter sums.
+template <typename T>
+struct HestonRev {
+  T xbar, vbar, s1, s2, sx, sr;
+};
+
+// One checkpoint segment of the reverse sweep: recompute the segment's
+// variances and normals from its checkpoint and the Philox counters into
+// registers, then walk its steps backwards.  FULL segments hold C steps.
+template <typename T, int C, bool FULL>
+__device__ __forceinline__ void heston_segment(const HestonArgs& a, const T* __restrict__ tb,
+                                               const T* __restrict__ ck, int64_t gid,
+                                               uint32_t path, int c, int n, T rho, T rhoc,
+                                               HestonRev<T>& rv) {
+  const int s0 = c * C;
+  T svs[C], z0s[C], ws[C];
+  T v = ck[(int64_t)c * a.paths + gid];
+#pragma unroll
+  for (int j = 0; j < C; j += 2) {
+    if (FULL || s0 + j < n) {
+      __asm__ volatile("" ::: "memory");
+      T z[4];
+      normals4<T>((uint32_t)((s0 + j) >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int s = s0 + j + q;
+        z0s[j + q] = z[2 * q];
+        ws[j + q] = rho * z[2 * q] + rhoc * z[2 * q + 1];
+        if (FULL || s < n) {
+          const T* r = tb + (int64_t)s * kHestonTab;
+          const T vp = pos_part(v);
+          const T sv = tsqrt<T>(vp);
+          svs[j + q] = sv;
+          v = v + fma(sv * r[7], ws[j + q], fma(-r[4], vp, r[5]));
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = C - 1; j >= 0; --j) {
+    const int s = s0 + j;
+    if (FULL || s < n) {
+      // keep the scalar table loads next to their step: hoisting all of a
+      // segment's rows at once overflows the SGPR file into VGPR lanes
+      __asm__ volatile("" ::: "memory");
+      const T* r = tb + (int64_t)s * kHestonTab;
+      const T sv = svs[j], z0 = z0s[j], w = ws[j];
+      const T vb = rv.vbar;
+      const T u = vb * r[0];
+      rv.s1 = rv.s1 + u;
+      rv.s2 = fma(u, sv * sv, rv.s2);
+      const T b = vb * (sv * r[1]);
+      rv.sx = fma(b, w, rv.sx);
+      rv.sr = fma(b, fma(-rho, w, z0), rv.sr);
+      const T h = trcp<T>(sv);
+      const T c1 = fma(w, h * r[8], -r[4]);
+      const T c2 = fma(z0, h * r[9], -r[3]);
+      const T d = fma(vb, c1, rv.xbar * c2);
+      rv.vbar = sv > (T)0 ? vb + d : vb;
+    }
+  }
+}
+
+template <typename T, int C>
 __global__ __launch_bounds__(kBlock) void k_heston_adjoint(const HestonArgs a,
-                                                           const T* __restrict__ st,
-                                                           T* __restrict__ ck) {
-  constexpr int C = kHestonSeg;
+                                                           const T* __restrict__ tb,
+                                                           T* __restrict__ ck,
+                                                           const double* __restrict__ sum_ds) {
+  static_assert(C % 2 == 0, "one Philox block feeds two steps");
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool live = gid < a.paths;
   const uint32_t path = (uint32_t)gid;
   const int64_t P = a.paths;
   const int n = a.nsteps;
-  const T mu = (T)a.r, sig = (T)a.sig, kk = (T)a.k, th = (T)a.th, xi = (T)a.xi;
   const T rho = (T)a.rho, rhoc = (T)a.rhoc;
-  const T hs2 = sig * sig * (T)0.5;
 
   double acc[kHestonNS];
 #pragma unroll
@@ -106,13 +234,13 @@ __global__ __launch_bounds__(kBlock) void k_heston_adjoint(const HestonArgs a,
         const int s = s0 + j;
         if (s < n) {
           if (s % C == 0) ck[(int64_t)(s / C) * P + gid] = V;
-          const T ds = st[2 * s], sq = st[2 * s + 1];
-          const T dw0 = z[2 * j] * sq;
-          const T dw1 = (rho * z[2 * j] + rhoc * z[2 * j + 1]) * sq;
-          const T vp = V > (T)0 ? V : (T)0;
+          const T* r = tb + (int64_t)s * kHestonTab;
+          const T z0 = z[2 * j];
+          const T w = rho * z0 + rhoc * z[2 * j + 1];
+          const T vp = pos_part(V);
           const T sv = tsqrt<T>(vp);
-          X = X + ((mu - sig * sig * vp * (T)0.5) * ds + (sig * sv) * dw0);
-          V = V + ((kk * (th - vp)) * ds + (xi * sv) * dw1);
+          X = X + fma(sv * r[6], z0, fma(-r[3], vp, r[2]));
+          V = V + fma(sv * r[7], w, fma(-r[4], vp, r[5]));
         }
       }
     }
@@ -123,71 +251,28 @@ __global__ __launch_bounds__(kBlock) void k_heston_adjoint(const HestonArgs a,
     const double intr = call ? S - a.strike : a.strike - S;
     const bool itm = intr > 0.0;
     const double pay = itm ? a.disc * intr : 0.0;
-    // dP/dX = disc * (+-S) on the money
-    const T Xbar = (T)(itm ? (call ? a.disc * S : -a.disc * S) : 0.0);
-    T Vbar = (T)0;
-    double gr = -a.maturity * pay;  // discount part of dP/dr
-    T rbar = (T)0, kbar = (T)0, thbar = (T)0, xibar = (T)0, rhobar = (T)0;
-    const T rho_over = rho / rhoc;
+    // dP/dX_T = disc * (+-S) on the money
+    const double xbar = itm ? (call ? a.disc * S : -a.disc * S) : 0.0;
+    const T Xbar = (T)xbar;
 
     // ---- reverse sweep over the checkpoint segments ----
-    const int nseg = (n + C - 1) / C;
-    for (int c = nseg - 1; c >= 0; --c) {
-      const int s0 = c * C;
-      T vs[C];
-      T zs[2 * C];
-      T v = ck[(int64_t)c * P + gid];
-#pragma unroll
-      for (int j = 0; j < C; j += 2) {
-        if (s0 + j < n) {
-          T z[4];
-          normals4<T>((uint32_t)((s0 + j) >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
-          zs[2 * j] = z[0];
-          zs[2 * j + 1] = z[1];
-          zs[2 * j + 2] = z[2];
-          zs[2 * j + 3] = z[3];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < C; ++j) {
-        const int s = s0 + j;
-        vs[j] = v;
-        if (s < n) {
-          const T ds = st[2 * s], sq = st[2 * s + 1];
-          const T dw1 = (rho * zs[2 * j] + rhoc * zs[2 * j + 1]) * sq;
-          const T vp = v > (T)0 ? v : (T)0;
-          const T sv = tsqrt<T>(vp);
-          v = v + ((kk * (th - vp)) * ds + (xi * sv) * dw1);
-        }
-      }
-#pragma unroll
-      for (int j = C - 1; j >= 0; --j) {
-        const int s = s0 + j;
-        if (s < n) {
-          const T ds = st[2 * s], sq = st[2 * s + 1];
-          const T z0 = zs[2 * j], z1 = zs[2 * j + 1];
-          const T Vn = vs[j];
-          const T vp = Vn > (T)0 ? Vn : (T)0;
-          const T sv = tsqrt<T>(vp);
-          const T dw0 = z0 * sq;
-          const T dw1 = (rho * z0 + rhoc * z1) * sq;
-          // parameter adjoints use the adjoints of the step's outputs
-          rbar = rbar + Xbar * ds;
-          kbar = kbar + Vbar * (th - vp) * ds;
-          thbar = thbar + Vbar * kk * ds;
-          xibar = xibar + Vbar * sv * dw1;
-          rhobar = rhobar + Vbar * xi * sv * (z0 - rho_over * z1) * sq;
-          if (Vn > (T)0) {
-            const T dsv = (T)0.5 * trcp<T>(sv);  // d sqrt(y+)/dy for y > 0
-            Vbar = Vbar + Xbar * (sig * dw0 * dsv - hs2 * ds) + Vbar * (xi * dw1 * dsv - kk * ds);
-          }
-        }
-      }
-    }
+    // Full segments run without step guards; only the last one can be short.
+    HestonRev<T> rv{Xbar, (T)0, (T)0, (T)0, (T)0, (T)0};
+    const int nfull = n / C;
+    if (nfull * C < n)
+      heston_segment<T, C, false>(a, tb, ck, gid, path, nfull, n, rho, rhoc, rv);
+    for (int c = nfull - 1; c >= 0; --c)
+      heston_segment<T, C, true>(a, tb, ck, gid, path, c, n, rho, rhoc, rv);
+    const T Vbar = rv.vbar, S1 = rv.s1, S2 = rv.s2, Sx = rv.sx, Sr = rv.sr;
 
-    gr += (double)rbar;
-    const double g[kHestonNG] = {(double)Xbar / a.x0, gr, (double)kbar, (double)thbar,
-                                 (double)xibar, (double)rhobar, (double)Vbar};
+    const double s1 = (double)S1;
+    const double g[kHestonNG] = {xbar / a.x0,
+                                 xbar * *sum_ds - a.maturity * pay,
+                                 a.th * s1 - (double)S2,
+                                 a.k * s1,
+                                 (double)Sx,
+                                 a.xi / (a.rhoc * a.rhoc) * (double)Sr,
+                                 (double)Vbar};
     acc[0] = pay;
     acc[1] = pay * pay;
 #pragma unroll
@@ -311,11 +396,25 @@ __global__ __launch_bounds__(kBlock) void k_sum_columns(const double* __restrict
   if (threadIdx.x == 0) out[c] = sh[0];
 }
 
-static size_t heston_ckpt_bytes(int dtype, int64_t paths, int nsteps) {
-  const int64_t nseg = (nsteps + kHestonSeg - 1) / kHestonSeg;
+// Workspace of one Heston adjoint launch, in this order (each part 256-byte
+// aligned): the per-step table [n][kHestonTab], the FP64 sum of ds, the
+// variance checkpoints [nseg][paths] and the per-block partial sums.
+struct HestonWs {
+  size_t tab, sum, ckpt, part, total;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static HestonWs heston_ws(int dtype, int64_t paths, int nsteps) {
   const size_t es = dtype == SDEGPU_F32 ? 4 : 8;
-  const size_t b = (size_t)(nseg > 0 ? nseg : 1) * (size_t)paths * es;
-  return (b + 255) & ~(size_t)255;
+  const int64_t nseg = (nsteps + kHestonSeg - 1) / kHestonSeg;
+  HestonWs w;
+  w.tab = 0;
+  w.sum = align256((size_t)(nsteps > 0 ? nsteps : 1) * kHestonTab * es);
+  w.ckpt = w.sum + 256;
+  w.part = w.ckpt + align256((size_t)(nseg > 0 ? nseg : 1) * (size_t)paths * es);
+  w.total = w.part + (size_t)blocks_for(paths) * kHestonNS * sizeof(double);
+  return w;
 }
 
 }  // namespace sdegpu
@@ -324,8 +423,7 @@ using namespace sdegpu;
 
 extern "C" size_t sdegpu_heston_greeks_workspace(const sdegpu_heston_greeks* g) {
   if (!g || g->paths < 1) return 0;
-  return heston_ckpt_bytes(g->dtype, g->paths, g->nsteps) +
-         (size_t)blocks_for(g->paths) * kHestonNS * sizeof(double);
+  return heston_ws(g->dtype, g->paths, g->nsteps).total;
 }
 
 extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* out, void* ws,
@@ -346,8 +444,10 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
     return fail(SDEGPU_ENOMEM, "heston_greeks: workspace of %zu bytes, %zu needed", ws_bytes, need);
   HestonArgs a = {};
   a.steps = g->steps;
-  a.ckpt = ws;
-  a.part = (double*)((char*)ws + heston_ckpt_bytes(g->dtype, g->paths, g->nsteps));
+  const HestonWs lay = heston_ws(g->dtype, g->paths, g->nsteps);
+  char* base = (char*)ws;
+  a.ckpt = base + lay.ckpt;
+  a.part = (double*)(base + lay.part);
   a.paths = g->paths;
   a.nsteps = g->nsteps;
   a.payoff = g->payoff;
@@ -369,12 +469,23 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
   a.epoch = g->epoch;
   hipStream_t st = (hipStream_t)stream;
   const unsigned nb = blocks_for(g->paths);
-  if (g->dtype == SDEGPU_F32)
-    hipLaunchKernelGGL(k_heston_adjoint<float>, dim3(nb), dim3(kBlock), 0, st, a,
-                       (const float*)a.steps, (float*)a.ckpt);
-  else
-    hipLaunchKernelGGL(k_heston_adjoint<double>, dim3(nb), dim3(kBlock), 0, st, a,
-                       (const double*)a.steps, (double*)a.ckpt);
+  double* sum_ds = (double*)(base + lay.sum);
+  if (g->dtype == SDEGPU_F32) {
+    float* tab = (float*)(base + lay.tab);
+    hipLaunchKernelGGL(k_heston_table<float>, dim3(1), dim3(kBlock), 0, st, (const float*)g->steps,
+                       g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi, tab, sum_ds);
+    if (int e = launch_status("k_heston_table")) return e;
+    hipLaunchKernelGGL((k_heston_adjoint<float, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st, a,
+                       (const float*)tab, (float*)a.ckpt, (const double*)sum_ds);
+  } else {
+    double* tab = (double*)(base + lay.tab);
+    hipLaunchKernelGGL(k_heston_table<double>, dim3(1), dim3(kBlock), 0, st,
+                       (const double*)g->steps, g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi,
+                       tab, sum_ds);
+    if (int e = launch_status("k_heston_table")) return e;
+    hipLaunchKernelGGL((k_heston_adjoint<double, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st, a,
+                       (const double*)tab, (double*)a.ckpt, (const double*)sum_ds);
+  }
   if (int e = launch_status("k_heston_adjoint")) return e;
   hipLaunchKernelGGL(k_sum_columns, dim3(kHestonNS), dim3(kBlock), 0, st, (const double*)a.part,
                      (int64_t)nb, kHestonNS, out);
