@@ -119,6 +119,11 @@ typedef struct sdegpu_euler_problem {
   uint32_t epoch_w;           /* Philox epoch of the dw source */
   uint32_t key_j[2];          /* Philox key of the dj source (JUMPDIFF) */
   uint32_t epoch_j;           /* Philox epoch of the dj source */
+  int64_t anti_w;             /* K > 0: antithetic dw (odd_wiener_source, infrastructure.py:
+                                 2047-2098, 2102-2117): paths K..2K-1 take the normals of
+                                 paths 0..K-1 with the sign reversed; 0: off */
+  int64_t anti_j;             /* K > 0: antithetic dj (even_cpoisson_source, :2139-2156):
+                                 paths K..2K-1 repeat the jumps of paths 0..K-1; 0: off */
 } sdegpu_euler_problem;
 
 /* Library and device queries. */

@@ -241,6 +241,26 @@ class PhiloxCPoisson(_PhiloxSource):
         return (sign * dz).astype(self.dtype).reshape(self.vshape + (P,))
 
 
+class Antithetic(_PhiloxSource):
+    """sdepy's sources with antithetics (_antithetics, infrastructure.py:
+    2047-2098): ``2K`` paths are the ``K`` paths of ``half`` followed by the
+    same increments times ``sign`` (-1: odd_wiener_source :2102-2117,
+    +1: even_poisson_source / even_cpoisson_source :2120-2156).  Like the
+    reference wrapper it exposes no ``dn_value``."""
+
+    def __init__(self, half, sign):
+        self.half, self.sign = half, sign
+        self.paths, self.vshape = 2 * half.paths, half.vshape
+
+    def begin_pass(self, tag):
+        if isinstance(self.half, _PhiloxSource):
+            self.half.begin_pass(tag)
+
+    def __call__(self, t, dt):
+        dz = self.half(t, dt)
+        return np.concatenate((dz, -dz if self.sign < 0 else dz), axis=-1)
+
+
 class NumpyWiener:
     """wiener_source.__call__ with NumPy draws, as the reference makes them
     (infrastructure.py:1515-1517, :1532-1534, :1552, :1558-1559)."""

@@ -16,8 +16,9 @@ launch).
 """
 from . import infrastructure
 from .infrastructure import (process, source, wiener_source, poisson_source,
-                             cpoisson_source, norm_rv, uniform_rv,
-                             path_moments)
+                             cpoisson_source, odd_wiener_source,
+                             even_poisson_source, even_cpoisson_source,
+                             norm_rv, uniform_rv, path_moments)
 from .integration import (wiener_process, lognorm_process,
                           ornstein_uhlenbeck_process, hull_white_process,
                           hull_white_1factor_process,
@@ -29,6 +30,9 @@ from .integration import (wiener_process, lognorm_process,
 dw = wiener_source
 dn = poisson_source
 dj = cpoisson_source
+odd_dw = odd_wiener_source
+even_dn = even_poisson_source
+even_dj = even_cpoisson_source
 wiener = wiener_process
 lognorm = lognorm_process
 oruh = ornstein_uhlenbeck_process

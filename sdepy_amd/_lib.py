@@ -47,6 +47,7 @@ class EulerProblem(ctypes.Structure):
         ('info_paths', c_void_p), ('info_rows', c_void_p),
         ('key_w', c_uint32 * 2), ('epoch_w', c_uint32),
         ('key_j', c_uint32 * 2), ('epoch_j', c_uint32),
+        ('anti_w', c_int64), ('anti_j', c_int64),
     ]
 
 

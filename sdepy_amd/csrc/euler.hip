@@ -33,6 +33,7 @@ struct EulerArgs {
   long long* info_rows;
   double x0[SDEGPU_MAX_VARS];
   int64_t paths;
+  int64_t anti_w, anti_j;  // antithetic half sizes (0: off)
   int32_t nsteps, ncoef, poff, nout, out_group, has_corr, row0;
   uint32_t exp_mask;
   uint32_t c2w, c2p, c2j;  // counter word c2 of the normal, Poisson and jump streams
@@ -120,6 +121,12 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
   if (gid >= a.paths) return;
   const int64_t P = a.paths;
   const uint32_t path = (uint32_t)gid;
+  // antithetic halves: the trailing paths replay the draws of the leading
+  // ones, with the Wiener normals negated (odd_wiener_source) and the jumps
+  // unchanged (even_cpoisson_source)
+  const bool flip_w = a.anti_w > 0 && gid >= a.anti_w;
+  const uint32_t path_w = flip_w ? (uint32_t)(gid - a.anti_w) : path;
+  const uint32_t path_j = (a.anti_j > 0 && gid >= a.anti_j) ? (uint32_t)(gid - a.anti_j) : path;
 
   T x[M];
   if (a.x0_paths) {
@@ -146,14 +153,18 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
       const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
 #pragma unroll
       for (int j = 0; j < NZ / 4; ++j)
-        normals4<T>(b0 + j, path, a.c2w, a.ew, a.kw0, a.kw1, z + 4 * j);
+        normals4<T>(b0 + j, path_w, a.c2w, a.ew, a.kw0, a.kw1, z + 4 * j);
+      if (flip_w) {
+#pragma unroll
+        for (int i = 0; i < NZ; ++i) z[i] = -z[i];
+      }
     }
     if constexpr (kJump) {
       if (!replay_j) {
         const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
 #pragma unroll
         for (int j = 0; j < NZ / 4; ++j) {
-          const ctr4 r = philox4x32_10(ctr4{b0 + j, path, a.c2p, a.ej}, a.kj0, a.kj1);
+          const ctr4 r = philox4x32_10(ctr4{b0 + j, path_j, a.c2p, a.ej}, a.kj0, a.kj1);
           upois[4 * j + 0] = r.x;
           upois[4 * j + 1] = r.y;
           upois[4 * j + 2] = r.z;
@@ -259,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
               T zz[4];
               for (int j = 0; j < n; ++j) {
                 if ((j & 3) == 0)
-                  normals4<T>(slot, path, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0,
+                  normals4<T>(slot, path_j, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0,
                               a.kj1, zz);
                 const int lane = j & 3;
                 const T zj = lane == 0 ? zz[0] : lane == 1 ? zz[1] : lane == 2 ? zz[2] : zz[3];
@@ -384,6 +395,14 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EINVAL, "jump model without Poisson table");
   if ((int64_t)pr->nsteps * m > 0xFFFFFFFFll)
     return fail(SDEGPU_EINVAL, "too many steps for the 32-bit Philox block counter");
+  // sources with antithetics hold an even number of paths, split in halves
+  // (infrastructure.py:2057-2062)
+  if (pr->anti_w != 0 && (pr->anti_w < 0 || 2 * pr->anti_w != pr->paths || pr->replay_dw))
+    return fail(SDEGPU_EINVAL, "antithetic dw half %lld does not fit %lld paths",
+                (long long)pr->anti_w, (long long)pr->paths);
+  if (pr->anti_j != 0 && (pr->anti_j < 0 || 2 * pr->anti_j != pr->paths || !jump || pr->replay_dj))
+    return fail(SDEGPU_EINVAL, "antithetic dj half %lld does not fit %lld paths",
+                (long long)pr->anti_j, (long long)pr->paths);
 
   const uint32_t want_mask = expected_exp_mask(pr->model, m, pr->nout);
   if (pr->exp_mask != want_mask)
@@ -402,6 +421,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.info_rows = pr->info_rows;
   for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
   a.paths = pr->paths;
+  a.anti_w = pr->anti_w;
+  a.anti_j = pr->anti_j;
   a.nsteps = pr->nsteps;
   a.ncoef = pr->ncoef;
   a.poff = poff;

@@ -448,6 +448,74 @@ class cpoisson_source(_philox_source):
         return dz
 
 
+# --------------------------------------------------------------------------
+# sources with antithetic paths
+# --------------------------------------------------------------------------
+
+def _antithetics(source_class, sign):
+    """A source class whose ``2K`` paths are ``K`` paths of ``source_class``
+    followed by the same ``K`` paths multiplied by ``sign`` (sdepy's
+    ``_antithetics``, infrastructure.py:2047-2098; like there, the class is
+    not a subclass of ``source_class``).  Inside the Euler kernel the
+    trailing half re-reads the Philox counters of the leading half."""
+
+    class antithetics_class(source):
+        def __init__(self, *, paths=2, vshape=(), dtype=None, rng=None,
+                     **args):
+            if paths % 2:
+                raise ValueError(
+                    'the number of paths for sources with antithetics '
+                    'should be even, not {}'.format(paths))
+            self._dz = source_class(paths=paths // 2, vshape=vshape,
+                                    dtype=dtype, rng=rng, **args)
+
+        def __call__(self, t, dt=None):
+            dz = self._dz(t, dt)
+            return np.concatenate((dz, -dz if sign < 0 else dz), axis=-1)
+
+        @property
+        def paths(self):
+            return 2 * self._dz.paths
+
+        @property
+        def vshape(self):
+            return self._dz.vshape
+
+        @property
+        def dtype(self):
+            return self._dz.dtype
+
+        @property
+        def rng(self):
+            return self._dz.rng
+
+        @property
+        def half(self):
+            """The source drawing the leading half of the paths."""
+            return self._dz
+
+        antithetic_sign = sign
+
+    antithetics_class.__init__.__doc__ = (
+        'See {} class documentation'.format(source_class.__name__))
+    return antithetics_class
+
+
+class odd_wiener_source(_antithetics(wiener_source, -1)):
+    """dw with antithetic paths (sdepy infrastructure.py:2102-2117): with
+    ``paths=2K``, ``x[..., i] == -x[..., K + i]``."""
+
+
+class even_poisson_source(_antithetics(poisson_source, 1)):
+    """dn with antithetic paths exposing identical increments (sdepy
+    infrastructure.py:2120-2136): ``x[..., i] == x[..., K + i]``."""
+
+
+class even_cpoisson_source(_antithetics(cpoisson_source, 1)):
+    """dj with antithetic paths exposing identical increments (sdepy
+    infrastructure.py:2139-2156): ``x[..., i] == x[..., K + i]``."""
+
+
 class _uniform_rv:
     def __init__(self, a=0., b=1.):
         self.a, self.b = a, b

@@ -29,6 +29,7 @@ import numpy as np
 from . import _lib
 from .infrastructure import (process, wiener_source, cpoisson_source,
                              poisson_source, norm_rv, per_component,
+                             odd_wiener_source, even_cpoisson_source,
                              _gpu_dtype, _shape_setup, _get_default_rng)
 
 __all__ = ['wiener_process', 'lognorm_process', 'ornstein_uhlenbeck_process',
@@ -328,7 +329,14 @@ class _gpu_sde:
         # Wiener source: in-kernel Philox draws, or recorded increments
         dw = self.sources['dw']
         host_sources = {}
-        if isinstance(dw, wiener_source):
+        if isinstance(dw, odd_wiener_source) and \
+                type(dw.half) is wiener_source:
+            # antithetic halves: the trailing paths negate the leading ones
+            tab, has_corr = self._coef_table(dw.half, t0s, ds)
+            prob.key_w[0], prob.key_w[1] = dw.half.key
+            prob.epoch_w = dw.half.next_epoch()
+            prob.anti_w = dw.half.paths
+        elif isinstance(dw, wiener_source):
             tab, has_corr = self._coef_table(dw, t0s, ds)
             prob.key_w[0], prob.key_w[1] = dw.key
             prob.epoch_w = dw.next_epoch()
@@ -339,13 +347,21 @@ class _gpu_sde:
             has_corr = False
             host_sources['dw'] = dw
         dj = self.sources.get('dj')
+        self._anti_dj = False
         if dj is not None:
-            if isinstance(dj, cpoisson_source) and dj.gpu_jumps:
-                prob.key_j[0], prob.key_j[1] = dj.key
-                prob.epoch_j = dj.next_epoch()
-                dj.dn.epoch = dj.epoch
-                pois = dj.dn.lam_table(t0s, ds)
-                ab = dj.ab_table(t0s, ds)
+            gdj = dj
+            if isinstance(dj, even_cpoisson_source) and \
+                    isinstance(dj.half, cpoisson_source) and dj.half.gpu_jumps:
+                # antithetic halves: the trailing paths repeat the jumps
+                gdj = dj.half
+                prob.anti_j = gdj.paths
+                self._anti_dj = True
+            if isinstance(gdj, cpoisson_source) and gdj.gpu_jumps:
+                prob.key_j[0], prob.key_j[1] = gdj.key
+                prob.epoch_j = gdj.next_epoch()
+                gdj.dn.epoch = gdj.epoch
+                pois = gdj.dn.lam_table(t0s, ds)
+                ab = gdj.ab_table(t0s, ds)
             else:
                 host_sources['dj'] = dj
                 pois = ab = None
@@ -395,7 +411,8 @@ class _gpu_sde:
         prob.out = xx_dev.data_ptr()
 
         info_paths = info_rows = None
-        if self.getinfo and self.model in (_lib.MODEL_HESTON, _lib.MODEL_JUMPDIFF):
+        if self.getinfo and (self.model == _lib.MODEL_HESTON or (
+                self.model == _lib.MODEL_JUMPDIFF and not self._anti_dj)):
             info_paths = _lib.zeros((M, P), np.int32)
             prob.info_paths = info_paths.data_ptr()
             if self.model == _lib.MODEL_JUMPDIFF:
@@ -697,7 +714,9 @@ class jumpdiff_process(_gpu_sde):
                     totals[j] += dn_host[s].sum()
                 if r >= 0:
                     j += 1
-        elif self._host_dj:
+        elif self._host_dj or self._anti_dj:
+            # a dj without dn_value (a source with antithetics, or a custom
+            # one) leaves the jump info at zero (jumpdiff_SDE.info_next)
             counts = np.zeros((self._nvars(), P), dtype=int)
         elif info_paths is not None:
             counts = _lib.to_host(info_paths)

@@ -234,6 +234,70 @@ def test_merton_matches_twin(gpu, dtype):
     assert _rel(x, ref) < tol
 
 
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_antithetic_sources_match_twin(gpu, dtype):
+    """odd_wiener_source / even_cpoisson_source (infrastructure.py:2047-2156):
+    the trailing half of the paths re-reads the leading half's counters,
+    negating the normals and keeping the jumps; checked against the oracle's
+    restatement of the antithetics wrapper, and the jump info stays zero
+    because the wrapper exposes no dn_value (jumpdiff_SDE.info_next)."""
+    K, n = 300, 30
+    rng = np.random.default_rng(51)
+    dw = sa.odd_wiener_source(paths=2 * K, rng=rng, dtype=dtype)
+    dj = sa.even_cpoisson_source(paths=2 * K, rng=rng, dtype=dtype, lam=4.,
+                                 y=sa.norm_rv(-.1, .15))
+    info = {}
+    p = sa.merton_jumpdiff_process(paths=2 * K, dtype=dtype, steps=n + 1,
+                                   info=info, dw=dw, dj=dj, x0=1., mu=.03,
+                                   sigma=.2)
+    tt = np.linspace(0, 1, 4)
+    x = p(tt)
+    ref_info = {}
+    ref = ora.integrate(
+        'jumpdiff', tt, paths=2 * K, steps=n + 1, info=ref_info,
+        sources={'dw': ora.Antithetic(ora.PhiloxWiener(dw.half.key, 0, K, ()), -1),
+                 'dj': ora.Antithetic(ora.PhiloxCPoisson(dj.half.key, 0, K, (), lam=4.,
+                                                         a=-.1, b=.15), 1)},
+        x0=1., mu=.03, sigma=.2)
+    tol = RTOL64 if dtype == np.float64 else 5e-5
+    assert _rel(x, ref) < tol
+    assert not info['jump_count'].any() and not info['jump_rate'].any()
+    assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+
+    # correlated OU: the folded Cholesky rows see the negated normals too
+    C = np.array([[1., .4, -.2], [.4, 1., .3], [-.2, .3, 1.]])
+    dw3 = sa.odd_wiener_source(paths=2 * K, vshape=3, corr=C, dtype=dtype,
+                               rng=np.random.default_rng(52))
+    prm = dict(x0=.1, theta=np.array([[.5], [-.2], [.1]]),
+               k=np.array([[1.], [2.], [.5]]), sigma=np.array([[.3], [.2], [.1]]))
+    x3 = sa.ornstein_uhlenbeck_process(paths=2 * K, vshape=3, dtype=dtype,
+                                       steps=n + 1, dw=dw3, **prm)(tt)
+    ref3 = ora.integrate(
+        'ou', tt, paths=2 * K, vshape=(3,), steps=n + 1,
+        sources={'dw': ora.Antithetic(ora.PhiloxWiener(dw3.half.key, 0, K, (3,),
+                                                       corr=C), -1)}, **prm)
+    assert np.max(np.abs(x3 - ref3)) < (1e-12 if dtype == np.float64 else 2e-5)
+
+
+def test_antithetic_gbm_symmetry_and_source_call(gpu):
+    """log-Euler GBM with odd dw: log x_i + log x_{K+i} = 2 (log x0 + (mu -
+    sigma^2/2) t) on every stored row; the stand-alone source call returns
+    (dz, -dz) along the paths axis."""
+    K = 512
+    dw = sa.odd_wiener_source(paths=2 * K, rng=np.random.default_rng(53))
+    tt = np.linspace(0., 2., 9)
+    x = sa.lognorm_process(paths=2 * K, x0=1.3, mu=.07, sigma=.4, steps=41,
+                           dw=dw)(tt)
+    lx = np.log(np.asarray(x))
+    drift = 2 * (np.log(1.3) + (.07 - .08) * tt)
+    assert np.allclose(lx[:, :K] + lx[:, K:], drift[:, None], rtol=0, atol=1e-12)
+    z = dw(0., .25)
+    assert z.shape == (2 * K,)
+    assert np.array_equal(z[:K], -z[K:])
+    with pytest.raises(ValueError):
+        sa.odd_wiener_source(paths=7)
+
+
 def test_backward_integration_i0_and_steps(gpu):
     """i0 != 0 runs a backward pass from tt[i0] (paths_generator.__call__,
     integration.py:571-580)."""

@@ -108,6 +108,12 @@ def test_entry_point_validation_maps_to_exceptions():
     pr.model, pr.nvars, pr.paths = _lib.MODEL_OU, 1, 0
     with pytest.raises(ValueError):
         _lib.check(lib.sdegpu_euler(pr, None))
+    pr.paths, pr.nout, pr.ncoef, pr.exp_mask, pr.anti_w = 10, 1, 5, 0, 4
+    with pytest.raises(ValueError):        # antithetic half must be paths/2
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.anti_w, pr.anti_j = 5, 5
+    with pytest.raises(ValueError):        # antithetic jumps without a jump model
+        _lib.check(lib.sdegpu_euler(pr, None))
     with pytest.raises(ValueError):
         _lib.check(lib.sdegpu_path_moments(None, 0, -1, 10, None, None, None, 0, None))
     assert lib.sdegpu_moments_workspace(3, 70000) == 3 * 2 * 2 * 8
@@ -253,6 +259,26 @@ def test_no_gpu_fails_loudly():
         sa.wiener_source(paths=3)(0., 1.)
     with pytest.raises(RuntimeError):
         G.heston_greeks(paths=10, steps=3)
+
+
+def test_antithetic_sources_host_side():
+    """sdepy's _antithetics (infrastructure.py:2047-2098): even path counts
+    only, the leading half is an ordinary source, properties delegate."""
+    with pytest.raises(ValueError):
+        sa.odd_wiener_source(paths=7)
+    dw = sa.odd_wiener_source(paths=8, vshape=2, rho=.3,
+                              rng=np.random.default_rng(1))
+    assert dw.paths == 8 and dw.vshape == (2,) and dw.half.paths == 4
+    assert isinstance(dw.half, sa.wiener_source)
+    assert not isinstance(dw, sa.wiener_source)
+    dj = sa.even_cpoisson_source(paths=6, lam=2., y=sa.norm_rv(0., 1.))
+    assert dj.half.gpu_jumps and dj.paths == 6
+    assert sa.odd_dw is sa.odd_wiener_source and sa.even_dj is sa.even_cpoisson_source
+    # the oracle's restatement of the wrapper
+    from oracle import euler as ora
+    src = ora.Antithetic(ora.NumpyWiener(np.random.RandomState(0), 5, ()), -1)
+    z = src(0., .5)
+    assert z.shape == (10,) and np.array_equal(z[:5], -z[5:])
 
 
 def test_bench_refuses_more_than_one_gpu():
