@@ -15,7 +15,14 @@ Printed as one JSON line:
               MI355X FP32 vector peak (the kernel is VALU-bound, not HBM or
               MFMA); HIP-event timing on the launch stream
   ou_fullpath the OU-4 correlated full-path kernel (1M x 1000, FP32), the
-              HBM-bound case: stored bytes / kernel time vs 8 TB/s
+              HBM-bound case: stored bytes / kernel time vs 8 TB/s, and the
+              largest deviation from the FP64 run of the same paths
+  merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
+              mu, sigma, a, b), with the FP64 run of the same paths in SE units
+  gbm_c1      GBM 10k x 252 FP64 full path + pmean/pvar (latency-bound):
+              kernel time and the end-to-end preset call
+  batched     1024 heterogeneous GBM / HW2F instances in one persistent
+              launch; per-instance FP32 moments vs FP64 runs of the same paths
   cpu_baseline the oracle's NumPy restatement of sdepy's loop (same
               operations as the reference, NumPy draws) on the host, one core,
               price + 7 Greeks by 15 forward runs on a bounded path sample
@@ -43,6 +50,18 @@ PATHS, NSTEPS, STRIKE, MATURITY = 4_000_000, 365, 100., 1.
 ADJ_FLOPS_PER_PATH_STEP = 74
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.
+# VALU issue ceiling: one wave64 vector instruction per clock per CU,
+# 256 CUs at 2.4 GHz
+VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
+
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per
+# dispatch), scaled by what the same counters read for a 1 GiB device copy
+# (FETCH reads half the copied bytes: x2.0; WRITE is exact: x1.0), and the
+# kernel's VALU wave-instructions per launch (SQ_INSTS_VALU); the summaries
+# they come from are in profiles/
+ADJ_TRAFFIC_BYTES = None
+ADJ_VALU_WAVE_INSTS = None
+OU_TRAFFIC_BYTES = None
 
 
 def _events(torch):
@@ -127,13 +146,20 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--legs', default='adjoint,fd,ou,batched,cpu',
-                    help='comma list of: adjoint, fd, ou, batched, cpu, copyref '
-                         '(a 1 GiB device copy used to calibrate rocprof byte '
-                         'counters)')
+    ap.add_argument('--legs', default='adjoint,fd,ou,merton,gbm,batched,cpu',
+                    help='comma list of: adjoint, fd, ou, merton, gbm, batched, '
+                         'cpu, copyref (a 1 GiB device copy used to calibrate '
+                         'rocprof byte counters), ou64 (FP64 twin of the OU '
+                         'leg for its error figure; on by default with ou), '
+                         'batched64 (FP64 twins of the batched instances; on '
+                         'by default with batched)')
     ap.add_argument('--cpu-paths', type=int, default=30000)
     args = ap.parse_args()
     legs = set(args.legs.split(','))
+    if 'ou' in legs:
+        legs.add('ou64')
+    if 'batched' in legs:
+        legs.add('batched64')
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build runs the path kernels on '
               'a single MI355X (paths shard by range across GPUs, see '
@@ -183,8 +209,15 @@ def main():
         'kernel': 'k_heston_adjoint<float>', 'bound': 'valu',
         'achieved': achieved, 'peak': FP32_VECTOR_PEAK_TFLOPS,
         'unit': 'TFLOP/s', 'frac': achieved / FP32_VECTOR_PEAK_TFLOPS,
-        'traffic': None,
-        'flops_per_path_step': ADJ_FLOPS_PER_PATH_STEP}
+        'traffic': ADJ_TRAFFIC_BYTES,
+        'flops_per_path_step': ADJ_FLOPS_PER_PATH_STEP,
+        'frac_of_unpacked_fp32_peak': achieved / (FP32_VECTOR_PEAK_TFLOPS / 2)}
+    if ADJ_VALU_WAVE_INSTS:
+        rate = ADJ_VALU_WAVE_INSTS / (ms * 1e-3)
+        out['roofline']['valu_issue'] = {
+            'wave_insts_per_launch': ADJ_VALU_WAVE_INSTS,
+            'achieved_per_s': rate, 'peak_per_s': VALU_ISSUE_PEAK_PER_S,
+            'frac': rate / VALU_ISSUE_PEAK_PER_S}
 
     # ---- FP64 forward + CRN finite differences, same paths ----
     if 'fd' in legs:
@@ -227,10 +260,74 @@ def main():
             'ms': ou_ms, 'path_steps_per_s': ou.path_steps / (ou_ms * 1e-3),
             'roofline': {'kernel': 'k_euler<OU,float,4>', 'bound': 'hbm',
                          'achieved': gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                         'frac': gbs / HBM_PEAK_GBS, 'traffic': None,
+                         'frac': gbs / HBM_PEAK_GBS,
+                         'traffic': OU_TRAFFIC_BYTES,
                          'bytes_per_path_step': 16}}
+        if 'ou64' in legs:
+            # the same paths (key and last epoch) stepped in FP64
+            p64 = sa.ornstein_uhlenbeck_process(
+                paths=10 ** 6, vshape=4, dtype=np.float64, x0=0., theta=th,
+                k=kk, sigma=sg, corr=C, rng=np.random.default_rng(1))
+            ou64 = p64.prepare(np.linspace(0., 1., 1001))
+            ou64.passes[0].prob.epoch_w = ou.passes[0].prob.epoch_w
+            ou64.launch()
+            torch.cuda.synchronize()
+            err = (ou.xx_dev.double() - ou64.xx_dev).abs().max().item()
+            scale = ou64.xx_dev.abs().max().item()
+            out['ou_fullpath']['max_abs_err_vs_fp64'] = err
+            out['ou_fullpath']['max_abs_fp64'] = scale
+            del ou64, p64
         del ou, p
         torch.cuda.empty_cache()
+
+    # ---- Merton jump-diffusion adjoint (C4) ----
+    if 'merton' in legs:
+        mprm = dict(x0=1., mu=.03, sigma=.2, lam=1., a=-.1, b=.15, r=.03,
+                    strike=1., maturity=1., steps=252, paths=2_000_000,
+                    key_w=(0x3e47, 0x11), key_j=(0x3e47, 0x12), epoch=0)
+        mrun = G.LogGreeks(dtype=np.float32, **mprm)
+        m_ms, _ = timed(torch, mrun.launch, 10, 2)
+        mres = mrun.result()
+        m64 = G.LogGreeks(dtype=np.float64, **mprm)
+        m64.launch()
+        r64 = m64.result()
+        err = {'price': abs(mres.price - r64.price) / r64.price_se}
+        for k in G.LOG_GREEKS:
+            err[k] = abs(mres.greeks[k] - r64.greeks[k]) / r64.greeks_se[k]
+        out['merton'] = {
+            'workload': 'merton_adjoint_2Mx252_fp32',
+            'ms': m_ms, 'path_steps_per_s': mrun.path_steps / (m_ms * 1e-3),
+            'greeks_per_s': len(G.LOG_GREEKS) / (m_ms * 1e-3),
+            'price': mres.price, 'price_se': mres.price_se,
+            'greeks': mres.greeks, 'greeks_se': mres.greeks_se,
+            'fp32_vs_fp64_same_paths_se': err}
+        del mrun, m64
+
+    # ---- GBM 10k x 252 FP64 full path + pmean/pvar (C1, latency-bound) ----
+    if 'gbm' in legs:
+        import sdepy_amd as sa
+        tt1 = np.linspace(0., 1., 253)
+
+        def c1_proc():
+            return sa.lognorm_process(paths=10_000, x0=1., mu=.05, sigma=.2,
+                                      dtype=np.float64,
+                                      rng=np.random.default_rng(1234))
+        prep = c1_proc().prepare(tt1)
+        k_ms, _ = timed(torch, lambda: prep.launch(fresh=True), 20, 3)
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            x = c1_proc()(tt1)
+            m, v = x[-1].pmean(), x[-1].pvar()
+        e2e = (time.perf_counter() - t0) / reps
+        out['gbm_c1'] = {
+            'workload': 'gbm_fullpath_10kx252_fp64',
+            'kernel_ms': k_ms,
+            'kernel_path_steps_per_s': 10_000 * 252 / (k_ms * 1e-3),
+            'end_to_end_ms': e2e * 1e3,
+            'terminal_mean': float(np.asarray(m).ravel()[0]),
+            'terminal_var': float(np.asarray(v).ravel()[0])}
+        del prep, x
 
     # ---- 1024 heterogeneous GBM / 2-factor Hull-White instances ----
     if 'batched' in legs:
@@ -241,6 +338,21 @@ def main():
         prep = time.perf_counter() - t0
         bms, _ = timed(torch, bt.launch, 3, 1, stream=bt.stream)
         mean, var = bt.result()
+        inst_err = None
+        if 'batched64' in legs:
+            # FP64 twins: same keys, epoch 0, stepped by the ordinary preset
+            errs = []
+            for (p64, tl), mu32 in zip(B.heterogeneous_specs(1024, seed=1234,
+                                                             dtype=np.float64),
+                                       mean):
+                xt = p64(tl)[-1]
+                m64 = float(np.asarray(xt.pmean()).ravel()[0])
+                v64 = float(np.asarray(xt.pvar()).ravel()[0])
+                se = math.sqrt(v64 / (p64.paths - 1))
+                errs.append((abs(mu32 - m64) / se, abs(mu32 - m64) / max(abs(m64), 1e-300)))
+            errs = np.array(errs)
+            inst_err = {'max_mean_err_se': float(errs[:, 0].max()),
+                        'median_mean_err_se': float(np.median(errs[:, 0]))}
         out['batched'] = {
             'workload': 'c5_1024_gbm_hw2f_heterogeneous_fp32',
             'instances': bt.n, 'tiles': bt.n_items,
@@ -248,7 +360,8 @@ def main():
             'path_steps_per_s': bt.path_steps / (bms * 1e-3),
             'blocks': bt.blocks, 'cu': bt.cu,
             'autotune_s': {str(k): v for k, v in bt.tuning.items()},
-            'prepare_s': prep, 'graph': bt.graph is not None}
+            'prepare_s': prep, 'graph': bt.graph is not None,
+            'fp32_vs_fp64_same_paths': inst_err}
         del bt
         torch.cuda.empty_cache()
 

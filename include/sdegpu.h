@@ -153,6 +153,33 @@ int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t path
  * np.maximum(X[-1]-K, 0)*exp(-r*T) + montecarlo step of sdepy's Black-Scholes
  * test (tests/test_quant.py:608-611).
  */
+/*
+ * montecarlo summaries of a [rows][paths] sample (montecarlo.update and
+ * _update_moments / _update_histogram, infrastructure.py:2862-3024; also the
+ * process summaries pmin/pmax, :833-849).  `use` folds antithetic halves
+ * first: 0 all values, 1 'even' (x[i] + x[i+H]) / 2, 2 'odd' (x[i] - x[i+H]) / 2
+ * with H = paths / 2, in the sample's type.  rows <= 65535.
+ * sdegpu_mc_minmax: out [rows][3] = (min, max, number of NaN values), NaNs
+ *   left out of min/max.
+ * sdegpu_mc_moments: out [rows][5] = sums of d, d^2, d^3, d^4 with
+ *   d = v - center[row], and the sum of v, accumulated in FP64 in a fixed
+ *   order.
+ * sdegpu_mc_histogram: counts [rows][nbins + 1]: counts[r][i] = number of v
+ *   with edges[r][i] <= v < edges[r][i+1] (the last bin also takes
+ *   v == edges[r][nbins]), counts[r][nbins] = values outside the edges or NaN
+ *   (np.histogram's rule); edges [rows][nbins + 1] increasing, FP64.
+ * Workspace: sdegpu_mc_workspace bytes (all three functions).
+ */
+size_t sdegpu_mc_workspace(int64_t rows, int64_t paths);
+int sdegpu_mc_minmax(const void* x, int32_t dtype, int64_t rows, int64_t paths, int32_t use,
+                     double* out, void* workspace, size_t workspace_bytes, void* stream);
+int sdegpu_mc_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths, int32_t use,
+                      const double* center, double* out, void* workspace, size_t workspace_bytes,
+                      void* stream);
+int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, int64_t paths, int32_t use,
+                        const double* edges, int32_t nbins, unsigned long long* counts,
+                        void* stream);
+
 size_t sdegpu_payoff_workspace(int64_t paths);
 int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t kind, double strike,
                           double scale, double* out, void* workspace, size_t workspace_bytes,

@@ -67,6 +67,16 @@ SIGNATURES = [
     ('sdegpu_payoff_moments', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int32, c_double, c_double, c_void_p,
       c_void_p, c_size_t, c_void_p]),
+    ('sdegpu_mc_workspace', c_size_t, [c_int64, c_int64]),
+    ('sdegpu_mc_minmax', ctypes.c_int,
+     [c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p, c_void_p,
+      c_size_t, c_void_p]),
+    ('sdegpu_mc_moments', ctypes.c_int,
+     [c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p, c_void_p,
+      c_void_p, c_size_t, c_void_p]),
+    ('sdegpu_mc_histogram', ctypes.c_int,
+     [c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p, c_int32,
+      c_void_p, c_void_p]),
     ('sdegpu_wiener_fill', ctypes.c_int,
      [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
       c_void_p, c_int32, c_int32, c_void_p, c_void_p]),

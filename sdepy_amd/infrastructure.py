@@ -877,16 +877,17 @@ class process(np.ndarray):
             axis=0, dtype=dtype, out=out, ddof=ddof, keepdims=True))
 
 
-def path_moments(x, device_tensor=None):
+def path_moments(x, device_tensor=None, paths=None):
     """FP64 (n, mean, m2) over the last axis of ``x``, reduced on the GPU
     (``sdegpu_path_moments``).  ``device_tensor``, when given, is a device
-    copy of ``x`` and avoids the upload."""
+    copy of ``x`` shaped [rows][paths] and avoids the upload; ``x`` then only
+    gives the leading shape and dtype (``paths`` the path count)."""
     x = np.asarray(x)
     if x.dtype.kind != 'f':
         raise NotImplementedError(
             'the GPU path reductions take floating point processes, not {}'
             .format(x.dtype))
-    paths = x.shape[-1]
+    paths = x.shape[-1] if paths is None else int(paths)
     rows = int(np.prod(x.shape[:-1], dtype=int))
     code, cdt, _ = _gpu_dtype(x.dtype)
     _lib.require_gpu()

@@ -476,6 +476,84 @@ class PreparedRun:
     def path_steps(self):
         return self.sde.paths * sum(p.nsteps for p in self.passes)
 
+    # -- summaries of the device-resident output ------------------------------
+    # process.pmin/pmax/psum/pmean/pvar/pstd (infrastructure.py:833-889)
+    # computed on the output buffer itself, so that a full-path run's paths
+    # never travel to the host; each returns what the same method returns on
+    # the host result (a process, or a tuple of processes, with one path).
+
+    def _rows(self):
+        n_t, nout, P = self.xx_dev.shape
+        return self.xx_dev.reshape(n_t * nout, P)
+
+    def _summary(self, vals, dtype):
+        sde = self.sde
+        odt = self.odt if dtype is None else np.dtype(dtype)
+        xx = np.asarray(vals).astype(odt).reshape(
+            self.tt.shape + sde.xshape + (1,))
+        return sde._result(self.tt_asis, xx)
+
+    def _moments(self):
+        from .infrastructure import path_moments
+        rows = self._rows()
+        f32 = rows.dtype == _lib.torch_mod().float32
+        return path_moments(np.empty((rows.shape[0], 0),
+                                     dtype=np.float32 if f32 else np.float64),
+                            device_tensor=rows, paths=rows.shape[1])
+
+    def _minmax(self):
+        rows = self._rows()
+        code = _lib.F32 if rows.dtype == _lib.torch_mod().float32 else _lib.F64
+        n, P = int(rows.shape[0]), int(rows.shape[1])
+        lib = _lib.lib()
+        out = _lib.empty((n, 3), np.float64)
+        nbytes = lib.sdegpu_mc_workspace(min(n, 65535), P)
+        ws = _lib.workspace(nbytes)
+        for r0 in range(0, n, 65535):
+            r1 = min(n, r0 + 65535)
+            _lib.check(lib.sdegpu_mc_minmax(
+                _lib.ptr(rows[r0:r1]), code, r1 - r0, P, 0, _lib.ptr(out[r0:r1]),
+                _lib.ptr(ws), nbytes, _lib.stream()))
+        mm = _lib.to_host(out)
+        nan = mm[:, 2] > 0
+        return np.where(nan, np.nan, mm[:, 0]), np.where(nan, np.nan, mm[:, 1])
+
+    def pmin(self):
+        """Minimum across paths, on the device."""
+        return self._summary(self._minmax()[0], None)
+
+    def pmax(self):
+        """Maximum across paths, on the device."""
+        return self._summary(self._minmax()[1], None)
+
+    def psum(self, dtype=None):
+        """Sum across paths (FP64 accumulation on the device)."""
+        n, mean, _ = self._moments()
+        return self._summary(mean * n, dtype)
+
+    def pmean(self, dtype=None):
+        """Mean across paths (FP64 accumulation on the device)."""
+        _, mean, _ = self._moments()
+        return self._summary(mean, dtype)
+
+    def pvar(self, dtype=None, ddof=0):
+        """Variance across paths (FP64 accumulation on the device)."""
+        n, _, m2 = self._moments()
+        return self._summary(m2 / (n - ddof), dtype)
+
+    def pstd(self, dtype=None, ddof=0):
+        """Standard deviation across paths (FP64 on the device)."""
+        n, _, m2 = self._moments()
+        return self._summary(np.sqrt(m2 / (n - ddof)), dtype)
+
+    def montecarlo(self, row=-1, **args):
+        """A ``montecarlo`` of the output at timeline index ``row`` (all stored
+        values of that time point), reduced on the device."""
+        from .montecarlo import montecarlo
+        n_t, nout, P = self.xx_dev.shape
+        return montecarlo(self.xx_dev[row].reshape(self.sde.xshape + (P,)),
+                          **args)
+
 
 # --------------------------------------------------------------------------
 # presets (integration.py:2056-2770)

@@ -376,3 +376,45 @@ def test_lognorm_adjoint_delta_vs_black_scholes():
     assert abs(g[0].mean() - d) < 3.5 * g[0].std() / math.sqrt(P)
     assert abs(pay.mean() - cf.bscall(1.1, 2., 1., .05, 0., .3)) < \
         3.5 * pay.std() / math.sqrt(P)
+
+
+# ---------------------------------------------------------------------------
+# montecarlo restatement (infrastructure.py:2716-3083), pinned by the
+# reference's own cumulation/antithetics checks (test_montecarlo.py:166-215,
+# 140-163) and by scipy's moment estimators
+# ---------------------------------------------------------------------------
+
+def test_montecarlo_oracle_cumulation_and_moments():
+    import scipy.stats
+    from oracle.montecarlo import Montecarlo
+    rng = np.random.default_rng(17)
+    s = 1 + rng.standard_normal((3, 2, 400)) * np.array([[1.], [3.]])
+    a = Montecarlo()
+    a.update(s)
+    b = Montecarlo()
+    b.update(s[..., :100])
+    b.update(s[..., 100:250])
+    b.update(s[..., 250:])
+    assert np.allclose(a.mean, s.mean(axis=-1), rtol=1e-14)
+    assert np.allclose(b.mean, a.mean, rtol=1e-14)
+    assert np.allclose(a.stderr(), s.std(axis=-1) / np.sqrt(399), rtol=1e-12)
+    assert np.allclose(b.stderr(), a.stderr(), rtol=1e-12)
+    assert np.allclose(a.skew(), scipy.stats.skew(s, axis=-1), rtol=1e-10)
+    # sdepy's kurtosis() is Pearson's (its -3 applies to paths < 2 only,
+    # infrastructure.py:3060-3066)
+    assert np.allclose(a.kurtosis(), scipy.stats.kurtosis(s, axis=-1, fisher=False),
+                       rtol=1e-10)
+    # same bins in three runs == one run
+    c = Montecarlo(bins=a.edges)
+    c.update(s[..., :100])
+    c.update(s[..., 100:250])
+    c.update(s[..., 250:])
+    for i in np.ndindex(a.counts.shape):
+        assert np.array_equal(c.counts[i], a.counts[i])
+        assert c.counts[i].sum() + c.outside[i] == 400
+    # antithetic folding: odd mean = mean of half-differences
+    t = rng.standard_normal((5, 60))
+    odd = Montecarlo(use='odd')
+    odd.update(t, axis=-1)
+    assert odd.paths == 30
+    assert np.allclose(odd.mean, (t[:, :30] - t[:, 30:]).mean(axis=-1) / 2, rtol=1e-14)
