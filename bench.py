@@ -59,9 +59,10 @@ VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
 # (FETCH reads half the copied bytes: x2.0; WRITE is exact: x1.0), and the
 # kernel's VALU wave-instructions per launch (SQ_INSTS_VALU); the summaries
 # they come from are in profiles/
-ADJ_TRAFFIC_BYTES = None
-ADJ_VALU_WAVE_INSTS = None
-OU_TRAFFIC_BYTES = None
+ADJ_TRAFFIC_BYTES = 1376644023        # k_heston_adjoint<float, 8>, 4M x 365
+ADJ_VALU_WAVE_INSTS = 2229140625      # of which transcendental:
+ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
+OU_TRAFFIC_BYTES = 16017922595         # k_euler<OU, float, 4>, 1M x 1000
 
 
 def _events(torch):
@@ -145,7 +146,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--legs', default='adjoint,fd,ou,merton,gbm,batched,cpu',
                     help='comma list of: adjoint, fd, ou, merton, gbm, batched, '
                          'cpu, copyref (a 1 GiB device copy used to calibrate '
@@ -153,6 +154,9 @@ def main():
                          'leg for its error figure; on by default with ou), '
                          'batched64 (FP64 twins of the batched instances; on '
                          'by default with batched)')
+    ap.add_argument('--skip', default='',
+                    help='comma list of legs to leave out (e.g. ou64,batched64 '
+                         'under a profiler)')
     ap.add_argument('--cpu-paths', type=int, default=30000)
     args = ap.parse_args()
     legs = set(args.legs.split(','))
@@ -160,6 +164,7 @@ def main():
         legs.add('ou64')
     if 'batched' in legs:
         legs.add('batched64')
+    legs -= set(args.skip.split(','))
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build runs the path kernels on '
               'a single MI355X (paths shard by range across GPUs, see '
@@ -213,9 +218,13 @@ def main():
         'flops_per_path_step': ADJ_FLOPS_PER_PATH_STEP,
         'frac_of_unpacked_fp32_peak': achieved / (FP32_VECTOR_PEAK_TFLOPS / 2)}
     if ADJ_VALU_WAVE_INSTS:
-        rate = ADJ_VALU_WAVE_INSTS / (ms * 1e-3)
+        # the kernel's true ceiling: non-transcendental VALU wave-instructions
+        # issued per CU per clock (rocprof SQ_INSTS_VALU minus
+        # SQ_INSTS_VALU_TRANS_F32 per launch, over the measured launch time)
+        rate = (ADJ_VALU_WAVE_INSTS - ADJ_VALU_TRANS_WAVE_INSTS) / (ms * 1e-3)
         out['roofline']['valu_issue'] = {
             'wave_insts_per_launch': ADJ_VALU_WAVE_INSTS,
+            'transcendental_wave_insts_per_launch': ADJ_VALU_TRANS_WAVE_INSTS,
             'achieved_per_s': rate, 'peak_per_s': VALU_ISSUE_PEAK_PER_S,
             'frac': rate / VALU_ISSUE_PEAK_PER_S}
 
@@ -252,7 +261,7 @@ def main():
             paths=10 ** 6, vshape=4, dtype=np.float32, x0=0., theta=th, k=kk,
             sigma=sg, corr=C, rng=np.random.default_rng(1))
         ou = p.prepare(np.linspace(0., 1., 1001))
-        ou_ms, _ = timed(torch, lambda: ou.launch(fresh=True), 5, 2)
+        ou_ms, _ = timed(torch, lambda: ou.launch(fresh=True), 10, 5)
         nbytes = 1001 * 4 * 10 ** 6 * 4
         gbs = nbytes / (ou_ms * 1e-3) / 1e9
         out['ou_fullpath'] = {
@@ -286,7 +295,7 @@ def main():
                     strike=1., maturity=1., steps=252, paths=2_000_000,
                     key_w=(0x3e47, 0x11), key_j=(0x3e47, 0x12), epoch=0)
         mrun = G.LogGreeks(dtype=np.float32, **mprm)
-        m_ms, _ = timed(torch, mrun.launch, 10, 2)
+        m_ms, _ = timed(torch, mrun.launch, 20, 10)
         mres = mrun.result()
         m64 = G.LogGreeks(dtype=np.float64, **mprm)
         m64.launch()
@@ -313,12 +322,12 @@ def main():
                                       dtype=np.float64,
                                       rng=np.random.default_rng(1234))
         prep = c1_proc().prepare(tt1)
-        k_ms, _ = timed(torch, lambda: prep.launch(fresh=True), 20, 3)
+        k_ms, _ = timed(torch, lambda: prep.launch(fresh=True), 20, 10)
         t0 = time.perf_counter()
         reps = 5
         for _ in range(reps):
             x = c1_proc()(tt1)
-            m, v = x[-1].pmean(), x[-1].pvar()
+            m, v = x.pmean()[-1], x.pvar()[-1]
         e2e = (time.perf_counter() - t0) / reps
         out['gbm_c1'] = {
             'workload': 'gbm_fullpath_10kx252_fp64',
@@ -345,9 +354,9 @@ def main():
             for (p64, tl), mu32 in zip(B.heterogeneous_specs(1024, seed=1234,
                                                              dtype=np.float64),
                                        mean):
-                xt = p64(tl)[-1]
-                m64 = float(np.asarray(xt.pmean()).ravel()[0])
-                v64 = float(np.asarray(xt.pvar()).ravel()[0])
+                xt = p64(tl)
+                m64 = float(np.asarray(xt.pmean()[-1]).ravel()[0])
+                v64 = float(np.asarray(xt.pvar()[-1]).ravel()[0])
                 se = math.sqrt(v64 / (p64.paths - 1))
                 errs.append((abs(mu32 - m64) / se, abs(mu32 - m64) / max(abs(m64), 1e-300)))
             errs = np.array(errs)

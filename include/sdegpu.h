@@ -180,6 +180,17 @@ int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, int64_t path
                         const double* edges, int32_t nbins, unsigned long long* counts,
                         void* stream);
 
+/*
+ * Characteristic function estimator of a [rows][paths] sample
+ * (process.chf, infrastructure.py:1125-1167): out [rows][nu][2] = FP64 sums
+ * over the paths of cos(u[k] x) and sin(u[k] x) (divide by paths for the
+ * mean of exp(i u x)).  rows <= 65535, nu <= 65535.
+ */
+size_t sdegpu_chf_workspace(int64_t rows, int64_t paths, int32_t nu);
+int sdegpu_path_chf(const void* x, int32_t dtype, int64_t rows, int64_t paths, const double* u,
+                    int32_t nu, double* out, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
 size_t sdegpu_payoff_workspace(int64_t paths);
 int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t kind, double strike,
                           double scale, double* out, void* workspace, size_t workspace_bytes,

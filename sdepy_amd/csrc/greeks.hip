@@ -10,7 +10,7 @@
 //
 // Heston: the variance recursion is not invertible (max(y, 0)), so the
 // reverse sweep needs the forward states.  They are not stored: pass 1 runs
-// forward and keeps the variance every C steps (checkpoints, C = 16); the
+// forward and keeps the variance every C steps (checkpoints, C = 8); the
 // reverse sweep then walks the segments backwards, recomputing each segment's
 // variances and normals from its checkpoint and the Philox counters into
 // registers, and sweeping them in reverse.  Memory: ceil(n/C) values per path.
@@ -19,7 +19,10 @@
 
 namespace sdegpu {
 
-constexpr int kHestonSeg = 16;  // steps per checkpoint segment (even: one Philox block per 2 steps)
+// Steps per checkpoint segment (even: one Philox block feeds 2 steps).  8
+// keeps the segment's registers (3 values per step) small enough for 8 waves
+// per SIMD; the checkpoints then cost ~0.7 GB of traffic at 4M x 365.
+constexpr int kHestonSeg = 8;
 constexpr int kHestonNG = 7;    // x0, r, k, theta, xi, rho, y0
 constexpr int kHestonNS = 2 + 2 * kHestonNG;
 constexpr int kLogNG = 5;       // x0, mu, sigma, a, b
@@ -207,7 +210,7 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const T* __r
 }
 
 template <typename T, int C>
-__global__ __launch_bounds__(kBlock) void k_heston_adjoint(const HestonArgs a,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoint(const HestonArgs a,
                                                            const T* __restrict__ tb,
                                                            T* __restrict__ ck,
                                                            const double* __restrict__ sum_ds) {

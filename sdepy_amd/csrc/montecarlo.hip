@@ -225,6 +225,56 @@ __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x
   }
 }
 
+// Characteristic function estimator (process.chf, infrastructure.py:
+// 1125-1167): for row r and argument u[k], the FP64 sums over the paths of
+// cos(u x) and sin(u x).  One block per (chunk, row, k).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_chf_partial(const T* __restrict__ x, int64_t paths,
+                                                        int64_t nchunks,
+                                                        const double* __restrict__ u, int nu,
+                                                        double* __restrict__ part) {
+  const int64_t chunk = blockIdx.x;
+  const int64_t row = blockIdx.y;
+  const int k = blockIdx.z;
+  const T* xr = x + row * paths;
+  const double uk = u[k];
+  const int64_t lo = chunk * kMcChunk;
+  const int64_t hi = lo + kMcChunk < paths ? lo + kMcChunk : paths;
+  double sc = 0.0, ss = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    double s, c;
+    sincos(uk * (double)xr[i], &s, &c);
+    sc += c;
+    ss += s;
+  }
+  __shared__ double sh[2][kBlock / 64];
+  sc = mc_wave_sum(sc);
+  ss = mc_wave_sum(ss);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[0][wave] = sc;
+    sh[1][wave] = ss;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double a = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) a += sh[threadIdx.x][w];
+    part[((row * nu + k) * nchunks + chunk) * 2 + threadIdx.x] = a;
+  }
+}
+
+__global__ void k_chf_final(int64_t n, int64_t nchunks, const double* __restrict__ part,
+                            double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * n) return;
+  const int64_t rk = t / 2;
+  const int j = (int)(t % 2);
+  double a = 0.0;
+  for (int64_t c = 0; c < nchunks; ++c) a += part[(rk * nchunks + c) * 2 + j];
+  out[t] = a;
+}
+
 struct McShape {
   int64_t n, nchunks;
 };
@@ -255,6 +305,38 @@ extern "C" size_t sdegpu_mc_workspace(int64_t rows, int64_t paths) {
   if (rows < 1 || paths < 1) return 0;
   const int64_t nchunks = (paths + kMcChunk - 1) / kMcChunk;
   return (size_t)rows * (size_t)nchunks * kMcMoments * sizeof(double);
+}
+
+extern "C" size_t sdegpu_chf_workspace(int64_t rows, int64_t paths, int32_t nu) {
+  if (rows < 1 || paths < 1 || nu < 1) return 0;
+  const int64_t nchunks = (paths + kMcChunk - 1) / kMcChunk;
+  return (size_t)rows * (size_t)nu * (size_t)nchunks * 2 * sizeof(double);
+}
+
+extern "C" int sdegpu_path_chf(const void* x, int32_t dtype, int64_t rows, int64_t paths,
+                               const double* u, int32_t nu, double* out, void* ws,
+                               size_t ws_bytes, void* stream) {
+  McShape sh;
+  if (int e = mc_check(x, dtype, rows, paths, 0, &sh, "path_chf")) return e;
+  if (!u || !out || nu < 1 || nu > 65535)
+    return fail(SDEGPU_EINVAL, "path_chf: %d arguments (1..65535) or null pointers", nu);
+  const size_t need = sdegpu_chf_workspace(rows, paths, nu);
+  if (!ws || ws_bytes < need)
+    return fail(SDEGPU_ENOMEM, "path_chf: workspace of %zu bytes, %zu needed", ws_bytes, need);
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)sh.nchunks, (unsigned)rows, (unsigned)nu);
+  double* part = (double*)ws;
+  if (dtype == SDEGPU_F32)
+    hipLaunchKernelGGL(k_chf_partial<float>, grid, dim3(kBlock), 0, st, (const float*)x, paths,
+                       sh.nchunks, u, nu, part);
+  else
+    hipLaunchKernelGGL(k_chf_partial<double>, grid, dim3(kBlock), 0, st, (const double*)x, paths,
+                       sh.nchunks, u, nu, part);
+  if (int e = launch_status("k_chf_partial")) return e;
+  const int64_t n = rows * (int64_t)nu;
+  hipLaunchKernelGGL(k_chf_final, dim3(blocks_for(2 * n)), dim3(kBlock), 0, st, n, sh.nchunks,
+                     (const double*)part, out);
+  return launch_status("k_chf_final");
 }
 
 extern "C" int sdegpu_mc_minmax(const void* x, int32_t dtype, int64_t rows, int64_t paths,

@@ -877,6 +877,144 @@ class process(np.ndarray):
             axis=0, dtype=dtype, out=out, ddof=ddof, keepdims=True))
 
 
+    # increments and integrals along the timeline (infrastructure.py:
+    # 1032-1121), host NumPy ------------------------------------------------
+
+    def tdiff(self, dt_exp=0, fwd=True):
+        """q[i] = (p[i+1] - p[i]) / (t[i+1] - t[i])**dt_exp, stored at t[i]
+        (fwd) or t[i+1]."""
+        t = self.t[:-1] if fwd else self.t[1:]
+        x = np.diff(self.view(np.ndarray), axis=0)
+        if dt_exp:
+            np.divide(x, self.dtx ** dt_exp, out=x)
+        return process(t=t, x=x)
+
+    def tder(self):
+        """Forward-looking time derivative: ``tdiff(dt_exp=1)``."""
+        return self.tdiff(fwd=True, dt_exp=1)
+
+    def tint(self):
+        """Integral along the timeline, linearly interpolated between time
+        points (left rectangles, as sdepy)."""
+        x = np.empty(self.shape, dtype=self.dtype)
+        x[0] = 0
+        x[1:] = self.view(np.ndarray)[:-1] * self.dtx
+        return process(t=self.t, x=x.cumsum(axis=0, out=x))
+
+    # distribution estimators (infrastructure.py:1125-1209), reduced across
+    # paths on the GPU ------------------------------------------------------
+
+    def _values_at(self, t, v):
+        """(t, v, values shaped t.shape + vshape + (paths,)) for the chf/cdf
+        argument conventions."""
+        if t is None and v is None:
+            raise TypeError('{} argument missing'.format('u or x'))
+        if v is None:
+            t, v = None, t
+        if t is None:
+            return self.t, np.asarray(v), self.view(np.ndarray)
+        t, v = np.asarray(t), np.asarray(v)
+        return t, v, np.asarray(self(t))
+
+    def chf(self, t=None, u=None):
+        """Characteristic function estimate: the mean across paths of
+        exp(1j*u*p(t)), shaped t.shape + u.shape + vshape."""
+        if t is None and u is None:
+            raise TypeError('u argument missing')
+        t, u, x = self._values_at(t, u)
+        vshape = x.shape[t.ndim:-1]
+        s = path_chf(x.reshape((-1, x.shape[-1])), u.reshape(-1))
+        s = s.reshape(t.shape + vshape + u.shape)
+        return np.moveaxis(s, tuple(range(t.ndim + len(vshape), s.ndim)),
+                           tuple(range(t.ndim, t.ndim + u.ndim)))
+
+    def cdf(self, t=None, x=None):
+        """Cumulative distribution estimate: the fraction of paths with
+        p(t) <= x, shaped t.shape + x.shape + vshape."""
+        if t is None and x is None:
+            raise TypeError('x argument missing')
+        t, x, y = self._values_at(t, x)
+        vshape = y.shape[t.ndim:-1]
+        c = path_cdf(y.reshape((-1, y.shape[-1])), x.reshape(-1))
+        c = c.reshape(t.shape + vshape + x.shape)
+        return np.moveaxis(c, tuple(range(t.ndim + len(vshape), c.ndim)),
+                           tuple(range(t.ndim, t.ndim + x.ndim)))
+
+
+def _device_rows(x, device_tensor):
+    x = np.asarray(x) if device_tensor is None else x
+    if device_tensor is not None:
+        torch = _lib.torch_mod()
+        code = _lib.F32 if device_tensor.dtype == torch.float32 else _lib.F64
+        return device_tensor, code, int(device_tensor.shape[0]), int(device_tensor.shape[1])
+    if x.dtype.kind not in 'fiub':
+        raise NotImplementedError(
+            'the GPU path reductions take real processes, not {}'.format(x.dtype))
+    code, cdt, _ = _gpu_dtype(x.dtype if x.dtype.kind == 'f' else np.float64)
+    _lib.require_gpu()
+    return (_lib.to_device(np.ascontiguousarray(x, dtype=cdt)), code,
+            int(x.shape[0]), int(x.shape[1]))
+
+
+def path_chf(x, u, device_tensor=None):
+    """mean over the last axis of exp(1j*u[k]*x[r]) for a [rows][paths]
+    sample: complex [rows][nu] (``sdegpu_path_chf``, FP64 sums)."""
+    dx, code, rows, paths = _device_rows(x, device_tensor)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    out = np.empty((rows, u.size), dtype=complex)
+    if u.size == 0 or rows == 0:
+        return out
+    lib = _lib.lib()
+    du = _lib.to_device(u)
+    for r0 in range(0, rows, 65535):
+        r1 = min(rows, r0 + 65535)
+        for k0 in range(0, u.size, 65535):
+            k1 = min(u.size, k0 + 65535)
+            res = _lib.empty((r1 - r0, k1 - k0, 2), np.float64)
+            nbytes = lib.sdegpu_chf_workspace(r1 - r0, paths, k1 - k0)
+            ws = _lib.workspace(nbytes)
+            _lib.check(lib.sdegpu_path_chf(
+                _lib.ptr(dx[r0:r1]), code, r1 - r0, paths, _lib.ptr(du[k0:k1]),
+                k1 - k0, _lib.ptr(res), _lib.ptr(ws), nbytes, _lib.stream()))
+            h = _lib.to_host(res)
+            out[r0:r1, k0:k1] = (h[..., 0] + 1j * h[..., 1]) / paths
+    return out
+
+
+def path_cdf(y, x, device_tensor=None):
+    """fraction of the values of each row of a [rows][paths] sample that
+    are <= x[k]: float [rows][nx].  Counted on the GPU as a histogram whose
+    bins are (-inf, s_0], (s_0, s_1], ... over the sorted thresholds s
+    (``sdegpu_mc_histogram`` with edges next above each threshold)."""
+    dy, code, rows, paths = _device_rows(y, device_tensor)
+    x = np.asarray(x, dtype=np.float64).reshape(-1)
+    out = np.zeros((rows, x.size))
+    ok = ~np.isnan(x)
+    if not ok.any() or rows == 0:
+        return out
+    s = np.unique(x[ok])
+    edges = np.concatenate(([-np.inf], np.nextafter(s, np.inf), [np.inf]))
+    nb = edges.size - 1
+    lib = _lib.lib()
+    de = _lib.to_device(np.broadcast_to(edges, (min(rows, 65535), edges.size)).copy())
+    counts = np.empty((rows, nb + 1), dtype=np.int64)
+    for r0 in range(0, rows, 65535):
+        r1 = min(rows, r0 + 65535)
+        c = _lib.empty((r1 - r0, nb + 1), np.int64)
+        _lib.check(lib.sdegpu_mc_histogram(
+            _lib.ptr(dy[r0:r1]), code, r1 - r0, paths, 0, _lib.ptr(de[:r1 - r0]),
+            nb, _lib.ptr(c), _lib.stream()))
+        counts[r0:r1] = _lib.to_host(c)
+    cum = np.cumsum(counts[:, :nb], axis=1)       # cum[:, i]: values <= s_i
+    idx = np.searchsorted(s, x[ok])
+    res = cum[:, idx].astype(float)
+    inf = np.isposinf(x[ok])
+    if inf.any():                                   # v <= +inf: every non-NaN value
+        res[:, inf] = cum[:, -1][:, None]
+    out[:, ok] = res / paths
+    return out
+
+
 def path_moments(x, device_tensor=None, paths=None):
     """FP64 (n, mean, m2) over the last axis of ``x``, reduced on the GPU
     (``sdegpu_path_moments``).  ``device_tensor``, when given, is a device

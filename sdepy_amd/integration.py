@@ -546,6 +546,26 @@ class PreparedRun:
         n, _, m2 = self._moments()
         return self._summary(np.sqrt(m2 / (n - ddof)), dtype)
 
+    def chf(self, u):
+        """process.chf(u) on the device output: mean across paths of
+        exp(1j*u*x), shaped (N_t,) + u.shape + xshape."""
+        from .infrastructure import path_chf
+        u = np.asarray(u)
+        c = path_chf(None, u, device_tensor=self._rows())
+        c = c.reshape(self.tt.shape + self.sde.xshape + u.shape)
+        return np.moveaxis(c, tuple(range(1 + len(self.sde.xshape), c.ndim)),
+                           tuple(range(1, 1 + u.ndim)))
+
+    def cdf(self, x):
+        """process.cdf(x) on the device output: fraction of paths with
+        values <= x, shaped (N_t,) + x.shape + xshape."""
+        from .infrastructure import path_cdf
+        x = np.asarray(x)
+        c = path_cdf(None, x, device_tensor=self._rows())
+        c = c.reshape(self.tt.shape + self.sde.xshape + x.shape)
+        return np.moveaxis(c, tuple(range(1 + len(self.sde.xshape), c.ndim)),
+                           tuple(range(1, 1 + x.ndim)))
+
     def montecarlo(self, row=-1, **args):
         """A ``montecarlo`` of the output at timeline index ``row`` (all stored
         values of that time point), reduced on the device."""

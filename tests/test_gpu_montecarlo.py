@@ -124,3 +124,39 @@ def test_device_summaries_of_a_prepared_run(gpu):
     X, Y = hr.result()
     mX, mY = hr.pmean()
     assert np.allclose(mX, X.pmean(), rtol=1e-12) and np.allclose(mY, Y.pmean(), rtol=1e-12)
+
+
+def test_process_chf_cdf_tdiff_tint(gpu):
+    """process.chf / cdf (infrastructure.py:1125-1209) against their NumPy
+    definitions, including values equal to the thresholds, +-inf and NaN
+    thresholds, interpolated time points, and the device-side forms of a
+    prepared run; tdiff / tder / tint (:1032-1121) against np.diff/cumsum."""
+    g = np.random.default_rng(11)
+    t = np.linspace(0., 1., 6)
+    x = np.round(g.standard_normal((6, 2, 3000)), 1).astype(np.float32)
+    p = sa.process(t, x=x)
+    u = np.array([[-2., .5], [1., 3.]])
+    ref = np.exp(1j * u.reshape(1, 2, 2, 1, 1) * x.reshape(6, 1, 1, 2, 3000)).mean(axis=-1)
+    assert np.allclose(p.chf(u), ref, rtol=0, atol=1e-12)
+    thr = np.array([-np.inf, -.5, 0., .1, .3, np.nan, np.inf])
+    ref = (x.reshape(6, 1, 2, 3000) <= thr.reshape(1, 7, 1, 1)).sum(axis=-1) / 3000
+    assert np.array_equal(p.cdf(thr), ref)
+    ti = np.array([.1, .55])
+    xi = np.asarray(p(ti))
+    assert np.array_equal(p.cdf(ti, thr), (xi.reshape(2, 1, 2, 3000) <= thr.reshape(1, 7, 1, 1)).sum(-1) / 3000)
+    assert np.allclose(p.chf(ti, 1.5),
+                       np.exp(1.5j * xi.astype(float)).mean(axis=-1), atol=1e-12)
+    # timeline summaries
+    q = sa.process(t, x=g.standard_normal((6, 3, 50)))
+    dt = np.diff(t).reshape(5, 1, 1)
+    assert np.allclose(q.tdiff(), np.diff(q, axis=0)) and np.array_equal(q.tdiff().t, t[:-1])
+    assert np.allclose(q.tder(), np.diff(q, axis=0) / dt)
+    assert np.array_equal(q.tdiff(fwd=False).t, t[1:])
+    assert np.allclose(q.tint()[1:], np.cumsum(np.asarray(q)[:-1] * dt, axis=0))
+    # the device forms on a prepared run
+    r = sa.lognorm_process(paths=4000, dtype=np.float64, x0=1., mu=.1, sigma=.3,
+                           rng=np.random.default_rng(12)).prepare(np.linspace(0., 1., 5))
+    r.launch()
+    xr = r.result()
+    assert np.allclose(r.chf(np.array([.3, -1.])), xr.chf(np.array([.3, -1.])), atol=1e-13)
+    assert np.array_equal(r.cdf(np.array([.9, 1., 1.2])), xr.cdf(np.array([.9, 1., 1.2])))
