@@ -328,6 +328,10 @@ class _norm_rv:
         b = self.b(t) if callable(self.b) else self.b
         return a, b
 
+    def at(self, t):
+        """The distribution at time t (callable parameters evaluated)."""
+        return _norm_rv(*self._ab(t))
+
     def rvs(self, size, random_state=None):
         a, b = self._ab()
         return scipy.stats.norm(loc=a, scale=b).rvs(size=size,
@@ -435,8 +439,13 @@ class cpoisson_source(_philox_source):
         y_value = []
         for i in np.ndindex(dt.shape):
             dnp = sign_dt[i] * dn[i]
-            rv = self.y(t[i] + dt[i] / 2) if (callable(self.y) and
-                                             not hasattr(self.y, 'rvs')) else self.y
+            ti = t[i] + dt[i] / 2
+            if isinstance(self.y, _norm_rv):
+                rv = self.y.at(ti)
+            elif callable(self.y) and not hasattr(self.y, 'rvs'):
+                rv = self.y(ti)
+            else:
+                rv = self.y
             for j in range(1, int(dnp.max(initial=0)) + 1):
                 index = dnp == j
                 if index.any():
@@ -516,16 +525,28 @@ class even_cpoisson_source(_antithetics(cpoisson_source, 1)):
     infrastructure.py:2139-2156): ``x[..., i] == x[..., K + i]``."""
 
 
+def _rv_params_variable(make, **params):
+    """``make(**params)``, or, when a parameter is callable in t, a callable
+    ``t -> make(**params at t)`` as cpoisson_source accepts it (sdepy's
+    _make_rv_params_variable, infrastructure.py:1640-1650)."""
+    if any(callable(x) for x in params.values()):
+        return lambda t: make(**{k: (x(t) if callable(x) else x)
+                                 for k, x in params.items()})
+    return make(**params)
+
+
 class _uniform_rv:
+    """Uniform on [a, b] (infrastructure.py:1723-1728), sampled on the host."""
+
     def __init__(self, a=0., b=1.):
-        self.a, self.b = a, b
+        self.a, self.b = np.broadcast_arrays(a, b)
 
     def rvs(self, size, random_state=None):
         return scipy.stats.uniform(loc=self.a, scale=self.b - self.a).rvs(
             size=size, random_state=random_state)
 
     def exp_mean(self):
-        a, b = np.asarray(self.a), np.asarray(self.b)
+        a, b = self.a, self.b
         return (np.exp(b) - np.exp(a)) / (b - a) + 0
 
 
@@ -534,9 +555,109 @@ def _uniform_default():
 
 
 def uniform_rv(a=0., b=1.):
-    """Uniform jump sizes on [a, b] (sdepy/infrastructure.py:1667-1677, 1723-1728);
-    sampled on the host through ``rvs``."""
-    return _uniform_rv(a, b)
+    """Uniform jump sizes on [a, b], possibly time-dependent
+    (sdepy/infrastructure.py:1667-1677); sampled on the host through
+    ``rvs``."""
+    return _rv_params_variable(_uniform_rv, a=a, b=b)
+
+
+class _exp_rv:
+    """Exponential with scale |a|, on the side of the sign of a
+    (infrastructure.py:1731-1754)."""
+
+    def __init__(self, a=1.):
+        a = self._a = np.asarray(a)
+        if (a == 0).any():
+            raise ValueError('domain error in arguments')
+        self._s = np.sign(a)
+        self._rv = scipy.stats.expon(scale=np.abs(a))
+
+    def rvs(self, size, random_state=None):
+        return self._s * self._rv.rvs(size=size, random_state=random_state)
+
+    def mean(self):
+        return self._s * self._rv.mean()
+
+    def var(self):
+        return self._rv.var()
+
+    def std(self):
+        return np.sqrt(self._rv.var())
+
+    def exp_mean(self):
+        a = self._a
+        return np.where(a < 1, 1 / (1 - a), np.inf) + 0
+
+
+def exp_rv(a=1.):
+    """Exponential jump sizes with scale a, possibly time-dependent
+    (infrastructure.py:1680-1693); sampled on the host."""
+    return _rv_params_variable(_exp_rv, a=a)
+
+
+class _double_exp_rv:
+    """Scale a with probability pa, -b with probability 1 - pa
+    (infrastructure.py:1757-1796)."""
+
+    def __init__(self, a=1., b=1., pa=.5):
+        a, b, pa, pb = self._a, self._b, self._pa, self._pb = \
+            np.broadcast_arrays(a, b, pa, 1 - np.asarray(pa))
+        if (a <= 0).any() or (b <= 0).any() or (pa > 1).any() or (pa < 0).any():
+            raise ValueError('domain error in arguments')
+        self._rvxa = scipy.stats.expon(scale=a)
+        self._rvxb = scipy.stats.expon(scale=b)
+        self._rvu = scipy.stats.uniform(scale=1.)
+
+    def rvs(self, size, random_state=None):
+        plus = self._rvxa.rvs(size=size, random_state=random_state)
+        minus = self._rvxb.rvs(size=size, random_state=random_state)
+        u = self._rvu.rvs(size=size, random_state=random_state)
+        return np.where(u <= self._pa, plus, -minus) + 0
+
+    def mean(self):
+        return (self._pa * self._a - self._pb * self._b) + 0
+
+    def var(self):
+        a, b, pa, pb = self._a, self._b, self._pa, self._pb
+        return pa * pb * (a + b) ** 2 + (pa * a ** 2 + pb * b ** 2) + 0
+
+    def std(self):
+        return np.sqrt(self.var())
+
+    def exp_mean(self):
+        a, b, pa, pb = self._a, self._b, self._pa, self._pb
+        return (np.where(a < 1, pa / (1 - np.where(a < 1, a, 0)), np.inf)
+                + pb / (1 + b) + 0)
+
+
+def double_exp_rv(a=1., b=1., pa=.5):
+    """Double exponential jump sizes (Kou), possibly time-dependent
+    (infrastructure.py:1696-1712); sampled on the host."""
+    return _rv_params_variable(_double_exp_rv, a=a, b=b, pa=pa)
+
+
+def rvmap(f, y):
+    """Map f over the variates of y, possibly time-dependent
+    (infrastructure.py:1799-1864): ``f(y)``, or ``f(t, y)`` when f's first
+    parameter is named t or s."""
+    import inspect
+    first = next(iter(inspect.signature(f).parameters), None)
+    time_dependent_f = first in ('t', 's')
+    if callable(y) or time_dependent_f:
+        def new_y(t):
+            yt = y(t) if callable(y) else y
+
+            class _mapped:
+                def rvs(self, size, random_state=None):
+                    v = yt.rvs(size=size, random_state=random_state)
+                    return f(t, v) if time_dependent_f else f(v)
+            return _mapped()
+        return new_y
+
+    class _mapped:
+        def rvs(self, size, random_state=None):
+            return f(y.rvs(size=size, random_state=random_state))
+    return _mapped()
 
 
 # --------------------------------------------------------------------------

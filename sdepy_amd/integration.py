@@ -30,6 +30,7 @@ from . import _lib
 from .infrastructure import (process, wiener_source, cpoisson_source,
                              poisson_source, norm_rv, per_component,
                              odd_wiener_source, even_cpoisson_source,
+                             double_exp_rv,
                              _gpu_dtype, _shape_setup, _get_default_rng)
 
 __all__ = ['wiener_process', 'lognorm_process', 'ornstein_uhlenbeck_process',
@@ -284,6 +285,12 @@ class _gpu_sde:
         process(es).  ``__call__`` is ``prepare(timeline).run()``."""
         tt, tt_asis, steps_tt = consolidate_timeline(timeline, self.steps,
                                                      self.i0)
+        if self._nvars() > _lib.MAX_VARS:
+            raise NotImplementedError(
+                'the GPU integrator steps at most {} working-state components '
+                'per path; {} with vshape {} needs {}'.format(
+                    _lib.MAX_VARS, type(self).__name__, self.vshape,
+                    self._nvars()))
         i0 = self.i0
         code, cdt, odt = _gpu_dtype(self.dtype)
         _lib.require_gpu()
@@ -853,20 +860,7 @@ class kou_jumpdiff_process(jumpdiff_process):
                    'a': .5, 'b': .5, 'pa': .5}
 
     def _y(self):
-        return _double_exp_rv(self._args['a'], self._args['b'],
-                              self._args['pa'])
+        return double_exp_rv(a=self._args['a'], b=self._args['b'],
+                             pa=self._args['pa'])
 
 
-class _double_exp_rv:
-    """Double exponential distribution (sdepy _double_exp_rv,
-    infrastructure.py:1756-1793), sampled on the host."""
-
-    def __init__(self, a=1., b=1., pa=.5):
-        self.a, self.b, self.pa = a, b, pa
-
-    def rvs(self, size, random_state=None):
-        import scipy.stats
-        plus = scipy.stats.expon(scale=self.a).rvs(size=size, random_state=random_state)
-        minus = scipy.stats.expon(scale=self.b).rvs(size=size, random_state=random_state)
-        u = scipy.stats.uniform(scale=1.).rvs(size=size, random_state=random_state)
-        return np.where(u <= self.pa, plus, -minus) + 0
