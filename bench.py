@@ -19,6 +19,8 @@ Printed as one JSON line:
               largest deviation from the FP64 run of the same paths
   merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
               mu, sigma, a, b), with the FP64 run of the same paths in SE units
+  summaries   moments, min/max, 100-bin histograms of every row of the OU
+              output and chf of its last row, on the device (HBM read passes)
   gbm_c1      GBM 10k x 252 FP64 full path + pmean/pvar (latency-bound):
               kernel time and the end-to-end preset call
   batched     1024 heterogeneous GBM / HW2F instances in one persistent
@@ -131,6 +133,69 @@ def cpu_baseline_subprocess(paths, steps):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
+def _device_summaries(torch, run):
+    """Time the cross-path summaries on a prepared run's device output (the
+    C2 full path, 4004 rows x 1M paths FP32 = 16 GB): FP64 moments of every
+    row (pmean/pvar), a 100-bin histogram of every row (montecarlo counts),
+    min/max of every row, and chf at 8 arguments of the last time point.
+    Each is a read pass over its input; bytes read / time vs 8 TB/s."""
+    import ctypes
+    from sdepy_amd import _lib
+    lib = _lib.lib()
+    rows = run.xx_dev.reshape(-1, run.xx_dev.shape[-1])
+    R, P = int(rows.shape[0]), int(rows.shape[1])
+    nbytes = R * P * 4
+    res = {}
+    mean = _lib.empty((R,), np.float64)
+    m2 = _lib.empty((R,), np.float64)
+    ws_n = lib.sdegpu_moments_workspace(R, P)
+    ws = _lib.workspace(ws_n)
+    st = _lib.stream()
+
+    def moments():
+        _lib.check(lib.sdegpu_path_moments(_lib.ptr(rows), _lib.F32, R, P,
+                                           _lib.ptr(mean), _lib.ptr(m2),
+                                           _lib.ptr(ws), ws_n, st))
+    ms, _ = timed(torch, moments, 5, 2)
+    res['moments'] = {'ms': ms, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9}
+    mc_n = lib.sdegpu_mc_workspace(R, P)
+    mws = _lib.workspace(mc_n)
+    mm = _lib.empty((R, 3), np.float64)
+
+    def minmax():
+        _lib.check(lib.sdegpu_mc_minmax(_lib.ptr(rows), _lib.F32, R, P, 0,
+                                        _lib.ptr(mm), _lib.ptr(mws), mc_n, st))
+    ms, _ = timed(torch, minmax, 5, 2)
+    res['minmax'] = {'ms': ms, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9}
+    h = _lib.to_host(mm)
+    # np.histogram's edges for 100 bins (a row of equal values widens to +-0.5)
+    edges = np.stack([np.histogram_bin_edges(np.array([lo, hi]), bins=100)
+                      for lo, hi in h[:, :2]])
+    de = _lib.to_device(edges)
+    counts = _lib.empty((R, 101), np.int64)
+
+    def hist():
+        _lib.check(lib.sdegpu_mc_histogram(_lib.ptr(rows), _lib.F32, R, P, 0,
+                                           _lib.ptr(de), 100, 1, _lib.ptr(counts),
+                                           st))
+    ms, _ = timed(torch, hist, 5, 2)
+    c = _lib.to_host(counts)
+    res['histogram_100_bins'] = {'ms': ms, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9,
+                                 'all_values_counted': bool((c.sum(axis=1) == P).all())}
+    last = rows[-4:]
+    u = _lib.to_device(np.linspace(-4., 4., 8))
+    cn = lib.sdegpu_chf_workspace(4, P, 8)
+    cws = _lib.workspace(cn)
+    cout = _lib.empty((4, 8, 2), np.float64)
+
+    def chf():
+        _lib.check(lib.sdegpu_path_chf(_lib.ptr(last), _lib.F32, 4, P, _lib.ptr(u),
+                                       8, _lib.ptr(cout), _lib.ptr(cws), cn, st))
+    ms, _ = timed(torch, chf, 5, 2)
+    res['chf_8_args_last_row'] = {'ms': ms, 'values_per_s': 4 * P * 8 / (ms * 1e-3)}
+    return res
+
+
 def _cpu_name():
     try:
         with open('/proc/cpuinfo') as f:
@@ -152,8 +217,9 @@ def main():
                          'cpu, copyref (a 1 GiB device copy used to calibrate '
                          'rocprof byte counters), ou64 (FP64 twin of the OU '
                          'leg for its error figure; on by default with ou), '
-                         'batched64 (FP64 twins of the batched instances; on '
-                         'by default with batched)')
+                         'summaries (device summaries of the OU output; on by '
+                         'default with ou), batched64 (FP64 twins of the '
+                         'batched instances; on by default with batched)')
     ap.add_argument('--skip', default='',
                     help='comma list of legs to leave out (e.g. ou64,batched64 '
                          'under a profiler)')
@@ -162,6 +228,7 @@ def main():
     legs = set(args.legs.split(','))
     if 'ou' in legs:
         legs.add('ou64')
+        legs.add('summaries')
     if 'batched' in legs:
         legs.add('batched64')
     legs -= set(args.skip.split(','))
@@ -272,6 +339,8 @@ def main():
                          'frac': gbs / HBM_PEAK_GBS,
                          'traffic': OU_TRAFFIC_BYTES,
                          'bytes_per_path_step': 16}}
+        if 'summaries' in legs:
+            out['summaries'] = _device_summaries(torch, ou)
         if 'ou64' in legs:
             # the same paths (key and last epoch) stepped in FP64
             p64 = sa.ornstein_uhlenbeck_process(

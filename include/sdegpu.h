@@ -168,6 +168,9 @@ int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t path
  *   with edges[r][i] <= v < edges[r][i+1] (the last bin also takes
  *   v == edges[r][nbins]), counts[r][nbins] = values outside the edges or NaN
  *   (np.histogram's rule); edges [rows][nbins + 1] increasing, FP64.
+ *   uniform != 0 promises equally spaced edges (np.histogram's int or named
+ *   bins): the bin is then estimated arithmetically and corrected against
+ *   the edges, with the same result as the binary search used otherwise.
  * Workspace: sdegpu_mc_workspace bytes (all three functions).
  */
 size_t sdegpu_mc_workspace(int64_t rows, int64_t paths);
@@ -177,8 +180,8 @@ int sdegpu_mc_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths,
                       const double* center, double* out, void* workspace, size_t workspace_bytes,
                       void* stream);
 int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, int64_t paths, int32_t use,
-                        const double* edges, int32_t nbins, unsigned long long* counts,
-                        void* stream);
+                        const double* edges, int32_t nbins, int32_t uniform,
+                        unsigned long long* counts, void* stream);
 
 /*
  * Characteristic function estimator of a [rows][paths] sample

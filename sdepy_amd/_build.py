@@ -72,7 +72,33 @@ def build(verbose=False, jobs=None):
     if res.returncode != 0:
         raise RuntimeError('link of libsdegpu.so failed:\n' + res.stderr)
     os.replace(tmp, LIB)
+    build_c_smoke(verbose)
     return LIB
+
+
+C_SMOKE_SRC = os.path.join(os.path.dirname(HERE), 'tests', 'c', 'abi_smoke.c')
+C_SMOKE_BIN = os.path.join(os.path.dirname(HERE), 'tests', 'c', 'abi_smoke')
+
+
+def build_c_smoke(verbose=False):
+    """Compile the plain-C caller of the library (tests/c/abi_smoke.c) with
+    gcc against include/sdegpu.h and the HIP runtime's C API; returns the
+    binary's path, or None when the test sources are not present."""
+    if not os.path.exists(C_SMOKE_SRC):
+        return None
+    rocm = os.environ.get('ROCM_PATH', '/opt/rocm')
+    cmd = ['gcc', '-std=c11', '-O2', '-D__HIP_PLATFORM_AMD__',
+           '-I', INCLUDE, '-I', os.path.join(rocm, 'include'),
+           C_SMOKE_SRC, '-o', C_SMOKE_BIN,
+           '-L', HERE, '-lsdegpu', '-Wl,-rpath,' + HERE,
+           '-L', os.path.join(rocm, 'lib'), '-lamdhip64',
+           '-Wl,-rpath,' + os.path.join(rocm, 'lib'), '-lm']
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('gcc failed on abi_smoke.c:\n' + res.stderr)
+    return C_SMOKE_BIN
 
 
 if __name__ == '__main__':

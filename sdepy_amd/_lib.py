@@ -76,7 +76,7 @@ SIGNATURES = [
       c_void_p, c_size_t, c_void_p]),
     ('sdegpu_mc_histogram', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p, c_int32,
-      c_void_p, c_void_p]),
+      c_int32, c_void_p, c_void_p]),
     ('sdegpu_chf_workspace', c_size_t, [c_int64, c_int64, c_int32]),
     ('sdegpu_path_chf', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int64, c_void_p, c_int32, c_void_p,

@@ -21,6 +21,7 @@
 // second kernel, so the floating point sums do not depend on scheduling.
 // Histogram counts are integers accumulated with atomics (order-free).
 #include "common.hpp"
+#include "span.hpp"
 
 namespace sdegpu {
 
@@ -33,6 +34,18 @@ __device__ __forceinline__ T mc_value(const T* __restrict__ xr, int64_t i, int64
   if (use == 0) return xr[i];
   const T a = xr[i], b = xr[i + half];
   return use == 1 ? (a + b) / (T)2 : (a - b) / (T)2;
+}
+
+// f(v) for the chunk's values: read as 16-byte vectors when no antithetic
+// folding is asked for, one value (or pair) per thread otherwise.
+template <typename T, typename F>
+__device__ __forceinline__ void mc_for_chunk(const T* __restrict__ xr, int64_t lo, int64_t hi,
+                                             int64_t half, int use, F& f) {
+  if (use == 0) {
+    for_span(xr, lo, hi, f);
+  } else {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) f(mc_value(xr, i, half, use));
+  }
 }
 
 __device__ __forceinline__ double mc_wave_sum(double v) {
@@ -66,15 +79,16 @@ __global__ __launch_bounds__(kBlock) void k_mc_minmax(const T* __restrict__ x, i
   const int64_t lo = chunk * kMcChunk;
   const int64_t hi = lo + kMcChunk < n ? lo + kMcChunk : n;
   double mn = __builtin_inf(), mx = -__builtin_inf(), nan = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const double v = (double)mc_value(xr, i, half, use);
+  auto visit = [&](T t) {
+    const double v = (double)t;
     if (v != v) {
       nan += 1.0;
     } else {
       mn = fmin(mn, v);
       mx = fmax(mx, v);
     }
-  }
+  };
+  mc_for_chunk(xr, lo, hi, half, use, visit);
   __shared__ double sh[3][kBlock / 64];
   mn = mc_wave_min(mn);
   mx = mc_wave_max(mx);
@@ -128,15 +142,16 @@ __global__ __launch_bounds__(kBlock) void k_mc_moments(const T* __restrict__ x, 
   const int64_t lo = chunk * kMcChunk;
   const int64_t hi = lo + kMcChunk < n ? lo + kMcChunk : n;
   double s[kMcMoments] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const double v = (double)mc_value(xr, i, half, use);
+  auto visit = [&](T t) {
+    const double v = (double)t;
     const double d = v - c, d2 = d * d;
     s[0] += d;
     s[1] += d2;
     s[2] += d2 * d;
     s[3] += d2 * d2;
     s[4] += v;
-  }
+  };
+  mc_for_chunk(xr, lo, hi, half, use, visit);
   __shared__ double sh[kMcMoments][kBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -169,6 +184,17 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// Bin of v for equally spaced edges: np.histogram's estimate
+// (v - e0) / (e_nb - e0) * nb, then moved to the bin whose edges hold v (the
+// estimate is off by at most one near an edge).
+__device__ __forceinline__ int mc_bin_uniform(const double* e, int nb, double v, double scale) {
+  int i = (int)((v - e[0]) * scale);
+  i = i < 0 ? 0 : (i > nb - 1 ? nb - 1 : i);
+  while (i > 0 && v < e[i]) --i;
+  while (i < nb - 1 && v >= e[i + 1]) ++i;
+  return i;
+}
+
 // Bin of v (edges[0] <= v <= edges[nb]): the largest i in [0, nb) with
 // edges[i] <= v.
 __device__ __forceinline__ int mc_bin(const double* e, int nb, double v) {
@@ -188,7 +214,7 @@ template <typename T, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x, int64_t paths,
                                                          int64_t n, int use,
                                                          const double* __restrict__ edges,
-                                                         int nb,
+                                                         int nb, int uniform,
                                                          unsigned long long* __restrict__ counts) {
   __shared__ double se[LDS ? kMcLdsBins + 1 : 1];
   __shared__ unsigned int sc[LDS ? kMcLdsBins + 1 : 1];
@@ -208,16 +234,20 @@ __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x
     e = se;
   }
   const double lo_edge = e[0], hi_edge = e[nb];
+  const double scale = hi_edge > lo_edge ? (double)nb / (hi_edge - lo_edge) : 0.0;
   const int64_t lo = chunk * kMcChunk;
   const int64_t hi = lo + kMcChunk < n ? lo + kMcChunk : n;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const double v = (double)mc_value(xr, i, half, use);
-    const int b = (v >= lo_edge && v <= hi_edge) ? mc_bin(e, nb, v) : nb;
+  auto visit = [&](T t) {
+    const double v = (double)t;
+    const int b = (v >= lo_edge && v <= hi_edge)
+                      ? (uniform ? mc_bin_uniform(e, nb, v, scale) : mc_bin(e, nb, v))
+                      : nb;
     if constexpr (LDS)
       atomicAdd(&sc[b], 1u);
     else
       atomicAdd(&cr[b], 1ull);
-  }
+  };
+  mc_for_chunk(xr, lo, hi, half, use, visit);
   if constexpr (LDS) {
     __syncthreads();
     for (int i = threadIdx.x; i <= nb; i += kBlock)
@@ -241,12 +271,13 @@ __global__ __launch_bounds__(kBlock) void k_chf_partial(const T* __restrict__ x,
   const int64_t lo = chunk * kMcChunk;
   const int64_t hi = lo + kMcChunk < paths ? lo + kMcChunk : paths;
   double sc = 0.0, ss = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+  auto visit = [&](T t) {
     double s, c;
-    sincos(uk * (double)xr[i], &s, &c);
+    sincos(uk * (double)t, &s, &c);
     sc += c;
     ss += s;
-  }
+  };
+  for_span(xr, lo, hi, visit);
   __shared__ double sh[2][kBlock / 64];
   sc = mc_wave_sum(sc);
   ss = mc_wave_sum(ss);
@@ -389,7 +420,7 @@ extern "C" int sdegpu_mc_moments(const void* x, int32_t dtype, int64_t rows, int
 
 extern "C" int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, int64_t paths,
                                    int32_t use, const double* edges, int32_t nbins,
-                                   unsigned long long* counts, void* stream) {
+                                   int32_t uniform, unsigned long long* counts, void* stream) {
   McShape sh;
   if (int e = mc_check(x, dtype, rows, paths, use, &sh, "mc_histogram")) return e;
   if (!edges || !counts) return fail(SDEGPU_EINVAL, "mc_histogram: null edges or counts");
@@ -403,17 +434,19 @@ extern "C" int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, i
   if (dtype == SDEGPU_F32) {
     if (lds)
       hipLaunchKernelGGL((k_mc_histogram<float, true>), grid, dim3(kBlock), 0, st,
-                         (const float*)x, paths, sh.n, use, edges, nbins, counts);
+                         (const float*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0, counts);
     else
       hipLaunchKernelGGL((k_mc_histogram<float, false>), grid, dim3(kBlock), 0, st,
-                         (const float*)x, paths, sh.n, use, edges, nbins, counts);
+                         (const float*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0, counts);
   } else {
     if (lds)
       hipLaunchKernelGGL((k_mc_histogram<double, true>), grid, dim3(kBlock), 0, st,
-                         (const double*)x, paths, sh.n, use, edges, nbins, counts);
+                         (const double*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0,
+                         counts);
     else
       hipLaunchKernelGGL((k_mc_histogram<double, false>), grid, dim3(kBlock), 0, st,
-                         (const double*)x, paths, sh.n, use, edges, nbins, counts);
+                         (const double*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0,
+                         counts);
   }
   return launch_status("k_mc_histogram");
 }

@@ -8,6 +8,7 @@
 // each row in chunk order.  No atomics: the result does not depend on
 // scheduling.
 #include "common.hpp"
+#include "span.hpp"
 
 namespace sdegpu {
 
@@ -54,11 +55,12 @@ __global__ __launch_bounds__(kBlock) void k_moments_partial(const T* __restrict_
   const int64_t lo = chunk * kChunk;
   const int64_t hi = lo + kChunk < paths ? lo + kChunk : paths;
   double s1 = 0.0, s2 = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const double d = (double)xr[i] - shift;
+  auto acc = [&](T v) {
+    const double d = (double)v - shift;
     s1 += d;
     s2 += d * d;
-  }
+  };
+  for_span(xr, lo, hi, acc);
   block_sum2(s1, s2);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = s1;

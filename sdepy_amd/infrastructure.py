@@ -1124,7 +1124,7 @@ def path_cdf(y, x, device_tensor=None):
         c = _lib.empty((r1 - r0, nb + 1), np.int64)
         _lib.check(lib.sdegpu_mc_histogram(
             _lib.ptr(dy[r0:r1]), code, r1 - r0, paths, 0, _lib.ptr(de[:r1 - r0]),
-            nb, _lib.ptr(c), _lib.stream()))
+            nb, 0, _lib.ptr(c), _lib.stream()))
         counts[r0:r1] = _lib.to_host(c)
     cum = np.cumsum(counts[:, :nb], axis=1)       # cum[:, i]: values <= s_i
     idx = np.searchsorted(s, x[ok])

@@ -221,6 +221,9 @@ class montecarlo:
             return
         N = self.paths
         if isfirst:
+            # int and named bins give equally spaced edges (np.histogram's
+            # uniform_bins); explicit edges are searched
+            self._uniform = isinstance(self._bins, (int, np.integer, str))
             self._bins = self._edges(dx, code, rows, n_all, vshape, sdt, ws,
                                      nbytes)
             self._counts = np.empty(vshape, dtype=object)
@@ -243,8 +246,8 @@ class montecarlo:
             for r0, r1 in _row_batches(len(sel)):
                 _lib.check(lib.sdegpu_mc_histogram(
                     _lib.ptr(sub[r0:r1]), code, r1 - r0, n_all, use,
-                    _lib.ptr(de[r0:r1]), nb, _lib.ptr(counts[r0:r1]),
-                    _lib.stream()))
+                    _lib.ptr(de[r0:r1]), nb, int(self._uniform),
+                    _lib.ptr(counts[r0:r1]), _lib.stream()))
             c = _lib.to_host(counts)
             for j, k in enumerate(sel):
                 i = idx[k]
@@ -268,6 +271,7 @@ class montecarlo:
         a = montecarlo()
         a._paths = self._paths
         a._use = self._use
+        a._uniform = getattr(self, '_uniform', False)
         if (self._bins is None) or isinstance(self._bins, (int, str)):
             a._bins = self._bins
             a._range = self._range

@@ -356,3 +356,18 @@ def test_gbm_statistics_within_3se(gpu):
     mu4 = np.mean((xT - xT.mean()) ** 4)
     se_v = np.sqrt((mu4 - v * v) / P)
     assert abs(xT.var() - v) < 3 * se_v
+
+
+def test_c_abi_caller(gpu):
+    """A plain-C program (tests/c/abi_smoke.c, built with gcc by the library
+    build) drives sdegpu_euler / sdegpu_path_moments through include/sdegpu.h:
+    exact deterministic lognormal paths, a mean within 4 SE of
+    lognorm_mean, and a status code plus message for an invalid problem."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), 'c', 'abi_smoke')
+    assert os.path.exists(exe), 'build the library first (it builds the C caller)'
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.strip().endswith('ok')

@@ -160,3 +160,31 @@ def test_process_chf_cdf_tdiff_tint(gpu):
     xr = r.result()
     assert np.allclose(r.chf(np.array([.3, -1.])), xr.chf(np.array([.3, -1.])), atol=1e-13)
     assert np.array_equal(r.cdf(np.array([.9, 1., 1.2])), xr.cdf(np.array([.9, 1., 1.2])))
+
+
+def test_histogram_uniform_estimate_equals_search(gpu):
+    """sdegpu_mc_histogram's arithmetic bin estimate for equally spaced edges
+    (uniform=1) gives exactly the binary-search result (uniform=0), with many
+    values lying on the edges, in FP32 and FP64, and both equal
+    np.histogram."""
+    from sdepy_amd import _lib
+    lib = _lib.lib()
+    g = np.random.default_rng(13)
+    for dt, code in ((np.float32, _lib.F32), (np.float64, _lib.F64)):
+        x = (g.integers(-50, 51, size=(3, 50001)) * dt(0.02)).astype(dt)
+        x[0, :100] = np.nan
+        edges = np.stack([np.histogram_bin_edges(x[i][~np.isnan(x[i])], bins=37)
+                          for i in range(3)]).astype(np.float64)
+        dx, de = _lib.to_device(x), _lib.to_device(edges)
+        got = []
+        for uniform in (1, 0):
+            c = _lib.empty((3, 38), np.int64)
+            _lib.check(lib.sdegpu_mc_histogram(_lib.ptr(dx), code, 3, x.shape[1], 0,
+                                               _lib.ptr(de), 37, uniform, _lib.ptr(c),
+                                               _lib.stream()))
+            got.append(_lib.to_host(c))
+        assert np.array_equal(got[0], got[1])
+        for i in range(3):
+            ref, _ = np.histogram(x[i][~np.isnan(x[i])], bins=37)
+            assert np.array_equal(got[0][i, :37], ref)
+            assert got[0][i, 37] == (100 if i == 0 else 0)

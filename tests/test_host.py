@@ -281,6 +281,16 @@ def test_antithetic_sources_host_side():
     assert z.shape == (10,) and np.array_equal(z[:5], -z[5:])
 
 
+def test_c_caller_builds_against_the_header():
+    """tests/c/abi_smoke.c compiles as C11 with gcc against include/sdegpu.h
+    and links libsdegpu.so (run on the GPU by test_gpu_euler)."""
+    from sdepy_amd import _build
+    exe = _build.build_c_smoke()
+    assert exe and os.path.exists(exe)
+    out = subprocess.run(['ldd', exe], capture_output=True, text=True).stdout
+    assert 'libsdegpu.so' in out
+
+
 def test_bench_refuses_more_than_one_gpu():
     res = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2'],
                          capture_output=True, text=True)
