@@ -311,6 +311,20 @@ def main():
                           'greeks': fdg, 'speedup_of_adjoint': value / fd_value}
         out['accuracy_se'] = acc
         out['accuracy_max_se'] = max(acc.values())
+        # the FP64 adjoint on the same paths: the k/theta/xi/rho/v0 pathwise
+        # estimators have heavy tails here (the variance reaches 0: Feller
+        # fails), so single paths with tiny variance dominate their sums
+        r64 = G.heston_greeks(paths=PATHS, steps=NSTEPS, strike=STRIKE,
+                              maturity=MATURITY, dtype=np.float64,
+                              key=(0x5eed, 0xc0ffee), epoch=0, **HPRM)
+        out['fp64_adjoint'] = {
+            'price': r64.price, 'greeks': r64.greeks,
+            'fp32_minus_fp64_over_se64': {
+                k: (res.greeks[k] - r64.greeks[k]) / r64.greeks_se[k]
+                for k in G.HESTON_GREEKS},
+            'fp64_adjoint_minus_fd_over_se64': {
+                k: (r64.greeks[k] - fdg[k]) / r64.greeks_se[k]
+                for k in G.HESTON_GREEKS}}
         del fd
         torch.cuda.empty_cache()
 

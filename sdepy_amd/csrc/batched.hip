@@ -59,6 +59,21 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
       if (s >= n) break;
       const float* c = tab + (int64_t)s * nc;
       const float ds = c[0], sq = c[1];
+      const float* p = c + poff;
+      if (MODEL == SDEGPU_MODEL_OU && M > 1 && d.has_corr) {
+        // rows: a = 1 - k ds, b = k theta ds; the Cholesky rows carry
+        // sigma sqrt|ds| (folded by the host): x_i = a_i x_i + b_i + sum Lf_ij z_j
+        const float* L = c + 2;
+        int idx = 0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+          float acc = fmaf(p[i], x[i], p[M + i]);
+#pragma unroll
+          for (int j = 0; j <= i; ++j) acc = fmaf(L[idx++], z[k * M + j], acc);
+          x[i] = acc;
+        }
+        continue;
+      }
       float dw[M];
       if (M > 1 && d.has_corr) {
         const float* L = c + 2;
@@ -68,22 +83,16 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
           float acc = 0.f;
 #pragma unroll
           for (int j = 0; j <= i; ++j) acc = acc + L[idx++] * z[k * M + j];
-          dw[i] = MODEL == SDEGPU_MODEL_OU ? acc : acc * sq;
+          dw[i] = acc * sq;
         }
       } else {
 #pragma unroll
         for (int v = 0; v < M; ++v) dw[v] = z[k * M + v] * sq;
       }
-      const float* p = c + poff;
       if constexpr (MODEL == SDEGPU_MODEL_OU) {
-        // rows: a = 1 - k ds, b = k theta ds, sigma; correlated draws carry
-        // sigma sqrt|ds| (folded into the Cholesky rows by the host)
-        const bool folded = M > 1 && d.has_corr;
+        // rows: a = 1 - k ds, b = k theta ds, sigma
 #pragma unroll
-        for (int v = 0; v < M; ++v) {
-          const float inc = folded ? dw[v] : p[2 * M + v] * dw[v];
-          x[v] = p[v] * x[v] + (p[M + v] + inc);
-        }
+        for (int v = 0; v < M; ++v) x[v] = p[v] * x[v] + (p[M + v] + p[2 * M + v] * dw[v]);
       } else {
 #pragma unroll
         for (int v = 0; v < M; ++v) x[v] = x[v] + (p[v] * ds + p[M + v] * dw[v]);

@@ -179,9 +179,10 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
       const T* c = coef + (int64_t)s * a.ncoef;
       const T ds = c[0];
 
-      // OU: the host folds sigma_i * sqrt|ds| into the Cholesky rows, so the
-      // correlated draw is already the diffusion increment sigma * dw.
-      constexpr bool kFold = MODEL == SDEGPU_MODEL_OU;
+      // OU with correlations: the host folds sigma_i * sqrt|ds| into the
+      // Cholesky rows, and the update below applies them to z directly.
+      constexpr bool kOU = MODEL == SDEGPU_MODEL_OU;
+      const bool ou_folded = kOU && a.has_corr && !replay;
       T dw[M];
       if (replay) {
         const T* rp = (const T*)a.replay_dw + (int64_t)s * M * P + gid;
@@ -189,7 +190,9 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
         for (int v = 0; v < M; ++v) dw[v] = rp[v * P];
       } else {
         const T sq = c[1];
-        if (a.has_corr) {
+        if (kOU && a.has_corr) {
+          // folded into the OU update
+        } else if (a.has_corr) {
           const T* L = c + 2;
           int idx = 0;
 #pragma unroll
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
             T acc = (T)0;
 #pragma unroll
             for (int j = 0; j <= i; ++j) acc = acc + L[idx++] * z[k * M + j];
-            dw[i] = kFold ? acc : acc * sq;
+            dw[i] = acc * sq;
           }
         } else {
 #pragma unroll
@@ -214,11 +217,21 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
         // ornstein_uhlenbeck_SDE: {'dt': k*(theta - x), 'dw': sigma}, i.e.
         // x + k (theta - x) ds + sigma dw = (1 - k ds) x + (k theta ds + sigma dw)
         // with a = 1 - k ds, b = k theta ds precomputed per step on the host.
-        const bool folded = a.has_corr && !replay;
+        // Correlated: sigma dw = (sigma sqrt|ds| L) z with the folded rows, so
+        // x_i = a_i x_i + b_i + sum_j Lf_ij z_j, one FMA chain per component.
+        if (ou_folded) {
+          const T* L = c + 2;
+          int idx = 0;
 #pragma unroll
-        for (int v = 0; v < M; ++v) {
-          const T inc = folded ? dw[v] : p[2 * M + v] * dw[v];
-          x[v] = p[v] * x[v] + (p[M + v] + inc);
+          for (int i = 0; i < M; ++i) {
+            T acc = fma(p[i], x[i], p[M + i]);
+#pragma unroll
+            for (int j = 0; j <= i; ++j) acc = fma(L[idx++], z[k * M + j], acc);
+            x[i] = acc;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < M; ++v) x[v] = p[v] * x[v] + (p[M + v] + p[2 * M + v] * dw[v]);
         }
       } else if constexpr (MODEL == SDEGPU_MODEL_CIR) {
         // cox_ingersoll_ross_SDE: x+ = max(x, 0); {'dt': k*(theta - x+), 'dw': xi*sqrt(x+)}

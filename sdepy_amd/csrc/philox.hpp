@@ -18,7 +18,8 @@
 //
 // Uniforms and normals:
 //   f64: u = (x + 0.5) * 2^-32, exact in double, in (0, 1)
-//   f32: u = fma(float(x), 2^-32, 2^-33), in (0, 1]
+//   f32: u = fma(float(x), 2^-32, 2^-33), in (0, 1]; the Box-Muller angle in
+//        f32 is (x >> 9) 2^-23 revolutions
 //   Box-Muller on the lane pairs (0,1) and (2,3):
 //     z0 = sqrt(-2 ln u_a) cos(2 pi u_b),  z1 = sqrt(-2 ln u_a) sin(2 pi u_b)
 // The normal with index g of a path's stream lives in block g / 4, lane g % 4.
@@ -118,9 +119,12 @@ template <>
 struct Gauss<float> {
   // v_log_f32 is log2 and v_sin/v_cos_f32 take their argument in revolutions,
   // so the pair costs one log, one sqrt, one sin and one cos instruction.
+  // The angle needs no conversion: one v_alignbit_b32 puts the top 23 bits
+  // of b under the exponent of 1.0, giving 1 + k 2^-23 in [1, 2), and
+  // cos/sin(2 pi (1 + f)) = cos/sin(2 pi f).
   static __device__ __forceinline__ void pair(uint32_t a, uint32_t b, float& z0, float& z1) {
     const float ua = u01_f32(a);
-    const float ub = u01_f32(b);
+    const float ub = __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, b, 9u));
     const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(ua));
     z0 = r * __builtin_amdgcn_cosf(ub);
     z1 = r * __builtin_amdgcn_sinf(ub);
