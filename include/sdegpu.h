@@ -56,7 +56,8 @@ extern "C" {
 /* ---- how the working state is stored (SDE.let, integration.py:1495-1528) ---- */
 #define SDEGPU_OUT_IDENTITY 0   /* out[v] = x[v] (SDE.let default) */
 #define SDEGPU_OUT_SUM 1        /* out[o] = sum_f x[o*group+f]  (hull_white_SDE.let :2271-2272) */
-#define SDEGPU_OUT_FIRST 2      /* out[o] = x[o], o < nout      (heston_SDE.let :2533-2537) */
+#define SDEGPU_OUT_FIRST 2      /* out[o] = x[o], o < nout; HESTON: the log-x components
+                                   (heston_SDE.let :2533-2537) */
 
 #define SDEGPU_MAX_VARS 8
 
@@ -81,7 +82,8 @@ extern "C" {
  *              Cholesky factor is pre-multiplied by sigma_i sqrt(|ds|), so the
  *              update is x = (1 - k ds) x + (k theta ds + sum_j L'_ij z_j)
  *     CIR      theta[M], k[M], xi[M]
- *     HESTON   mu[N], sigma[N], theta[N], k[N], xi[N]   (state = [log x[N], y[N]])
+ *     HESTON   mu[N], sigma[N], theta[N], k[N], xi[N]   (N log-x values in vshape order;
+ *              state layout: see heston_width)
  *     JUMPDIFF mu - sigma^2/2 [M], sigma[M], a[M], b[M]  (jumps a + b*N(0,1))
  * SDE parameters are evaluated at the step start, correlation and intensity at
  * the step midpoint, as sdepy does (integration.py:1218-1221,
@@ -96,7 +98,8 @@ typedef struct sdegpu_euler_problem {
   int32_t out_group;  /* factor count for SDEGPU_OUT_SUM */
   uint32_t exp_mask;  /* bit o set: store exp(value o); must be the model's log components:
                          all nout for LOGNORM/JUMPDIFF (SDE.log, integration.py:1193-1197),
-                         the first M/2 for HESTON (log x, :2441-2444), none otherwise */
+                         the log-x components for HESTON (:2441-2444; all nout bits when
+                         nout = M/2), none otherwise */
   int32_t has_corr;   /* table rows carry a Cholesky factor */
   int64_t paths;      /* 1 .. 2^32-1 */
   int32_t nsteps;     /* Euler intervals in this pass (>= 0) */
@@ -124,6 +127,11 @@ typedef struct sdegpu_euler_problem {
                                  paths 0..K-1 with the sign reversed; 0: off */
   int64_t anti_j;             /* K > 0: antithetic dj (even_cpoisson_source, :2139-2156):
                                  paths K..2K-1 repeat the jumps of paths 0..K-1; 0: off */
+  int32_t heston_width;       /* HESTON: D = vshape[-1]; the working state is groups of 2D
+                                 components, D log x then the D y of the same leading index
+                                 (SDEs.pack, integration.py:1759-1783); 0 means nvars/2 (one
+                                 group).  Heston stores nout = nvars (the state, log x exp'd)
+                                 or nout = nvars/2 (x only, in vshape order). */
 } sdegpu_euler_problem;
 
 /* Library and device queries. */

@@ -48,6 +48,7 @@ class EulerProblem(ctypes.Structure):
         ('key_w', c_uint32 * 2), ('epoch_w', c_uint32),
         ('key_j', c_uint32 * 2), ('epoch_j', c_uint32),
         ('anti_w', c_int64), ('anti_j', c_int64),
+        ('heston_width', c_int32),
     ]
 
 

@@ -61,25 +61,48 @@ __device__ __forceinline__ double fast_sqrt<double>(double v) {
   return sqrt(v);
 }
 
+// Heston working state (SDEs.pack, integration.py:1759-1783, with the two
+// equations stacked on the last vshape axis of length D): groups of 2D
+// components, D log x followed by the D y of the same leading index.  The
+// log-x component i (vshape order) is at xi(i) and its variance at xi(i) + D.
+// For vshape () or (N,) there is one group: log x first, y after.
+template <int D>
+struct HestonLayout {
+  static constexpr int xi(int i) { return (i / D) * 2 * D + i % D; }
+  static constexpr int yi(int i) { return xi(i) + D; }
+  static constexpr bool is_x(int c) { return (c % (2 * D)) < D; }
+};
+
 // Which stored values are exponentiated is fixed by the model, as sdepy's
 // `log` attribute is fixed by the preset: lognorm/jumpdiff integrate log x
 // (SDE.exit exps everything, integration.py:1193-1197), full_heston/heston
-// keep log x in the first N components (full_heston_SDE.result :2441-2444,
+// keep log x in the x components (full_heston_SDE.result :2441-2444,
 // heston_SDE.result :2539-2541); the rest store the state as is.  A
 // compile-time rule keeps the exp out of the store path of the other models.
-template <int MODEL, int M>
+template <int MODEL>
 struct OutRule {
   static constexpr bool exp_all = MODEL == SDEGPU_MODEL_LOGNORM || MODEL == SDEGPU_MODEL_JUMPDIFF;
-  static constexpr int exp_first = MODEL == SDEGPU_MODEL_HESTON ? M / 2 : (exp_all ? M : 0);
-  static constexpr uint32_t mask = exp_first >= 32 ? 0xFFFFFFFFu : ((1u << exp_first) - 1u);
 };
 
-template <int MODEL, typename T, int M>
+// Stores one output row.  Heston stores either the whole state (nout = M,
+// full_heston) or the N log-x components in vshape order (nout = M/2, heston).
+template <int MODEL, typename T, int M, int D>
 __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs& a, int row,
                                             const T (&x)[M], int64_t gid) {
-  constexpr int EF = OutRule<MODEL, M>::exp_first;
   const int64_t P = a.paths;
   T* o_row = out + (int64_t)row * a.nout * P + gid;
+  if constexpr (MODEL == SDEGPU_MODEL_HESTON) {
+    using L = HestonLayout<D>;
+    if (a.nout == M) {
+#pragma unroll
+      for (int o = 0; o < M; ++o)
+        __builtin_nontemporal_store(L::is_x(o) ? exp(x[o]) : x[o], o_row + o * P);
+    } else {
+#pragma unroll
+      for (int i = 0; i < M / 2; ++i) __builtin_nontemporal_store(exp(x[L::xi(i)]), o_row + i * P);
+    }
+    return;
+  }
   if (MODEL == SDEGPU_MODEL_OU && a.out_group != 1) {
     // hull_white_SDE.let: out_x[...] = x.sum(axis=-2) over each factor group.
 #pragma unroll
@@ -99,15 +122,13 @@ __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs
   for (int o = 0; o < M; ++o) {
     if (o < a.nout) {
       T val = x[o];
-      if constexpr (EF > 0) {
-        if (o < EF) val = exp(val);
-      }
+      if constexpr (OutRule<MODEL>::exp_all) val = exp(val);
       __builtin_nontemporal_store(val, o_row + o * P);
     }
   }
 }
 
-template <int MODEL, typename T, int M>
+template <int MODEL, typename T, int M, int D>
 __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __restrict__ coef,
                                                   const int32_t* __restrict__ store_row,
                                                   T* __restrict__ out) {
@@ -136,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 #pragma unroll
     for (int v = 0; v < M; ++v) x[v] = (T)a.x0[v];
   }
-  if (a.row0 >= 0) store_state<MODEL, T, M>(out, a, a.row0, x, gid);
+  if (a.row0 >= 0) store_state<MODEL, T, M, D>(out, a, a.row0, x, gid);
 
   int cnt[M];
 #pragma unroll
@@ -246,9 +267,11 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
         // full_heston_SDE (full truncation): y+ = max(y, 0);
         //   log x: {'dt': mu - sigma*sigma*y+/2, 'dw': sigma*sqrt(y+)}
         //   y:     {'dt': k*(theta - y+),        'dw': xi*sqrt(y+)}
+        using L = HestonLayout<D>;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          const T y = x[N + i];
+          const int ix = L::xi(i), iy = L::yi(i);
+          const T y = x[iy];
           cnt[i] += (y < (T)0) ? 1 : 0;
           const T yp = y > (T)0 ? y : (T)0;
           const T sy = fast_sqrt<T>(yp);
@@ -258,8 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
           const T axw = sig * sy;
           const T ayt = kk * (th - yp);
           const T ayw = xi * sy;
-          x[i] = x[i] + (axt * ds + axw * dw[i]);
-          x[N + i] = y + (ayt * ds + ayw * dw[N + i]);
+          x[ix] = x[ix] + (axt * ds + axw * dw[ix]);
+          x[iy] = y + (ayt * ds + ayw * dw[iy]);
         }
       } else if constexpr (kJump) {
         // jumpdiff_SDE: {'dt': mu - sigma^2/2, 'dw': sigma, 'dj': 1}, no martingale correction;
@@ -303,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 
       const int row = store_row[s];
       if (row >= 0) {
-        store_state<MODEL, T, M>(out, a, row, x, gid);
+        store_state<MODEL, T, M, D>(out, a, row, x, gid);
         if constexpr (kJump) {
           if (a.info_rows && interval_jumps != 0)
             atomicAdd((unsigned long long*)(a.info_rows + row), (unsigned long long)(long long)interval_jumps);
@@ -325,37 +348,61 @@ using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, T*);
 template <int MODEL, typename T>
 static EulerKernel<T> pick_m(int m) {
   switch (m) {
-    case 1: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 1>;
-    case 2: return k_euler<MODEL, T, 2>;
-    case 3: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 3>;
-    case 4: return k_euler<MODEL, T, 4>;
-    case 5: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 5>;
-    case 6: return k_euler<MODEL, T, 6>;
-    case 7: return MODEL == SDEGPU_MODEL_HESTON ? nullptr : k_euler<MODEL, T, 7>;
-    case 8: return k_euler<MODEL, T, 8>;
+    case 1: return k_euler<MODEL, T, 1, 1>;
+    case 2: return k_euler<MODEL, T, 2, 1>;
+    case 3: return k_euler<MODEL, T, 3, 1>;
+    case 4: return k_euler<MODEL, T, 4, 1>;
+    case 5: return k_euler<MODEL, T, 5, 1>;
+    case 6: return k_euler<MODEL, T, 6, 1>;
+    case 7: return k_euler<MODEL, T, 7, 1>;
+    case 8: return k_euler<MODEL, T, 8, 1>;
+    default: return nullptr;
+  }
+}
+
+// Heston kernels by (M, D): D log-x components per group of 2D.
+template <typename T>
+static EulerKernel<T> pick_heston(int m, int d) {
+  constexpr int H = SDEGPU_MODEL_HESTON;
+  switch (m * 16 + d) {
+    case 2 * 16 + 1: return k_euler<H, T, 2, 1>;
+    case 4 * 16 + 1: return k_euler<H, T, 4, 1>;
+    case 4 * 16 + 2: return k_euler<H, T, 4, 2>;
+    case 6 * 16 + 1: return k_euler<H, T, 6, 1>;
+    case 6 * 16 + 3: return k_euler<H, T, 6, 3>;
+    case 8 * 16 + 1: return k_euler<H, T, 8, 1>;
+    case 8 * 16 + 2: return k_euler<H, T, 8, 2>;
+    case 8 * 16 + 4: return k_euler<H, T, 8, 4>;
     default: return nullptr;
   }
 }
 
 template <typename T>
-static EulerKernel<T> pick(int model, int m) {
+static EulerKernel<T> pick(int model, int m, int d) {
   switch (model) {
     case SDEGPU_MODEL_WIENER: return pick_m<SDEGPU_MODEL_WIENER, T>(m);
     case SDEGPU_MODEL_LOGNORM: return pick_m<SDEGPU_MODEL_LOGNORM, T>(m);
     case SDEGPU_MODEL_OU: return pick_m<SDEGPU_MODEL_OU, T>(m);
     case SDEGPU_MODEL_CIR: return pick_m<SDEGPU_MODEL_CIR, T>(m);
-    case SDEGPU_MODEL_HESTON: return pick_m<SDEGPU_MODEL_HESTON, T>(m);
+    case SDEGPU_MODEL_HESTON: return pick_heston<T>(m, d);
     case SDEGPU_MODEL_JUMPDIFF: return pick_m<SDEGPU_MODEL_JUMPDIFF, T>(m);
     default: return nullptr;
   }
 }
 
-// exp_mask the model implies for nout stored values (see OutRule).
-static uint32_t expected_exp_mask(int model, int m, int nout) {
-  int ef = 0;
-  if (model == SDEGPU_MODEL_LOGNORM || model == SDEGPU_MODEL_JUMPDIFF) ef = nout;
-  if (model == SDEGPU_MODEL_HESTON) ef = m / 2 < nout ? m / 2 : nout;
-  return ef >= 32 ? 0xFFFFFFFFu : ((1u << ef) - 1u);
+// exp_mask the model implies for nout stored values (see OutRule and
+// store_state): every value for lognorm/jumpdiff, the log-x values for
+// Heston (bit o of component o when the whole state is stored, all N bits
+// when only log x is), none otherwise.
+static uint32_t expected_exp_mask(int model, int m, int nout, int d) {
+  if (model == SDEGPU_MODEL_LOGNORM || model == SDEGPU_MODEL_JUMPDIFF)
+    return nout >= 32 ? 0xFFFFFFFFu : ((1u << nout) - 1u);
+  if (model != SDEGPU_MODEL_HESTON) return 0u;
+  if (nout == m / 2) return (1u << nout) - 1u;
+  uint32_t mask = 0u;
+  for (int o = 0; o < m; ++o)
+    if (o % (2 * d) < d) mask |= 1u << o;
+  return mask;
 }
 
 static int param_blocks(int model, int m) {
@@ -387,8 +434,16 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EINVAL, "paths must be in [1, %lld], not %lld", (long long)kMaxPaths,
                 (long long)pr->paths);
   if (pr->nsteps < 0) return fail(SDEGPU_EINVAL, "negative step count %d", pr->nsteps);
-  if (pr->model == SDEGPU_MODEL_HESTON && (m % 2) != 0)
+  const bool heston = pr->model == SDEGPU_MODEL_HESTON;
+  if (heston && (m % 2) != 0)
     return fail(SDEGPU_EINVAL, "the Heston working state has an even size, not %d", m);
+  const int hd = heston ? (pr->heston_width == 0 ? m / 2 : pr->heston_width) : 1;
+  if (heston && (hd < 1 || (m / 2) % hd != 0))
+    return fail(SDEGPU_EINVAL, "Heston group width %d does not divide %d log-x components",
+                pr->heston_width, m / 2);
+  if (heston && pr->nout != m && pr->nout != m / 2)
+    return fail(SDEGPU_EINVAL, "Heston stores %d (whole state) or %d (log x) values, not %d", m,
+                m / 2, pr->nout);
   const int nparam = param_blocks(pr->model, m);
   if (nparam < 0) return fail(SDEGPU_EUNSUPPORTED, "unknown model code %d", pr->model);
   const int ncorr = pr->has_corr ? m * (m + 1) / 2 : 0;
@@ -417,7 +472,7 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EINVAL, "antithetic dj half %lld does not fit %lld paths",
                 (long long)pr->anti_j, (long long)pr->paths);
 
-  const uint32_t want_mask = expected_exp_mask(pr->model, m, pr->nout);
+  const uint32_t want_mask = expected_exp_mask(pr->model, m, pr->nout, hd);
   if (pr->exp_mask != want_mask)
     return fail(SDEGPU_EINVAL, "exp_mask 0x%x does not match model %d (0x%x expected)",
                 pr->exp_mask, pr->model, want_mask);
@@ -458,12 +513,12 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {
-    EulerKernel<float> k = pick<float>(pr->model, m);
+    EulerKernel<float> k = pick<float>(pr->model, m, hd);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
                        (float*)pr->out);
   } else {
-    EulerKernel<double> k = pick<double>(pr->model, m);
+    EulerKernel<double> k = pick<double>(pr->model, m, hd);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
                        (double*)pr->out);

@@ -328,6 +328,8 @@ class _gpu_sde:
         prob.model, prob.dtype, prob.nvars = self.model, code, M
         prob.nout, prob.out_mode, prob.out_group = nout, out_mode, out_group
         prob.exp_mask = exp_mask
+        if self.model == _lib.MODEL_HESTON:
+            prob.heston_width = self._width()
         prob.paths, prob.nsteps = P, nsteps
         prob.row0 = int(row_of(0)) if first else -1
         prob.stream_tag = _lib.STREAM_BACKWARD if reverse else 0
@@ -737,9 +739,15 @@ class full_heston_process(_gpu_sde):
         return np.concatenate([self._eval(n, t0s, v) for n in
                                ('mu', 'sigma', 'theta', 'k', 'xi')], axis=1)
 
+    def _width(self):
+        """Log-x components per group of the working state: vshape[-1]
+        (1 for vshape ()), the packing of SDEs.pack (integration.py:1759-1783)."""
+        return 1 if self.addaxis else self.vshape[-1]
+
     def _output_map(self):
-        M = self._nvars()
-        return M, _lib.OUT_IDENTITY, 1, (1 << (M // 2)) - 1
+        M, d = self._nvars(), self._width()
+        mask = sum(1 << o for o in range(M) if o % (2 * d) < d)
+        return M, _lib.OUT_IDENTITY, 1, mask
 
     def _unpack(self, xx):
         if self.addaxis:

@@ -211,6 +211,50 @@ def test_full_heston_matches_twin(gpu, dtype):
     assert np.array_equal(x2, x)
 
 
+@pytest.mark.parametrize('vshape', [(2, 2), (3, 1), (1, 4), (2, 1, 2)])
+def test_full_heston_multi_axis_layout(gpu, vshape):
+    """vshape with leading axes: the working state packs, per leading index,
+    the log x of the last axis followed by its y (SDEs.pack,
+    integration.py:1759-1783), and dw's last axis of length 2d splits into
+    dw_x = dw[..., :d] and dw_y = dw[..., d:] (full_heston_process docstring,
+    :2457-2464).  Distinct x0, y0 and parameters per component, a full
+    correlation over dw's last axis, FP64 against the oracle."""
+    P, n = 1500, 40
+    g = np.random.default_rng(31)
+    d = vshape[-1]
+    shp = vshape + (1,)
+    args = dict(x0=g.uniform(50., 150., shp), mu=g.uniform(-.05, .05, shp),
+                sigma=g.uniform(.5, 1.5, shp), y0=g.uniform(.01, .09, shp),
+                theta=g.uniform(.02, .08, shp), k=g.uniform(1., 3., shp),
+                xi=g.uniform(.3, .8, shp))
+    A = g.standard_normal((2 * d, 2 * d))
+    C = A @ A.T + 2 * d * np.eye(2 * d)
+    C = C / np.sqrt(np.outer(np.diag(C), np.diag(C)))
+    info = {}
+    p = sa.full_heston_process(paths=P, vshape=vshape, dtype=np.float64,
+                               steps=n, corr=C, info=info,
+                               rng=np.random.default_rng(32), **args)
+    key = p.sources['dw'].key
+    x, y = p((0., .5, 1.))
+    ref_info = {}
+    wshape = vshape[:-1] + (2 * d,)
+    rx, ry = ora.integrate('full_heston', (0., .5, 1.), paths=P, steps=n,
+                           vshape=vshape, info=ref_info,
+                           sources={'dw': ora.PhiloxWiener(key, 0, P, wshape, corr=C)},
+                           **args)
+    assert x.shape == y.shape == (3,) + vshape + (P,)
+    assert _rel(x, rx) < 1e-10
+    assert np.max(np.abs(y - ry)) < 1e-12
+    assert info['negative_y_count'].shape == vshape + (P,)
+    assert np.array_equal(info['negative_y_count'], ref_info['negative_y_count'])
+    # heston_process stores x only, in vshape order
+    p2 = sa.heston_process(paths=P, vshape=vshape, dtype=np.float64, steps=n,
+                           corr=C, rng=np.random.default_rng(32), **args)
+    x2 = p2((0., .5, 1.))
+    assert x2.shape == (3,) + vshape + (P,)
+    assert np.array_equal(x2, x)
+
+
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_merton_matches_twin(gpu, dtype):
     P, n = 400, 40

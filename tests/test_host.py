@@ -105,6 +105,16 @@ def test_entry_point_validation_maps_to_exceptions():
     pr.model, pr.nvars = _lib.MODEL_HESTON, 3
     with pytest.raises(ValueError):        # odd Heston state
         _lib.check(lib.sdegpu_euler(pr, None))
+    pr.nvars, pr.nout, pr.ncoef, pr.heston_width = 8, 8, 22, 3
+    with pytest.raises(ValueError):        # width 3 does not divide 4 log-x values
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.heston_width, pr.nout = 2, 5
+    with pytest.raises(ValueError):        # Heston stores 8 or 4 values
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.nout, pr.exp_mask = 8, 0x0F
+    with pytest.raises(ValueError, match='0x33'):   # log x of groups [x x y y]
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.heston_width = 0
     pr.model, pr.nvars, pr.paths = _lib.MODEL_OU, 1, 0
     with pytest.raises(ValueError):
         _lib.check(lib.sdegpu_euler(pr, None))
