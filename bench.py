@@ -193,6 +193,29 @@ def _device_summaries(torch, run):
                                        8, _lib.ptr(cout), _lib.ptr(cws), cn, st))
     ms, _ = timed(torch, chf, 5, 2)
     res['chf_8_args_last_row'] = {'ms': ms, 'values_per_s': 4 * P * 8 / (ms * 1e-3)}
+    # along the timeline (tmax, tvar in FP64: two read passes) and across the
+    # 4 values (vmean), NumPy's reduction order; bytes = reads + writes
+    n_t, nout, _ = run.xx_dev.shape
+    cols = nout * P
+    tout = _lib.empty((1, cols), np.float64)
+
+    def tred(op, acc):
+        return lambda: _lib.check(lib.sdegpu_axis_reduce(
+            _lib.ptr(run.xx_dev), _lib.F32, 1, n_t, cols, 0, cols, None, op, acc, 0,
+            _lib.ptr(tout), st))
+    for name, op, acc, passes, osz in (('tmax', _lib.REDUCE_MAX, _lib.F32, 1, 4),
+                                       ('tvar_f64', _lib.REDUCE_VAR, _lib.F64, 2, 8)):
+        ms, _ = timed(torch, tred(op, acc), 5, 2)
+        res[name] = {'ms': ms, 'gb_per_s': (passes * nbytes + cols * osz) / (ms * 1e-3) / 1e9}
+    vout = _lib.empty((n_t, P), np.float32)
+    comps = (ctypes.c_int32 * nout)(*range(nout))
+
+    def vmean():
+        _lib.check(lib.sdegpu_axis_reduce(_lib.ptr(run.xx_dev), _lib.F32, n_t, nout, P,
+                                          nout * P, P, comps, _lib.REDUCE_MEAN, _lib.F32,
+                                          0, _lib.ptr(vout), st))
+    ms, _ = timed(torch, vmean, 5, 2)
+    res['vmean'] = {'ms': ms, 'gb_per_s': (nbytes + n_t * P * 4) / (ms * 1e-3) / 1e9}
     return res
 
 

@@ -139,6 +139,31 @@ int sdegpu_version(void);
 int sdegpu_last_error(char* buf, size_t len);
 int sdegpu_device_count(int* count);
 
+/*
+ * Reduction of a device array along one axis, element by element in index
+ * order as NumPy reduces an axis that is not the innermost one: the process
+ * summaries along the timeline tmin/tmax/tsum/tmean/tvar/tstd
+ * (infrastructure.py:963-1027) and across values vmin/vmax/vsum/vmean/vvar/
+ * vstd (:894-959).  x is viewed as x[o][k][c] at element offset
+ * o*outer_stride + off(k) + c (o < outer, k < n, c < inner), with
+ * off(k) = k*n_stride, or comps[k]*n_stride when `comps` (host array of n <= 8
+ * component indices) is given; out[o][c] (contiguous, outer*inner values) is
+ * the reduction over k.  op: SDEGPU_REDUCE_*.  Sums, means, variances and
+ * standard deviations accumulate in acc_dtype (>= dtype) as ndarray.sum/mean/
+ * var/std(dtype=acc) do; mean = sum/n, var = sum((x - mean)^2)/max(n - ddof,
+ * 0); min/max are in dtype and propagate NaN.  Results equal NumPy's bit for
+ * bit except when inner == 1, where NumPy sums pairwise.
+ */
+#define SDEGPU_REDUCE_MIN 0
+#define SDEGPU_REDUCE_MAX 1
+#define SDEGPU_REDUCE_SUM 2
+#define SDEGPU_REDUCE_MEAN 3
+#define SDEGPU_REDUCE_VAR 4
+#define SDEGPU_REDUCE_STD 5
+int sdegpu_axis_reduce(const void* x, int32_t dtype, int64_t outer, int64_t n, int64_t inner,
+                       int64_t outer_stride, int64_t n_stride, const int32_t* comps, int32_t op,
+                       int32_t acc_dtype, int32_t ddof, void* out, void* stream);
+
 /* Euler pass, see sdegpu_euler_problem. */
 int sdegpu_euler(const sdegpu_euler_problem* prob, void* stream);
 

@@ -24,6 +24,7 @@ MODEL_WIENER, MODEL_LOGNORM, MODEL_OU, MODEL_CIR, MODEL_HESTON, \
 OUT_IDENTITY, OUT_SUM, OUT_FIRST = range(3)
 MAX_VARS = 8
 STREAM_BACKWARD = 16
+REDUCE_MIN, REDUCE_MAX, REDUCE_SUM, REDUCE_MEAN, REDUCE_VAR, REDUCE_STD = range(6)
 
 _EXC = {EINVAL: ValueError, EINDEX: IndexError, ERUNTIME: RuntimeError,
         ENOMEM: MemoryError, EUNSUPPORTED: NotImplementedError}
@@ -78,6 +79,9 @@ SIGNATURES = [
     ('sdegpu_mc_histogram', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p, c_int32,
       c_int32, c_void_p, c_void_p]),
+    ('sdegpu_axis_reduce', ctypes.c_int,
+     [c_void_p, c_int32, c_int64, c_int64, c_int64, c_int64, c_int64,
+      c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p]),
     ('sdegpu_chf_workspace', c_size_t, [c_int64, c_int64, c_int32]),
     ('sdegpu_path_chf', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int64, c_void_p, c_int32, c_void_p,

@@ -22,6 +22,7 @@ compound Poisson source with another jump law) is honoured by calling it on
 the host once per step, in sorted source-id order (SDE.dZ, :1227-1234), and
 replaying the recorded increments in the kernel.
 """
+import ctypes
 import inspect
 
 import numpy as np
@@ -260,6 +261,10 @@ class _gpu_sde:
 
     def _result(self, tt, xx):
         return process(t=tt, x=xx)
+
+    def _output_parts(self):
+        """Stored components of each process the preset returns."""
+        return [list(range(self._output_map()[0]))]
 
     # per-parameter evaluation -------------------------------------------
 
@@ -555,6 +560,105 @@ class PreparedRun:
         n, _, m2 = self._moments()
         return self._summary(np.sqrt(m2 / (n - ddof)), dtype)
 
+    # process.tmin/tmax/tsum/tmean/tvar/tstd (infrastructure.py:963-1027)
+    # and vmin/vmax/vsum/vmean/vvar/vstd (:894-959) on the device output,
+    # equal to the host methods on the copied result (sdegpu_axis_reduce
+    # reduces in NumPy's order).
+
+    def _axis_reduce(self, op, outer, n, inner, so, sk, comps, dtype, ddof):
+        xx = self.xx_dev
+        code = _lib.F32 if xx.dtype == _lib.torch_mod().float32 else _lib.F64
+        dev = np.float32 if code == _lib.F32 else np.float64
+        if op in (_lib.REDUCE_MIN, _lib.REDUCE_MAX):
+            acc = dev
+        else:
+            acc = np.dtype(self.odt if dtype is None else dtype).type
+            if acc not in (np.float32, np.float64) or \
+                    (acc is np.float32 and dev is np.float64):
+                raise NotImplementedError(
+                    'device summaries accumulate in float32 or float64, at '
+                    'least the precision of the values ({}), not {}'.format(
+                        np.dtype(dev).name, np.dtype(acc).name))
+        acc_code = _lib.F32 if acc is np.float32 else _lib.F64
+        out = _lib.empty((outer, inner), acc)
+        cm = None
+        if comps is not None:
+            cm = (ctypes.c_int32 * len(comps))(*comps)
+        _lib.check(_lib.lib().sdegpu_axis_reduce(
+            _lib.ptr(xx), code, outer, n, inner, so, sk, cm, op, acc_code,
+            int(ddof), _lib.ptr(out), _lib.stream()))
+        return _lib.to_host(out)
+
+    def _t_summary(self, op, dtype=None, ddof=0):
+        n_t, nout, P = self.xx_dev.shape
+        vals = self._axis_reduce(op, 1, n_t, nout * P, 0, nout * P, None,
+                                 dtype, ddof)
+        sde = self.sde
+        xx = vals.reshape((1,) + sde.xshape + (P,))
+        if op in (_lib.REDUCE_MIN, _lib.REDUCE_MAX):
+            xx = xx.astype(self.odt, copy=False)
+        return sde._result(np.asarray(self.tt_asis)[:1], xx)
+
+    def _v_summary(self, op, dtype=None, ddof=0):
+        n_t, nout, P = self.xx_dev.shape
+        parts = self.sde._output_parts()
+        res = []
+        for comps in parts:
+            vals = self._axis_reduce(op, n_t, len(comps), P, nout * P, P,
+                                     comps, dtype, ddof)
+            if op in (_lib.REDUCE_MIN, _lib.REDUCE_MAX):
+                vals = vals.astype(self.odt, copy=False)
+            res.append(process(t=self.tt_asis, x=vals))
+        return res[0] if len(res) == 1 else tuple(res)
+
+    def tmin(self):
+        """Minimum along the timeline, per path and value, on the device."""
+        return self._t_summary(_lib.REDUCE_MIN)
+
+    def tmax(self):
+        """Maximum along the timeline, per path and value, on the device."""
+        return self._t_summary(_lib.REDUCE_MAX)
+
+    def tsum(self, dtype=None):
+        """Sum along the timeline, on the device."""
+        return self._t_summary(_lib.REDUCE_SUM, dtype)
+
+    def tmean(self, dtype=None):
+        """Mean along the timeline, on the device."""
+        return self._t_summary(_lib.REDUCE_MEAN, dtype)
+
+    def tvar(self, dtype=None, ddof=0):
+        """Variance along the timeline, on the device."""
+        return self._t_summary(_lib.REDUCE_VAR, dtype, ddof)
+
+    def tstd(self, dtype=None, ddof=0):
+        """Standard deviation along the timeline, on the device."""
+        return self._t_summary(_lib.REDUCE_STD, dtype, ddof)
+
+    def vmin(self):
+        """Minimum across the values of each time point and path."""
+        return self._v_summary(_lib.REDUCE_MIN)
+
+    def vmax(self):
+        """Maximum across the values of each time point and path."""
+        return self._v_summary(_lib.REDUCE_MAX)
+
+    def vsum(self, dtype=None):
+        """Sum across the values of each time point and path."""
+        return self._v_summary(_lib.REDUCE_SUM, dtype)
+
+    def vmean(self, dtype=None):
+        """Mean across the values of each time point and path."""
+        return self._v_summary(_lib.REDUCE_MEAN, dtype)
+
+    def vvar(self, dtype=None, ddof=0):
+        """Variance across the values of each time point and path."""
+        return self._v_summary(_lib.REDUCE_VAR, dtype, ddof)
+
+    def vstd(self, dtype=None, ddof=0):
+        """Standard deviation across the values of each time point and path."""
+        return self._v_summary(_lib.REDUCE_STD, dtype, ddof)
+
     def chf(self, u):
         """process.chf(u) on the device output: mean across paths of
         exp(1j*u*x), shaped (N_t,) + u.shape + xshape."""
@@ -749,6 +853,11 @@ class full_heston_process(_gpu_sde):
         mask = sum(1 << o for o in range(M) if o % (2 * d) < d)
         return M, _lib.OUT_IDENTITY, 1, mask
 
+    def _output_parts(self):
+        M, d = self._nvars(), self._width()
+        return [[o for o in range(M) if o % (2 * d) < d],
+                [o for o in range(M) if o % (2 * d) >= d]]
+
     def _unpack(self, xx):
         if self.addaxis:
             return xx[..., 0, :], xx[..., 1, :]
@@ -781,6 +890,9 @@ class heston_process(full_heston_process):
 
     def _result(self, tt, xx):
         return process(tt, x=xx)
+
+    def _output_parts(self):
+        return [list(range(self._nvars() // 2))]
 
 
 class jumpdiff_process(_gpu_sde):

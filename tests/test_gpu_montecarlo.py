@@ -188,3 +188,99 @@ def test_histogram_uniform_estimate_equals_search(gpu):
             ref, _ = np.histogram(x[i][~np.isnan(x[i])], bins=37)
             assert np.array_equal(got[0][i, :37], ref)
             assert got[0][i, 37] == (100 if i == 0 else 0)
+
+
+def _axis(x, op, acc, outer, n, inner, so, sk, comps=None, ddof=0):
+    from sdepy_amd import _lib
+    import ctypes
+    code = _lib.F32 if x.dtype == np.float32 else _lib.F64
+    acc_code = _lib.F32 if acc == np.float32 else _lib.F64
+    dx = _lib.to_device(x)
+    out = _lib.empty((outer, inner), x.dtype if op < 2 else acc)
+    cm = None if comps is None else (ctypes.c_int32 * len(comps))(*comps)
+    _lib.check(_lib.lib().sdegpu_axis_reduce(
+        _lib.ptr(dx), code, outer, n, inner, so, sk, cm, op, acc_code, ddof,
+        _lib.ptr(out), _lib.stream()))
+    return _lib.to_host(out)
+
+
+@pytest.mark.parametrize('dtype,acc', [(np.float32, np.float32), (np.float32, np.float64),
+                                       (np.float64, np.float64)])
+@pytest.mark.parametrize('inner', [4 * 777, 1001])
+def test_axis_reduce_matches_numpy(gpu, dtype, acc, inner):
+    """sdegpu_axis_reduce over axis 0 and over a listed middle axis equals
+    ndarray.min/max/sum/mean/var/std bit for bit, with NaNs, signed zeros and
+    ddof; inner divisible by 4 takes the 16-byte vector path, 1001 the
+    scalar one."""
+    g = np.random.default_rng(41)
+    n = 37
+    x = (g.standard_normal((n, inner)) * 3 + 1).astype(dtype)
+    x[5, 3] = np.nan
+    x[:, 7] = 0.
+    x[::2, 7] = -0.
+    x[:, 8] = 2.5
+    ops = [('min', 0), ('max', 1), ('sum', 2), ('mean', 3), ('var', 4), ('std', 5)]
+    for name, op in ops:
+        for ddof in ((0, 1) if op >= 4 else (0,)):
+            got = _axis(x, op, acc, 1, n, inner, 0, inner, ddof=ddof)[0]
+            kw = {} if op < 2 else {'dtype': acc}
+            if op >= 4:
+                kw['ddof'] = ddof
+            with np.errstate(all='ignore'):
+                ref = getattr(x, name)(axis=0, **kw)
+            assert got.dtype == ref.dtype, name
+            assert np.array_equal(got, ref, equal_nan=True), (name, ddof)
+            ok = ~np.isnan(ref)
+            assert np.array_equal(np.signbit(got[ok]), np.signbit(ref[ok])), name
+    # middle axis with a component list: x viewed as (rows, comps, paths)
+    y = (g.standard_normal((5, 6, 211))).astype(dtype)
+    comps = [0, 2, 3, 5]
+    for name, op in ops:
+        got = _axis(y, op, acc, 5, len(comps), 211, 6 * 211, 211, comps=comps)
+        kw = {} if op < 2 else {'dtype': acc}
+        ref = getattr(y[:, comps, :], name)(axis=1, **kw)
+        assert np.array_equal(got, ref), name
+
+
+@pytest.mark.filterwarnings('ignore::RuntimeWarning')
+def test_device_time_and_value_summaries(gpu):
+    """PreparedRun.tmin/tmax/tsum/tmean/tvar/tstd and vmin/.../vstd on the
+    device output equal the host process methods on the copied result, bit
+    for bit, for a scalar, a vector, a factor-summed and a pair-returning
+    preset."""
+    runs = [
+        sa.ornstein_uhlenbeck_process(paths=3000, vshape=3, dtype=np.float32,
+                                      x0=.1, theta=np.array([[.1], [-.3], [.2]]),
+                                      k=1.5, sigma=.3, rho=None,
+                                      rng=np.random.default_rng(42)),
+        sa.lognorm_process(paths=2001, dtype=np.float64, x0=1., mu=.1, sigma=.3,
+                           rng=np.random.default_rng(43)),
+        sa.hull_white_process(paths=2000, factors=2, dtype=np.float32, x0=0.,
+                              theta=0., k=np.array([[.1], [.2]]),
+                              sigma=np.array([[.01], [.02]]), rho=.5,
+                              rng=np.random.default_rng(44)),
+        sa.full_heston_process(paths=1600, vshape=(2, 2), dtype=np.float64,
+                               x0=100., mu=.02, y0=.04, theta=.04, k=2., xi=.5,
+                               rng=np.random.default_rng(45), steps=20),
+    ]
+    for p in runs:
+        run = p.prepare(np.linspace(0., 1., 11))
+        run.launch()
+        res = run.result()
+        host = res if isinstance(res, tuple) else (res,)
+        for name in ('tmin', 'tmax', 'tsum', 'tmean', 'tvar', 'tstd',
+                     'vmin', 'vmax', 'vsum', 'vmean', 'vvar', 'vstd'):
+            kws = [{}]
+            if name[1:] in ('sum', 'mean', 'var', 'std'):
+                kws.append({'dtype': np.float64})
+            if name[1:] in ('var', 'std'):
+                kws.append({'ddof': 1})
+            for kw in kws:
+                got = getattr(run, name)(**kw)
+                got = got if isinstance(got, tuple) else (got,)
+                assert len(got) == len(host)
+                for a, h in zip(got, host):
+                    ref = getattr(h, name)(**kw)
+                    assert a.shape == ref.shape and a.dtype == ref.dtype, (name, kw)
+                    assert np.array_equal(a, ref, equal_nan=True), (type(p).__name__, name, kw)
+                    assert np.array_equal(a.t, ref.t)

@@ -126,6 +126,22 @@ def test_entry_point_validation_maps_to_exceptions():
         _lib.check(lib.sdegpu_euler(pr, None))
     with pytest.raises(ValueError):
         _lib.check(lib.sdegpu_path_moments(None, 0, -1, 10, None, None, None, 0, None))
+    ax = lib.sdegpu_axis_reduce
+    one = ctypes.c_void_p(16)
+    with pytest.raises(NotImplementedError):       # unknown dtype
+        _lib.check(ax(one, 7, 1, 3, 4, 0, 4, None, _lib.REDUCE_SUM, 0, 0, one, None))
+    with pytest.raises(ValueError):                # unknown operation
+        _lib.check(ax(one, 0, 1, 3, 4, 0, 4, None, 9, 0, 0, one, None))
+    with pytest.raises(NotImplementedError):       # float64 values, float32 sums
+        _lib.check(ax(one, 1, 1, 3, 4, 0, 4, None, _lib.REDUCE_MEAN, 0, 0, one, None))
+    with pytest.raises(ValueError):                # empty reduced axis
+        _lib.check(ax(one, 0, 1, 0, 4, 0, 4, None, _lib.REDUCE_MIN, 0, 0, one, None))
+    nine = (ctypes.c_int32 * 9)(*range(9))
+    with pytest.raises(NotImplementedError):       # more than 8 listed components
+        _lib.check(ax(one, 0, 1, 9, 4, 36, 4, nine, _lib.REDUCE_MAX, 0, 0, one, None))
+    with pytest.raises(IndexError):                # negative component
+        _lib.check(ax(one, 0, 1, 1, 4, 4, 4, (ctypes.c_int32 * 1)(-1), _lib.REDUCE_MAX,
+                      0, 0, one, None))
     assert lib.sdegpu_moments_workspace(3, 70000) == 3 * 2 * 2 * 8
     with pytest.raises(MemoryError):
         _lib.check(lib.sdegpu_path_moments(ctypes.c_void_p(8), 0, 3, 70000,
