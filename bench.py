@@ -20,7 +20,9 @@ Printed as one JSON line:
   merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
               mu, sigma, a, b), with the FP64 run of the same paths in SE units
   summaries   moments, min/max, 100-bin histograms of every row of the OU
-              output and chf of its last row, on the device (HBM read passes)
+              output, chf of its last row, tmax and FP64 tvar along the
+              timeline and vmean across the 4 values, on the device (HBM
+              read passes)
   gbm_c1      GBM 10k x 252 FP64 full path + pmean/pvar (latency-bound):
               kernel time and the end-to-end preset call
   batched     1024 heterogeneous GBM / HW2F instances in one persistent
@@ -61,10 +63,10 @@ VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
 # (FETCH reads half the copied bytes: x2.0; WRITE is exact: x1.0), and the
 # kernel's VALU wave-instructions per launch (SQ_INSTS_VALU); the summaries
 # they come from are in profiles/
-ADJ_TRAFFIC_BYTES = 1376644023        # k_heston_adjoint<float, 8>, 4M x 365
-ADJ_VALU_WAVE_INSTS = 2229140625      # of which transcendental:
+ADJ_TRAFFIC_BYTES = 1382896887        # k_heston_adjoint<float, 8>, 4M x 365
+ADJ_VALU_WAVE_INSTS = 2183953125      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
-OU_TRAFFIC_BYTES = 16017922595         # k_euler<OU, float, 4>, 1M x 1000
+OU_TRAFFIC_BYTES = 16019343777         # k_euler<OU, float, 4>, 1M x 1000
 
 
 def _events(torch):
