@@ -28,6 +28,7 @@ namespace sdegpu {
 constexpr int64_t kMcChunk = 65536;
 constexpr int kMcMoments = 5;      // sum d, d^2, d^3, d^4, sum v
 constexpr int kMcLdsBins = 4096;   // histograms up to this size count in LDS
+constexpr int kMcLdsCounterWords = 8192;  // LDS words for the counter copies
 
 template <typename T>
 __device__ __forceinline__ T mc_value(const T* __restrict__ xr, int64_t i, int64_t half, int use) {
@@ -184,14 +185,21 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-// Bin of v for equally spaced edges: np.histogram's estimate
-// (v - e0) / (e_nb - e0) * nb, then moved to the bin whose edges hold v (the
-// estimate is off by at most one near an edge).
+// Bin of v (edges[0] <= v <= edges[nb]) for equally spaced edges,
+// np.histogram's rule (numpy/lib/_histograms_impl.py, the equal-bins branch
+// of histogram): the estimate (v - e0) / (e_nb - e0) * nb truncated and
+// clamped to nb - 1, then one step down if v < e[i], or one step up if
+// v >= e[i+1] and i < nb - 1.  The estimate is within one bin of the bin
+// whose edges hold v, so this lands on it; computing the estimate as
+// (v - e0) * (nb / (e_nb - e0)) instead of NumPy's division can move it by
+// one near an edge, which the same step corrects.
 __device__ __forceinline__ int mc_bin_uniform(const double* e, int nb, double v, double scale) {
   int i = (int)((v - e[0]) * scale);
-  i = i < 0 ? 0 : (i > nb - 1 ? nb - 1 : i);
-  while (i > 0 && v < e[i]) --i;
-  while (i < nb - 1 && v >= e[i + 1]) ++i;
+  i = i > nb - 1 ? nb - 1 : i;
+  if (v < e[i])
+    --i;
+  else if (i < nb - 1 && v >= e[i + 1])
+    ++i;
   return i;
 }
 
@@ -210,14 +218,19 @@ __device__ __forceinline__ int mc_bin(const double* e, int nb, double v) {
 }
 
 // counts[row][0..nb) per bin, counts[row][nb] = values outside the edges.
+// LDS variant: the edges and `copies` interleaved copies of the counters live
+// in dynamic LDS sized for nb (counter of bin b for lane l at b*copies +
+// l % copies), so lanes of one wave that land in the same bin mostly hit
+// different words and banks, and a small histogram leaves room for many
+// blocks per CU.  The copies are added when the block flushes to global
+// memory.  Without LDS (more bins than fit), global 64-bit atomics.
 template <typename T, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x, int64_t paths,
                                                          int64_t n, int use,
                                                          const double* __restrict__ edges,
-                                                         int nb, int uniform,
+                                                         int nb, int uniform, int copies,
                                                          unsigned long long* __restrict__ counts) {
-  __shared__ double se[LDS ? kMcLdsBins + 1 : 1];
-  __shared__ unsigned int sc[LDS ? kMcLdsBins + 1 : 1];
+  extern __shared__ double mc_lds[];
   const int64_t row = blockIdx.y;
   const int64_t chunk = blockIdx.x;
   const T* xr = x + row * paths;
@@ -225,11 +238,13 @@ __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x
   const double* er = edges + row * (int64_t)(nb + 1);
   unsigned long long* cr = counts + row * (int64_t)(nb + 1);
   const double* e = er;
+  unsigned int* sc = nullptr;
+  const int lane_copy = (int)(threadIdx.x & (unsigned)(copies - 1));
   if constexpr (LDS) {
-    for (int i = threadIdx.x; i <= nb; i += kBlock) {
-      se[i] = er[i];
-      sc[i] = 0u;
-    }
+    double* se = mc_lds;
+    sc = (unsigned int*)(se + nb + 1);
+    for (int i = threadIdx.x; i <= nb; i += kBlock) se[i] = er[i];
+    for (int i = threadIdx.x; i < (nb + 1) * copies; i += kBlock) sc[i] = 0u;
     __syncthreads();
     e = se;
   }
@@ -243,15 +258,18 @@ __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x
                       ? (uniform ? mc_bin_uniform(e, nb, v, scale) : mc_bin(e, nb, v))
                       : nb;
     if constexpr (LDS)
-      atomicAdd(&sc[b], 1u);
+      atomicAdd(&sc[b * copies + lane_copy], 1u);
     else
       atomicAdd(&cr[b], 1ull);
   };
   mc_for_chunk(xr, lo, hi, half, use, visit);
   if constexpr (LDS) {
     __syncthreads();
-    for (int i = threadIdx.x; i <= nb; i += kBlock)
-      if (sc[i]) atomicAdd(&cr[i], (unsigned long long)sc[i]);
+    for (int i = threadIdx.x; i <= nb; i += kBlock) {
+      unsigned int c = 0u;
+      for (int k = 0; k < copies; ++k) c += sc[i * copies + k];
+      if (c) atomicAdd(&cr[i], (unsigned long long)c);
+    }
   }
 }
 
@@ -431,22 +449,27 @@ extern "C" int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, i
   if (int e = launch_status("k_zero_u64")) return e;
   const dim3 grid((unsigned)sh.nchunks, (unsigned)rows);
   const bool lds = nbins <= kMcLdsBins;
+  // counter copies: a power of two up to 16 within kMcLdsCounterWords words
+  int copies = 1;
+  while (copies < 16 && (int64_t)(nbins + 1) * copies * 2 <= kMcLdsCounterWords) copies *= 2;
+  const size_t lds_bytes =
+      lds ? (size_t)(nbins + 1) * sizeof(double) + (size_t)(nbins + 1) * copies * sizeof(unsigned)
+          : 0;
+  const int up = uniform ? 1 : 0;
   if (dtype == SDEGPU_F32) {
     if (lds)
-      hipLaunchKernelGGL((k_mc_histogram<float, true>), grid, dim3(kBlock), 0, st,
-                         (const float*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0, counts);
+      hipLaunchKernelGGL((k_mc_histogram<float, true>), grid, dim3(kBlock), lds_bytes, st,
+                         (const float*)x, paths, sh.n, use, edges, nbins, up, copies, counts);
     else
       hipLaunchKernelGGL((k_mc_histogram<float, false>), grid, dim3(kBlock), 0, st,
-                         (const float*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0, counts);
+                         (const float*)x, paths, sh.n, use, edges, nbins, up, 1, counts);
   } else {
     if (lds)
-      hipLaunchKernelGGL((k_mc_histogram<double, true>), grid, dim3(kBlock), 0, st,
-                         (const double*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0,
-                         counts);
+      hipLaunchKernelGGL((k_mc_histogram<double, true>), grid, dim3(kBlock), lds_bytes, st,
+                         (const double*)x, paths, sh.n, use, edges, nbins, up, copies, counts);
     else
       hipLaunchKernelGGL((k_mc_histogram<double, false>), grid, dim3(kBlock), 0, st,
-                         (const double*)x, paths, sh.n, use, edges, nbins, uniform ? 1 : 0,
-                         counts);
+                         (const double*)x, paths, sh.n, use, edges, nbins, up, 1, counts);
   }
   return launch_status("k_mc_histogram");
 }
