@@ -1,394 +1,13 @@
-// Euler-Maruyama path kernels: one thread per path, the whole step loop of
-// sdepy's paths_generator._generate_paths (integration.py:392-474) inside the
-// thread, the working state in registers, Philox normals generated in
-// registers, and coalesced path-major stores of the output rows.
-//
-// Per step the thread
-//   1. draws M standard normals from its own Philox stream (or reads the
-//      recorded increments in replay mode), correlates them with the packed
-//      Cholesky factor of the step and scales them by sqrt(|ds|)
-//      (wiener_source.__call__, infrastructure.py:1503-1560);
-//   2. evaluates the preset's coefficients and applies
-//      x_new = x + (A_dt*ds + A_dw*dw [+ dj])  (integrator.euler_next,
-//      integration.py:707-723, with the increments summed first, then added);
-//   3. stores the state (SDE.let, exp for log SDEs) when the step lands on a
-//      point of the output timeline (paths_generator._store_if_reached,
-//      integration.py:377-390).
-// The per-step table is the same for every path, so its loads are scalar
-// (SMEM) loads served by the scalar cache, not per-lane traffic.
-#include "common.hpp"
-#include "philox.hpp"
+// Entry point of the Euler path kernels (the kernels are in euler_kernel.hpp):
+// validation of the problem, the kernel choice and the launch.
+#include "euler_kernel.hpp"
 
 namespace sdegpu {
 
-struct EulerArgs {
-  const void* coef;
-  const double* pois;
-  const int32_t* store_row;
-  const void* x0_paths;
-  void* out;
-  const void* replay_dw;
-  const void* replay_dj;
-  int32_t* info_paths;
-  long long* info_rows;
-  double x0[SDEGPU_MAX_VARS];
-  int64_t paths;
-  int64_t anti_w, anti_j;  // antithetic half sizes (0: off)
-  int32_t nsteps, ncoef, poff, nout, out_group, has_corr, row0;
-  uint32_t exp_mask;
-  uint32_t c2w, c2p, c2j;  // counter word c2 of the normal, Poisson and jump streams
-  uint32_t kw0, kw1, ew, kj0, kj1, ej;
-};
-
-// Steps per chunk so that a chunk consumes a whole number of Philox blocks:
-// the normal with index g = s*M + v of a path lives in block g/4, lane g%4.
-template <int M>
-struct Chunking {
-  static constexpr int CH = (M % 4 == 0) ? 1 : ((M % 2 == 0) ? 2 : 4);
-  static constexpr int NZ = CH * M;
-};
-
-constexpr int kPoissonCap = 4096;
-
-template <typename T>
-__device__ __forceinline__ T fast_sqrt(T v);
-template <>
-__device__ __forceinline__ float fast_sqrt<float>(float v) {
-  return __builtin_amdgcn_sqrtf(v);
-}
-template <>
-__device__ __forceinline__ double fast_sqrt<double>(double v) {
-  return sqrt(v);
-}
-
-// Heston working state (SDEs.pack, integration.py:1759-1783, with the two
-// equations stacked on the last vshape axis of length D): groups of 2D
-// components, D log x followed by the D y of the same leading index.  The
-// log-x component i (vshape order) is at xi(i) and its variance at xi(i) + D.
-// For vshape () or (N,) there is one group: log x first, y after.
-template <int D>
-struct HestonLayout {
-  static constexpr int xi(int i) { return (i / D) * 2 * D + i % D; }
-  static constexpr int yi(int i) { return xi(i) + D; }
-  static constexpr bool is_x(int c) { return (c % (2 * D)) < D; }
-};
-
-// Which stored values are exponentiated is fixed by the model, as sdepy's
-// `log` attribute is fixed by the preset: lognorm/jumpdiff integrate log x
-// (SDE.exit exps everything, integration.py:1193-1197), full_heston/heston
-// keep log x in the x components (full_heston_SDE.result :2441-2444,
-// heston_SDE.result :2539-2541); the rest store the state as is.  A
-// compile-time rule keeps the exp out of the store path of the other models.
-template <int MODEL>
-struct OutRule {
-  static constexpr bool exp_all = MODEL == SDEGPU_MODEL_LOGNORM || MODEL == SDEGPU_MODEL_JUMPDIFF;
-};
-
-// Stores one output row.  Heston stores either the whole state (nout = M,
-// full_heston) or the N log-x components in vshape order (nout = M/2, heston).
-template <int MODEL, typename T, int M, int D>
-__device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs& a, int row,
-                                            const T (&x)[M], int64_t gid) {
-  const int64_t P = a.paths;
-  T* o_row = out + (int64_t)row * a.nout * P + gid;
-  if constexpr (MODEL == SDEGPU_MODEL_HESTON) {
-    using L = HestonLayout<D>;
-    if (a.nout == M) {
-#pragma unroll
-      for (int o = 0; o < M; ++o)
-        __builtin_nontemporal_store(L::is_x(o) ? exp(x[o]) : x[o], o_row + o * P);
-    } else {
-#pragma unroll
-      for (int i = 0; i < M / 2; ++i) __builtin_nontemporal_store(exp(x[L::xi(i)]), o_row + i * P);
-    }
-    return;
-  }
-  if (MODEL == SDEGPU_MODEL_OU && a.out_group != 1) {
-    // hull_white_SDE.let: out_x[...] = x.sum(axis=-2) over each factor group.
-#pragma unroll
-    for (int o = 0; o < M; ++o) {
-      if (o < a.nout) {
-        const int lo = o * a.out_group, hi = lo + a.out_group;
-        T val = (T)0;
-#pragma unroll
-        for (int v = 0; v < M; ++v)
-          if (v >= lo && v < hi) val = val + x[v];
-        __builtin_nontemporal_store(val, o_row + o * P);
-      }
-    }
-    return;
-  }
-#pragma unroll
-  for (int o = 0; o < M; ++o) {
-    if (o < a.nout) {
-      T val = x[o];
-      if constexpr (OutRule<MODEL>::exp_all) val = exp(val);
-      __builtin_nontemporal_store(val, o_row + o * P);
-    }
-  }
-}
-
-template <int MODEL, typename T, int M, int D>
-__global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __restrict__ coef,
-                                                  const int32_t* __restrict__ store_row,
-                                                  T* __restrict__ out) {
-  constexpr int CH = Chunking<M>::CH;
-  constexpr int NZ = Chunking<M>::NZ;
-  constexpr bool kJump = MODEL == SDEGPU_MODEL_JUMPDIFF;
-  constexpr bool kHeston = MODEL == SDEGPU_MODEL_HESTON;
-  constexpr int N = kHeston ? M / 2 : M;
-
-  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (gid >= a.paths) return;
-  const int64_t P = a.paths;
-  const uint32_t path = (uint32_t)gid;
-  // antithetic halves: the trailing paths replay the draws of the leading
-  // ones, with the Wiener normals negated (odd_wiener_source) and the jumps
-  // unchanged (even_cpoisson_source)
-  const bool flip_w = a.anti_w > 0 && gid >= a.anti_w;
-  const uint32_t path_w = flip_w ? (uint32_t)(gid - a.anti_w) : path;
-  const uint32_t path_j = (a.anti_j > 0 && gid >= a.anti_j) ? (uint32_t)(gid - a.anti_j) : path;
-
-  T x[M];
-  if (a.x0_paths) {
-#pragma unroll
-    for (int v = 0; v < M; ++v) x[v] = ((const T*)a.x0_paths)[v * P + gid];
-  } else {
-#pragma unroll
-    for (int v = 0; v < M; ++v) x[v] = (T)a.x0[v];
-  }
-  if (a.row0 >= 0) store_state<MODEL, T, M, D>(out, a, a.row0, x, gid);
-
-  int cnt[M];
-#pragma unroll
-  for (int v = 0; v < M; ++v) cnt[v] = 0;
-  int interval_jumps = 0;
-
-  const bool replay = a.replay_dw != nullptr;
-  const bool replay_j = a.replay_dj != nullptr;
-
-  for (int s0 = 0; s0 < a.nsteps; s0 += CH) {
-    T z[NZ];
-    uint32_t upois[kJump ? NZ : 1];
-    if (!replay) {
-      const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
-#pragma unroll
-      for (int j = 0; j < NZ / 4; ++j)
-        normals4<T>(b0 + j, path_w, a.c2w, a.ew, a.kw0, a.kw1, z + 4 * j);
-      if (flip_w) {
-#pragma unroll
-        for (int i = 0; i < NZ; ++i) z[i] = -z[i];
-      }
-    }
-    if constexpr (kJump) {
-      if (!replay_j) {
-        const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
-#pragma unroll
-        for (int j = 0; j < NZ / 4; ++j) {
-          const ctr4 r = philox4x32_10(ctr4{b0 + j, path_j, a.c2p, a.ej}, a.kj0, a.kj1);
-          upois[4 * j + 0] = r.x;
-          upois[4 * j + 1] = r.y;
-          upois[4 * j + 2] = r.z;
-          upois[4 * j + 3] = r.w;
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < CH; ++k) {
-      const int s = s0 + k;
-      if (s >= a.nsteps) break;
-      const T* c = coef + (int64_t)s * a.ncoef;
-      const T ds = c[0];
-
-      // OU with correlations: the host folds sigma_i * sqrt|ds| into the
-      // Cholesky rows, and the update below applies them to z directly.
-      constexpr bool kOU = MODEL == SDEGPU_MODEL_OU;
-      const bool ou_folded = kOU && a.has_corr && !replay;
-      T dw[M];
-      if (replay) {
-        const T* rp = (const T*)a.replay_dw + (int64_t)s * M * P + gid;
-#pragma unroll
-        for (int v = 0; v < M; ++v) dw[v] = rp[v * P];
-      } else {
-        const T sq = c[1];
-        if (kOU && a.has_corr) {
-          // folded into the OU update
-        } else if (a.has_corr) {
-          const T* L = c + 2;
-          int idx = 0;
-#pragma unroll
-          for (int i = 0; i < M; ++i) {
-            T acc = (T)0;
-#pragma unroll
-            for (int j = 0; j <= i; ++j) acc = acc + L[idx++] * z[k * M + j];
-            dw[i] = acc * sq;
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < M; ++v) dw[v] = z[k * M + v] * sq;
-        }
-      }
-
-      const T* p = c + a.poff;
-      if constexpr (MODEL == SDEGPU_MODEL_WIENER || MODEL == SDEGPU_MODEL_LOGNORM) {
-        // wiener_SDE: {'dt': mu, 'dw': sigma}; lognorm_SDE: {'dt': mu - sigma^2/2, 'dw': sigma}
-#pragma unroll
-        for (int v = 0; v < M; ++v) x[v] = x[v] + (p[v] * ds + p[M + v] * dw[v]);
-      } else if constexpr (MODEL == SDEGPU_MODEL_OU) {
-        // ornstein_uhlenbeck_SDE: {'dt': k*(theta - x), 'dw': sigma}, i.e.
-        // x + k (theta - x) ds + sigma dw = (1 - k ds) x + (k theta ds + sigma dw)
-        // with a = 1 - k ds, b = k theta ds precomputed per step on the host.
-        // Correlated: sigma dw = (sigma sqrt|ds| L) z with the folded rows, so
-        // x_i = a_i x_i + b_i + sum_j Lf_ij z_j, one FMA chain per component.
-        if (ou_folded) {
-          const T* L = c + 2;
-          int idx = 0;
-#pragma unroll
-          for (int i = 0; i < M; ++i) {
-            T acc = fma(p[i], x[i], p[M + i]);
-#pragma unroll
-            for (int j = 0; j <= i; ++j) acc = fma(L[idx++], z[k * M + j], acc);
-            x[i] = acc;
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < M; ++v) x[v] = p[v] * x[v] + (p[M + v] + p[2 * M + v] * dw[v]);
-        }
-      } else if constexpr (MODEL == SDEGPU_MODEL_CIR) {
-        // cox_ingersoll_ross_SDE: x+ = max(x, 0); {'dt': k*(theta - x+), 'dw': xi*sqrt(x+)}
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-          const T xp = x[v] > (T)0 ? x[v] : (T)0;
-          const T adt = p[M + v] * (p[v] - xp);
-          const T adw = p[2 * M + v] * fast_sqrt<T>(xp);
-          x[v] = x[v] + (adt * ds + adw * dw[v]);
-        }
-      } else if constexpr (kHeston) {
-        // full_heston_SDE (full truncation): y+ = max(y, 0);
-        //   log x: {'dt': mu - sigma*sigma*y+/2, 'dw': sigma*sqrt(y+)}
-        //   y:     {'dt': k*(theta - y+),        'dw': xi*sqrt(y+)}
-        using L = HestonLayout<D>;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const int ix = L::xi(i), iy = L::yi(i);
-          const T y = x[iy];
-          cnt[i] += (y < (T)0) ? 1 : 0;
-          const T yp = y > (T)0 ? y : (T)0;
-          const T sy = fast_sqrt<T>(yp);
-          const T mu = p[i], sig = p[N + i], th = p[2 * N + i], kk = p[3 * N + i],
-                  xi = p[4 * N + i];
-          const T axt = mu - sig * sig * yp * (T)0.5;
-          const T axw = sig * sy;
-          const T ayt = kk * (th - yp);
-          const T ayw = xi * sy;
-          x[ix] = x[ix] + (axt * ds + axw * dw[ix]);
-          x[iy] = y + (ayt * ds + ayw * dw[iy]);
-        }
-      } else if constexpr (kJump) {
-        // jumpdiff_SDE: {'dt': mu - sigma^2/2, 'dw': sigma, 'dj': 1}, no martingale correction;
-        // dj = sign(ds) * sum of n ~ Poisson(lam*|ds|) variates a + b*N(0,1)
-        // (cpoisson_source.__call__ with y = norm_rv(a, b)).
-        T dj[M];
-        if (replay_j) {
-          const T* rp = (const T*)a.replay_dj + (int64_t)s * M * P + gid;
-#pragma unroll
-          for (int v = 0; v < M; ++v) dj[v] = rp[v * P];
-        } else {
-          const double* pq = a.pois + (int64_t)s * M * 2;
-#pragma unroll
-          for (int v = 0; v < M; ++v) {
-            const int n =
-                poisson_inv(u01_f64(upois[k * M + v]), pq[2 * v], pq[2 * v + 1], kPoissonCap);
-            T acc = (T)0;
-            if (n > 0) {
-              const T ja = p[2 * M + v], jb = p[3 * M + v];
-              const uint32_t slot = (uint32_t)(s * M + v);
-              T zz[4];
-              for (int j = 0; j < n; ++j) {
-                if ((j & 3) == 0)
-                  normals4<T>(slot, path_j, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0,
-                              a.kj1, zz);
-                const int lane = j & 3;
-                const T zj = lane == 0 ? zz[0] : lane == 1 ? zz[1] : lane == 2 ? zz[2] : zz[3];
-                acc = acc + (ja + jb * zj);
-              }
-            }
-            const bool neg = ds < (T)0;
-            dj[v] = neg ? -acc : acc;
-            cnt[v] += neg ? -n : n;
-            interval_jumps += neg ? -n : n;
-          }
-        }
-#pragma unroll
-        for (int v = 0; v < M; ++v)
-          x[v] = x[v] + ((p[v] * ds + p[M + v] * dw[v]) + dj[v]);
-      }
-
-      const int row = store_row[s];
-      if (row >= 0) {
-        store_state<MODEL, T, M, D>(out, a, row, x, gid);
-        if constexpr (kJump) {
-          if (a.info_rows && interval_jumps != 0)
-            atomicAdd((unsigned long long*)(a.info_rows + row), (unsigned long long)(long long)interval_jumps);
-          interval_jumps = 0;
-        }
-      }
-    }
-  }
-
-  if (a.info_paths && (kHeston || kJump)) {
-#pragma unroll
-    for (int v = 0; v < (kHeston ? N : M); ++v) a.info_paths[v * P + gid] = cnt[v];
-  }
-}
-
-template <typename T>
-using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, T*);
-
-template <int MODEL, typename T>
-static EulerKernel<T> pick_m(int m) {
-  switch (m) {
-    case 1: return k_euler<MODEL, T, 1, 1>;
-    case 2: return k_euler<MODEL, T, 2, 1>;
-    case 3: return k_euler<MODEL, T, 3, 1>;
-    case 4: return k_euler<MODEL, T, 4, 1>;
-    case 5: return k_euler<MODEL, T, 5, 1>;
-    case 6: return k_euler<MODEL, T, 6, 1>;
-    case 7: return k_euler<MODEL, T, 7, 1>;
-    case 8: return k_euler<MODEL, T, 8, 1>;
-    default: return nullptr;
-  }
-}
-
-// Heston kernels by (M, D): D log-x components per group of 2D.
-template <typename T>
-static EulerKernel<T> pick_heston(int m, int d) {
-  constexpr int H = SDEGPU_MODEL_HESTON;
-  switch (m * 16 + d) {
-    case 2 * 16 + 1: return k_euler<H, T, 2, 1>;
-    case 4 * 16 + 1: return k_euler<H, T, 4, 1>;
-    case 4 * 16 + 2: return k_euler<H, T, 4, 2>;
-    case 6 * 16 + 1: return k_euler<H, T, 6, 1>;
-    case 6 * 16 + 3: return k_euler<H, T, 6, 3>;
-    case 8 * 16 + 1: return k_euler<H, T, 8, 1>;
-    case 8 * 16 + 2: return k_euler<H, T, 8, 2>;
-    case 8 * 16 + 4: return k_euler<H, T, 8, 4>;
-    default: return nullptr;
-  }
-}
-
-template <typename T>
-static EulerKernel<T> pick(int model, int m, int d) {
-  switch (model) {
-    case SDEGPU_MODEL_WIENER: return pick_m<SDEGPU_MODEL_WIENER, T>(m);
-    case SDEGPU_MODEL_LOGNORM: return pick_m<SDEGPU_MODEL_LOGNORM, T>(m);
-    case SDEGPU_MODEL_OU: return pick_m<SDEGPU_MODEL_OU, T>(m);
-    case SDEGPU_MODEL_CIR: return pick_m<SDEGPU_MODEL_CIR, T>(m);
-    case SDEGPU_MODEL_HESTON: return pick_heston<T>(m, d);
-    case SDEGPU_MODEL_JUMPDIFF: return pick_m<SDEGPU_MODEL_JUMPDIFF, T>(m);
-    default: return nullptr;
-  }
-}
+// The instances without recorded increments and antithetic halves
+// (euler_plain.hip).
+EulerKernel<float> euler_plain_f32(int model, int m, int d);
+EulerKernel<double> euler_plain_f64(int model, int m, int d);
 
 // exp_mask the model implies for nout stored values (see OutRule and
 // store_state): every value for lognorm/jumpdiff, the log-x values for
@@ -510,15 +129,19 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.kj1 = pr->key_j[1];
   a.ej = pr->epoch_j;
 
+  // no recorded increments and no antithetic halves: the plain instance
+  const bool plain = !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0;
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {
-    EulerKernel<float> k = pick<float>(pr->model, m, hd);
+    EulerKernel<float> k =
+        plain ? euler_plain_f32(pr->model, m, hd) : pick_kernel<true, float>(pr->model, m, hd);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
                        (float*)pr->out);
   } else {
-    EulerKernel<double> k = pick<double>(pr->model, m, hd);
+    EulerKernel<double> k =
+        plain ? euler_plain_f64(pr->model, m, hd) : pick_kernel<true, double>(pr->model, m, hd);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
                        (double*)pr->out);
