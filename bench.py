@@ -258,9 +258,9 @@ def main():
         legs.add('batched64')
     legs -= set(args.skip.split(','))
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-        print('bench.py measures one GPU: this build runs the path kernels on '
-              'a single MI355X (paths shard by range across GPUs, see '
-              'DESIGN.md); --gpus {} is not supported'.format(args.gpus),
+        print('bench.py measures one GPU: the path kernels run on a single '
+              'MI355X (a multi-GPU run would shard the paths by range); '
+              '--gpus {} is not supported'.format(args.gpus),
               file=sys.stderr)
         sys.exit(2)
 
