@@ -132,6 +132,21 @@ typedef struct sdegpu_euler_problem {
                                  (SDEs.pack, integration.py:1759-1783); 0 means nvars/2 (one
                                  group).  Heston stores nout = nvars (the state, log x exp'd)
                                  or nout = nvars/2 (x only, in vshape order). */
+  /* Component group, for working states of more than SDEGPU_MAX_VARS
+     components (0 for all four: one group, the whole state).  The call steps
+     components comp0 .. comp0+nvars-1 of a state of nvars_total components,
+     none of which is coupled to a component outside the group (correlation,
+     Heston pairs and Hull-White factors stay inside it), and stores its nout
+     values at out0 .. out0+nout-1 of output rows of out_stride values.  The
+     group's normals keep their flat indices s*nvars_total + comp0 + v, so the
+     paths equal those of one launch over the whole state.  coef, pois and x0
+     hold the group's own table and values; x0_paths, replay_dw/dj,
+     info_paths and out are the full-state arrays ([nvars_total][paths],
+     [nsteps][nvars_total][paths], ..., [nrows][out_stride][paths]). */
+  int32_t comp0;
+  int32_t nvars_total;
+  int32_t out0;
+  int32_t out_stride;
 } sdegpu_euler_problem;
 
 /* Library and device queries. */
@@ -239,7 +254,11 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  *
  * sdegpu_wiener_fill: wiener_source.__call__ (infrastructure.py:1503-1560).
  *   coef [nslots][ncoef] dtype with row layout [ds, sqrt|ds|, L...] as above;
- *   out [nslots][nvars][paths] = (L z) * sqrt|ds|.
+ *   out [nslots][nvars][paths] = (L z) * sqrt|ds|.  With nvars_total > 0 the
+ *   call fills the component group comp0 .. comp0+nvars-1 (uncorrelated with
+ *   the others, L its own block) of out [nslots][nvars_total][paths], with the
+ *   normals of the flat indices s*nvars_total + comp0 + v, as
+ *   sdegpu_euler_problem's component groups draw them.
  * sdegpu_poisson_fill: poisson_source.__call__ (infrastructure.py:1617-1633).
  *   pois [nslots][nvars][2] = (lam*|ds|, exp(-lam*|ds|)); sign [nslots] = sign(ds);
  *   out int64 [nslots][nvars][paths].
@@ -249,7 +268,8 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  */
 int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
                        int32_t nvars, int64_t paths, int64_t nslots, const void* coef,
-                       int32_t ncoef, int32_t has_corr, void* out, void* stream);
+                       int32_t ncoef, int32_t has_corr, int32_t comp0, int32_t nvars_total,
+                       void* out, void* stream);
 int sdegpu_poisson_fill(const uint32_t key[2], uint32_t epoch, uint32_t stream_tag, int32_t nvars,
                         int64_t paths, int64_t nslots, const double* pois, const int32_t* sign,
                         long long* out, void* stream);

@@ -51,6 +51,8 @@ class EulerProblem(ctypes.Structure):
         ('key_j', c_uint32 * 2), ('epoch_j', c_uint32),
         ('anti_w', c_int64), ('anti_j', c_int64),
         ('heston_width', c_int32),
+        ('comp0', c_int32), ('nvars_total', c_int32),
+        ('out0', c_int32), ('out_stride', c_int32),
     ]
 
 
@@ -89,7 +91,7 @@ SIGNATURES = [
       c_void_p, c_size_t, c_void_p]),
     ('sdegpu_wiener_fill', ctypes.c_int,
      [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
-      c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
+      c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p]),
     ('sdegpu_poisson_fill', ctypes.c_int,
      [_U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64, c_void_p,
       c_void_p, c_void_p, c_void_p]),

@@ -57,9 +57,10 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   if (heston && (m % 2) != 0)
     return fail(SDEGPU_EINVAL, "the Heston working state has an even size, not %d", m);
   const int hd = heston ? (pr->heston_width == 0 ? m / 2 : pr->heston_width) : 1;
-  if (heston && (hd < 1 || (m / 2) % hd != 0))
-    return fail(SDEGPU_EINVAL, "Heston group width %d does not divide %d log-x components",
-                pr->heston_width, m / 2);
+  if (heston && (hd < 1 || (m / 2) % hd != 0 || pr->comp0 % (2 * hd) != 0))
+    return fail(SDEGPU_EINVAL,
+                "Heston group width %d does not divide %d log-x components starting at %d",
+                pr->heston_width, m / 2, pr->comp0);
   if (heston && pr->nout != m && pr->nout != m / 2)
     return fail(SDEGPU_EINVAL, "Heston stores %d (whole state) or %d (log x) values, not %d", m,
                 m / 2, pr->nout);
@@ -71,16 +72,28 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EINVAL, "table row width %d < %d required by the model", pr->ncoef,
                 poff + nparam);
   const int group = pr->out_mode == SDEGPU_OUT_SUM ? pr->out_group : 1;
-  if (group < 1 || pr->nout < 1 || (int64_t)pr->nout * group > m)
-    return fail(SDEGPU_EINVAL, "invalid output map: nout=%d group=%d for %d components",
-                pr->nout, group, m);
+  if (group < 1 || pr->nout < 1 || (int64_t)pr->nout * group > m || pr->comp0 % group != 0)
+    return fail(SDEGPU_EINVAL,
+                "invalid output map: nout=%d group=%d for %d components starting at %d",
+                pr->nout, group, m, pr->comp0);
   if (pr->nsteps > 0 && (!pr->coef || !pr->store_row))
     return fail(SDEGPU_EINVAL, "missing step table");
   if (!pr->out) return fail(SDEGPU_EINVAL, "missing output buffer");
   const bool jump = pr->model == SDEGPU_MODEL_JUMPDIFF;
   if (jump && pr->nsteps > 0 && !pr->replay_dj && !pr->pois)
     return fail(SDEGPU_EINVAL, "jump model without Poisson table");
-  if ((int64_t)pr->nsteps * m > 0xFFFFFFFFll)
+  // component group (0 everywhere: the whole state)
+  const bool grouped =
+      pr->comp0 != 0 || pr->nvars_total != 0 || pr->out0 != 0 || pr->out_stride != 0;
+  const int m_total = pr->nvars_total != 0 ? pr->nvars_total : m;
+  const int out_stride = pr->out_stride != 0 ? pr->out_stride : pr->nout;
+  if (pr->comp0 < 0 || m_total < m || pr->comp0 > m_total - m)
+    return fail(SDEGPU_EINVAL, "component group %d..%d outside a state of %d components",
+                pr->comp0, pr->comp0 + m - 1, m_total);
+  if (pr->out0 < 0 || out_stride < pr->nout || pr->out0 > out_stride - pr->nout)
+    return fail(SDEGPU_EINVAL, "output values %d..%d outside rows of %d values", pr->out0,
+                pr->out0 + pr->nout - 1, out_stride);
+  if ((int64_t)pr->nsteps * m_total > 0xFFFFFFFFll)
     return fail(SDEGPU_EINVAL, "too many steps for the 32-bit Philox block counter");
   // sources with antithetics hold an even number of paths, split in halves
   // (infrastructure.py:2057-2062)
@@ -117,6 +130,10 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.out_group = group;
   a.has_corr = pr->has_corr ? 1 : 0;
   a.row0 = pr->row0;
+  a.comp0 = pr->comp0;
+  a.m_total = m_total;
+  a.out0 = pr->out0;
+  a.out_stride = out_stride;
   a.exp_mask = pr->exp_mask;
   const uint32_t tag = (uint32_t)pr->stream_tag & 0xF0u;
   a.c2w = tag | kStreamNormal;
@@ -130,7 +147,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.ej = pr->epoch_j;
 
   // no recorded increments and no antithetic halves: the plain instance
-  const bool plain = !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0;
+  const bool plain =
+      !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 && !grouped;
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {

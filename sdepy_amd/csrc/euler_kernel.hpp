@@ -46,6 +46,11 @@ struct EulerArgs {
   uint32_t exp_mask;
   uint32_t c2w, c2p, c2j;  // counter word c2 of the normal, Poisson and jump streams
   uint32_t kw0, kw1, ew, kj0, kj1, ej;
+  // component group (general instance only): this launch steps components
+  // comp0 .. comp0+M-1 of a working state of m_total and stores its values at
+  // out0 .. out0+nout-1 of rows of out_stride values; for an ungrouped launch
+  // comp0 = out0 = 0, m_total = M, out_stride = nout
+  int32_t comp0, m_total, out0, out_stride;
 };
 
 // Steps per chunk so that a chunk consumes a whole number of Philox blocks:
@@ -95,11 +100,12 @@ struct OutRule {
 // Stores one output row.  Heston stores either the whole state (nout = M,
 // full_heston) or the N log-x components in vshape order (nout = M/2, heston).
 // Non-temporal stores: the output is not read again by this kernel.
-template <int MODEL, typename T, int M, int D>
+template <int MODEL, typename T, int M, int D, bool GRP>
 __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs& a, int row,
                                             const T (&x)[M], int64_t gid) {
   const int64_t P = a.paths;
-  T* o_row = out + (int64_t)row * a.nout * P + gid;
+  T* o_row = GRP ? out + ((int64_t)row * a.out_stride + a.out0) * P + gid
+                 : out + (int64_t)row * a.nout * P + gid;
   auto put = [&](T v, int o) { __builtin_nontemporal_store(v, o_row + o * P); };
   if constexpr (MODEL == SDEGPU_MODEL_HESTON) {
     using L = HestonLayout<D>;
@@ -159,15 +165,20 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
   const uint32_t path_j =
       (GEN && a.anti_j > 0 && gid >= a.anti_j) ? (uint32_t)(gid - a.anti_j) : path;
 
+  // component group: offsets into the full-state arrays (general instance)
+  const int64_t mt = GEN ? a.m_total : M;
+  const int64_t c0 = GEN ? a.comp0 : 0;
+  const bool grouped = GEN && (a.m_total != M || a.comp0 != 0);
+
   T x[M];
   if (a.x0_paths) {
 #pragma unroll
-    for (int v = 0; v < M; ++v) x[v] = ((const T*)a.x0_paths)[v * P + gid];
+    for (int v = 0; v < M; ++v) x[v] = ((const T*)a.x0_paths)[(c0 + v) * P + gid];
   } else {
 #pragma unroll
     for (int v = 0; v < M; ++v) x[v] = (T)a.x0[v];
   }
-  if (a.row0 >= 0) store_state<MODEL, T, M, D>(out, a, a.row0, x, gid);
+  if (a.row0 >= 0) store_state<MODEL, T, M, D, GEN>(out, a, a.row0, x, gid);
 
   int cnt[M];
 #pragma unroll
@@ -181,10 +192,18 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
     T z[NZ];
     uint32_t upois[kJump ? NZ : 1];
     if (!replay) {
-      const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
+      if (grouped) {
+        // the group's normals keep their flat indices s*m_total + comp0 + v
 #pragma unroll
-      for (int j = 0; j < NZ / 4; ++j)
-        normals4<T>(b0 + j, path_w, a.c2w, a.ew, a.kw0, a.kw1, z + 4 * j);
+        for (int k = 0; k < CH; ++k)
+          normals_at<T, M>((uint64_t)(s0 + k) * mt + c0, path_w, a.c2w, a.ew, a.kw0, a.kw1,
+                           z + k * M);
+      } else {
+        const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
+#pragma unroll
+        for (int j = 0; j < NZ / 4; ++j)
+          normals4<T>(b0 + j, path_w, a.c2w, a.ew, a.kw0, a.kw1, z + 4 * j);
+      }
       if (flip_w) {
 #pragma unroll
         for (int i = 0; i < NZ; ++i) z[i] = -z[i];
@@ -192,14 +211,21 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
     }
     if constexpr (kJump) {
       if (!replay_j) {
-        const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
+        if (grouped) {
 #pragma unroll
-        for (int j = 0; j < NZ / 4; ++j) {
-          const ctr4 r = philox4x32_10(ctr4{b0 + j, path_j, a.c2p, a.ej}, a.kj0, a.kj1);
-          upois[4 * j + 0] = r.x;
-          upois[4 * j + 1] = r.y;
-          upois[4 * j + 2] = r.z;
-          upois[4 * j + 3] = r.w;
+          for (int k = 0; k < CH; ++k)
+            words_at<M>((uint64_t)(s0 + k) * mt + c0, path_j, a.c2p, a.ej, a.kj0, a.kj1,
+                        upois + k * M);
+        } else {
+          const uint32_t b0 = (uint32_t)(((int64_t)s0 * M) >> 2);
+#pragma unroll
+          for (int j = 0; j < NZ / 4; ++j) {
+            const ctr4 r = philox4x32_10(ctr4{b0 + j, path_j, a.c2p, a.ej}, a.kj0, a.kj1);
+            upois[4 * j + 0] = r.x;
+            upois[4 * j + 1] = r.y;
+            upois[4 * j + 2] = r.z;
+            upois[4 * j + 3] = r.w;
+          }
         }
       }
     }
@@ -216,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
       const bool ou_folded = kOU && a.has_corr && !replay;
       T dw[M];
       if (replay) {
-        const T* rp = (const T*)a.replay_dw + (int64_t)s * M * P + gid;
+        const T* rp = (const T*)a.replay_dw + ((int64_t)s * mt + c0) * P + gid;
 #pragma unroll
         for (int v = 0; v < M; ++v) dw[v] = rp[v * P];
       } else {
@@ -300,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
         // (cpoisson_source.__call__ with y = norm_rv(a, b)).
         T dj[M];
         if (replay_j) {
-          const T* rp = (const T*)a.replay_dj + (int64_t)s * M * P + gid;
+          const T* rp = (const T*)a.replay_dj + ((int64_t)s * mt + c0) * P + gid;
 #pragma unroll
           for (int v = 0; v < M; ++v) dj[v] = rp[v * P];
         } else {
@@ -312,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
             T acc = (T)0;
             if (n > 0) {
               const T ja = p[2 * M + v], jb = p[3 * M + v];
-              const uint32_t slot = (uint32_t)(s * M + v);
+              const uint32_t slot = (uint32_t)((int64_t)s * mt + c0 + v);
               T zz[4];
               for (int j = 0; j < n; ++j) {
                 if ((j & 3) == 0)
@@ -336,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 
       const int row = store_row[s];
       if (row >= 0) {
-        store_state<MODEL, T, M, D>(out, a, row, x, gid);
+        store_state<MODEL, T, M, D, GEN>(out, a, row, x, gid);
         if constexpr (kJump) {
           if (a.info_rows && interval_jumps != 0)
             atomicAdd((unsigned long long*)(a.info_rows + row), (unsigned long long)(long long)interval_jumps);
@@ -347,8 +373,11 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
   }
 
   if (a.info_paths && (kHeston || kJump)) {
+    // Heston counts per log-x value (the group's first is comp0 / 2), jump
+    // counts per component
+    const int64_t i0 = kHeston ? c0 / 2 : c0;
 #pragma unroll
-    for (int v = 0; v < (kHeston ? N : M); ++v) a.info_paths[v * P + gid] = cnt[v];
+    for (int v = 0; v < (kHeston ? N : M); ++v) a.info_paths[(i0 + v) * P + gid] = cnt[v];
   }
 }
 

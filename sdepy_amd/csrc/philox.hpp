@@ -153,6 +153,74 @@ __device__ __forceinline__ void normals4(uint32_t blk, uint32_t path, uint32_t c
   Gauss<T>::pair(r.z, r.w, z[2], z[3]);
 }
 
+// The normals with flat indices g .. g+M-1 of a path's stream, for any g
+// (used when a launch steps a group of components that starts at an index
+// that is not a multiple of 4): the blocks covering them are drawn and the
+// values picked by the offset g % 4, which is the same for every lane.
+template <typename T, int M>
+__device__ __forceinline__ void normals_at(uint64_t g, uint32_t path, uint32_t c2, uint32_t epoch,
+                                           uint32_t k0, uint32_t k1, T* z) {
+  constexpr int NB = (M + 6) / 4;  // blocks covering M values at any offset
+  T zz[4 * NB];
+  const uint32_t b0 = (uint32_t)(g >> 2);
+#pragma unroll
+  for (int q = 0; q < NB; ++q) normals4<T>(b0 + q, path, c2, epoch, k0, k1, zz + 4 * q);
+  switch ((int)(g & 3u)) {
+    case 0:
+#pragma unroll
+      for (int j = 0; j < M; ++j) z[j] = zz[j];
+      break;
+    case 1:
+#pragma unroll
+      for (int j = 0; j < M; ++j) z[j] = zz[j + 1];
+      break;
+    case 2:
+#pragma unroll
+      for (int j = 0; j < M; ++j) z[j] = zz[j + 2];
+      break;
+    default:
+#pragma unroll
+      for (int j = 0; j < M; ++j) z[j] = zz[j + 3];
+      break;
+  }
+}
+
+// The raw 32-bit words with flat indices g .. g+M-1 (Poisson uniforms), as
+// normals_at.
+template <int M>
+__device__ __forceinline__ void words_at(uint64_t g, uint32_t path, uint32_t c2, uint32_t epoch,
+                                         uint32_t k0, uint32_t k1, uint32_t* w) {
+  constexpr int NB = (M + 6) / 4;
+  uint32_t ww[4 * NB];
+  const uint32_t b0 = (uint32_t)(g >> 2);
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const ctr4 r = philox4x32_10(ctr4{b0 + q, path, c2, epoch}, k0, k1);
+    ww[4 * q] = r.x;
+    ww[4 * q + 1] = r.y;
+    ww[4 * q + 2] = r.z;
+    ww[4 * q + 3] = r.w;
+  }
+  switch ((int)(g & 3u)) {
+    case 0:
+#pragma unroll
+      for (int j = 0; j < M; ++j) w[j] = ww[j];
+      break;
+    case 1:
+#pragma unroll
+      for (int j = 0; j < M; ++j) w[j] = ww[j + 1];
+      break;
+    case 2:
+#pragma unroll
+      for (int j = 0; j < M; ++j) w[j] = ww[j + 2];
+      break;
+    default:
+#pragma unroll
+      for (int j = 0; j < M; ++j) w[j] = ww[j + 3];
+      break;
+  }
+}
+
 // Poisson variate by sequential inversion of the CDF, in double precision on
 // every target so that counts are bit-identical to the host twin.
 // lam = intensity * |ds|, p0 = exp(-lam) (precomputed on the host in double).

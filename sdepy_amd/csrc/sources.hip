@@ -28,28 +28,40 @@ int hip_status(hipError_t e, const char* what) {
 }
 
 // wiener_source.__call__: dz = L z sqrt(|dt|), z ~ N(0, I), slot-major output.
+// A component group (m_total > M or comp0 > 0) fills components
+// comp0 .. comp0+M-1 of slots of m_total, with the normals of their flat
+// indices s*m_total + comp0 + v.
 template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_wiener_fill(uint32_t k0, uint32_t k1, uint32_t epoch,
                                                         uint32_t c2, int64_t paths, int64_t nslots,
                                                         const T* __restrict__ coef, int ncoef,
-                                                        int has_corr, T* __restrict__ out) {
+                                                        int has_corr, int comp0, int m_total,
+                                                        T* __restrict__ out) {
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (gid >= paths) return;
   const uint32_t path = (uint32_t)gid;
   constexpr int CH = (M % 4 == 0) ? 1 : ((M % 2 == 0) ? 2 : 4);
   constexpr int NZ = CH * M;
+  const bool grouped = m_total != M || comp0 != 0;
   for (int64_t s0 = 0; s0 < nslots; s0 += CH) {
     T z[NZ];
-    const uint32_t b0 = (uint32_t)((s0 * M) >> 2);
+    if (grouped) {
 #pragma unroll
-    for (int j = 0; j < NZ / 4; ++j) normals4<T>(b0 + j, path, c2, epoch, k0, k1, z + 4 * j);
+      for (int k = 0; k < CH; ++k)
+        normals_at<T, M>((uint64_t)(s0 + k) * m_total + comp0, path, c2, epoch, k0, k1,
+                         z + k * M);
+    } else {
+      const uint32_t b0 = (uint32_t)((s0 * M) >> 2);
+#pragma unroll
+      for (int j = 0; j < NZ / 4; ++j) normals4<T>(b0 + j, path, c2, epoch, k0, k1, z + 4 * j);
+    }
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
       const int64_t s = s0 + k;
       if (s >= nslots) break;
       const T* c = coef + s * ncoef;
       const T sq = c[1];
-      T* o = out + s * M * paths + gid;
+      T* o = out + (s * m_total + comp0) * paths + gid;
       if (has_corr) {
         const T* L = c + 2;
         int idx = 0;
@@ -122,7 +134,7 @@ __global__ void k_philox_fill(uint32_t k0, uint32_t k1, const uint32_t* __restri
 
 template <typename T>
 using WienerFill = void (*)(uint32_t, uint32_t, uint32_t, uint32_t, int64_t, int64_t, const T*,
-                            int, int, T*);
+                            int, int, int, int, T*);
 
 template <typename T>
 static WienerFill<T> pick_wiener(int m) {
@@ -183,11 +195,15 @@ extern "C" int sdegpu_philox_fill(const uint32_t key[2], const uint32_t* ctr, ui
 
 extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                                   uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
-                                  const void* coef, int32_t ncoef, int32_t has_corr, void* out,
-                                  void* stream) {
+                                  const void* coef, int32_t ncoef, int32_t has_corr, int32_t comp0,
+                                  int32_t nvars_total, void* out, void* stream) {
   if (!key || nvars < 1 || nvars > SDEGPU_MAX_VARS)
     return fail(SDEGPU_EUNSUPPORTED, "wiener_fill: %d components per path (1..%d supported)", nvars,
                 SDEGPU_MAX_VARS);
+  const int m_total = nvars_total != 0 ? nvars_total : nvars;
+  if (comp0 < 0 || m_total < nvars || comp0 > m_total - nvars)
+    return fail(SDEGPU_EINVAL, "wiener_fill: components %d..%d outside %d", comp0,
+                comp0 + nvars - 1, m_total);
   if (paths < 1 || paths > kMaxPaths || nslots < 0)
     return fail(SDEGPU_EINVAL, "wiener_fill: paths=%lld slots=%lld", (long long)paths,
                 (long long)nslots);
@@ -195,16 +211,18 @@ extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t
   if (!coef || !out) return fail(SDEGPU_EINVAL, "wiener_fill: null pointer");
   if (ncoef < 2 + (has_corr ? nvars * (nvars + 1) / 2 : 0))
     return fail(SDEGPU_EINVAL, "wiener_fill: table row width %d too small", ncoef);
-  if (nslots * nvars > 0xFFFFFFFFll) return fail(SDEGPU_EINVAL, "wiener_fill: too many slots");
+  if (nslots * m_total > 0xFFFFFFFFll) return fail(SDEGPU_EINVAL, "wiener_fill: too many slots");
   const uint32_t c2 = ((uint32_t)stream_tag & 0xF0u) | kStreamNormal;
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(blocks_for(paths)), block(kBlock);
   if (dtype == SDEGPU_F32)
     hipLaunchKernelGGL(pick_wiener<float>(nvars), grid, block, 0, st, key[0], key[1], epoch, c2,
-                       paths, nslots, (const float*)coef, ncoef, has_corr, (float*)out);
+                       paths, nslots, (const float*)coef, ncoef, has_corr, comp0, m_total,
+                       (float*)out);
   else if (dtype == SDEGPU_F64)
     hipLaunchKernelGGL(pick_wiener<double>(nvars), grid, block, 0, st, key[0], key[1], epoch, c2,
-                       paths, nslots, (const double*)coef, ncoef, has_corr, (double*)out);
+                       paths, nslots, (const double*)coef, ncoef, has_corr, comp0, m_total,
+                       (double*)out);
   else
     return fail(SDEGPU_EUNSUPPORTED, "wiener_fill: dtype code %d", dtype);
   return launch_status("k_wiener_fill");

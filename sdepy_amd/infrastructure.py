@@ -207,9 +207,30 @@ class wiener_source(_philox_source):
                     'and correlation matrix shape {}'
                     .format(self.vshape, corr.shape))
 
-    def step_table(self, tt, dtt):
-        """Rows [ds, sqrt|ds|, L...] (FP64) for slots at times tt, steps dtt."""
+    def groups(self):
+        """[(first component, size)] of the fill launches: all components at
+        once up to MAX_VARS, else groups of whole correlated blocks of the
+        last axis (of single components without correlation).  Each keeps
+        the flat normal indices, so the draws do not depend on the split."""
         M = int(np.prod(self.vshape, dtype=int))
+        if M <= _lib.MAX_VARS:
+            return [(0, M)]
+        unit = self.vshape[-1] if self.corr is not None else 1
+        if unit > _lib.MAX_VARS:
+            raise NotImplementedError(
+                'the GPU sampler correlates at most {} components, not {}'
+                .format(_lib.MAX_VARS, unit))
+        mg = (_lib.MAX_VARS // unit) * unit
+        return [(g0, min(mg, M - g0)) for g0 in range(0, M, mg)]
+
+    def step_table(self, tt, dtt, m=None):
+        """Rows [ds, sqrt|ds|, L...] (FP64) for slots at times tt, steps dtt.
+        With ``m``, the factor of a group of m components (whole correlated
+        blocks of the last axis), the same for every group."""
+        full = int(np.prod(self.vshape, dtype=int))
+        M = full if m is None else int(m)
+        gshape = self.vshape if M == full else (M // self.vshape[-1],
+                                                self.vshape[-1])
         has_corr = self.corr is not None
         ncoef = 2 + (M * (M + 1) // 2 if has_corr else 0)
         tab = np.zeros((tt.size, ncoef))
@@ -219,10 +240,9 @@ class wiener_source(_philox_source):
             if callable(self.corr):
                 for i in range(tt.size):
                     tab[i, 2:] = cholesky_packed(
-                        _corr_at(self.corr, tt[i] + dtt[i] / 2), self.vshape)
+                        _corr_at(self.corr, tt[i] + dtt[i] / 2), gshape)
             else:
-                tab[:, 2:] = cholesky_packed(_corr_at(self.corr, 0.),
-                                             self.vshape)
+                tab[:, 2:] = cholesky_packed(_corr_at(self.corr, 0.), gshape)
         return tab, has_corr
 
     def __call__(self, t, dt):
@@ -232,14 +252,18 @@ class wiener_source(_philox_source):
         code, cdt, odt = _gpu_dtype(self.dtype)
         M = int(np.prod(self.vshape, dtype=int))
         n = int(np.prod(tshape, dtype=int))
-        tab, has_corr = self.step_table(t.reshape(-1), dt.reshape(-1))
         torch = _lib.require_gpu()
-        coef = _lib.to_device(tab.astype(cdt))
         out = _lib.empty((n, M, self.paths), cdt)
-        _lib.check(_lib.lib().sdegpu_wiener_fill(
-            code, _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
-            _lib.ptr(coef), tab.shape[1], int(has_corr), _lib.ptr(out),
-            _lib.stream()))
+        epoch = self.next_epoch()
+        groups = self.groups()
+        for g0, mg in groups:
+            tab, has_corr = self.step_table(t.reshape(-1), dt.reshape(-1), m=mg)
+            coef = _lib.to_device(tab.astype(cdt))
+            one = len(groups) == 1
+            _lib.check(_lib.lib().sdegpu_wiener_fill(
+                code, _lib.key2(self.key), epoch, 0, mg, self.paths, n,
+                _lib.ptr(coef), tab.shape[1], int(has_corr), 0 if one else g0,
+                0 if one else M, _lib.ptr(out), _lib.stream()))
         res = _lib.to_host(out).astype(odt, copy=False)
         del torch
         return res.reshape(tshape + self.vshape + (self.paths,))

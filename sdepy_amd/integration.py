@@ -290,12 +290,6 @@ class _gpu_sde:
         process(es).  ``__call__`` is ``prepare(timeline).run()``."""
         tt, tt_asis, steps_tt = consolidate_timeline(timeline, self.steps,
                                                      self.i0)
-        if self._nvars() > _lib.MAX_VARS:
-            raise NotImplementedError(
-                'the GPU integrator steps at most {} working-state components '
-                'per path; {} with vshape {} needs {}'.format(
-                    _lib.MAX_VARS, type(self).__name__, self.vshape,
-                    self._nvars()))
         i0 = self.i0
         code, cdt, odt = _gpu_dtype(self.dtype)
         _lib.require_gpu()
@@ -343,25 +337,26 @@ class _gpu_sde:
         # Wiener source: in-kernel Philox draws, or recorded increments
         dw = self.sources['dw']
         host_sources = {}
+        wsrc = None
         if isinstance(dw, odd_wiener_source) and \
                 type(dw.half) is wiener_source:
             # antithetic halves: the trailing paths negate the leading ones
-            tab, has_corr = self._coef_table(dw.half, t0s, ds)
+            wsrc = dw.half
             prob.key_w[0], prob.key_w[1] = dw.half.key
             prob.epoch_w = dw.half.next_epoch()
             prob.anti_w = dw.half.paths
         elif isinstance(dw, wiener_source):
-            tab, has_corr = self._coef_table(dw, t0s, ds)
+            wsrc = dw
             prob.key_w[0], prob.key_w[1] = dw.key
             prob.epoch_w = dw.next_epoch()
         else:
-            wtab = np.stack((ds, np.sqrt(np.abs(ds))), axis=1) if nsteps \
-                else np.zeros((0, 2))
-            tab = np.concatenate((wtab, self._param_rows(t0s, ds)), axis=1)
-            has_corr = False
             host_sources['dw'] = dw
+        has_corr = wsrc is not None and wsrc.corr is not None
+        groups = self._groups(has_corr, out_mode, out_group)
+        params = self._param_rows(t0s, ds)
         dj = self.sources.get('dj')
         self._anti_dj = False
+        pois = ab = None
         if dj is not None:
             gdj = dj
             if isinstance(dj, even_cpoisson_source) and \
@@ -378,22 +373,12 @@ class _gpu_sde:
                 ab = gdj.ab_table(t0s, ds)
             else:
                 host_sources['dj'] = dj
-                pois = ab = None
-            extra = np.zeros((nsteps, 2 * M)) if ab is None else \
-                np.concatenate((ab[:, :, 0], ab[:, :, 1]), axis=1)
-            tab = np.concatenate((tab, extra), axis=1)
         self._host_dj = 'dj' in host_sources
         prob.has_corr = int(has_corr)
-        prob.ncoef = tab.shape[1]
         if nsteps:
-            d_tab = _lib.to_device(tab.astype(cdt))
             d_rows = _lib.to_device(rows)
-            keep += [d_tab, d_rows]
-            prob.coef, prob.store_row = d_tab.data_ptr(), d_rows.data_ptr()
-        if dj is not None and pois is not None and nsteps:
-            d_pois = _lib.to_device(pois)
-            keep.append(d_pois)
-            prob.pois = d_pois.data_ptr()
+            keep.append(d_rows)
+            prob.store_row = d_rows.data_ptr()
 
         # host sources, drawn in sorted id order per step (SDE.dZ)
         recorded = {k: [] for k in host_sources}
@@ -415,10 +400,7 @@ class _gpu_sde:
             keep.append(d)
             setattr(prob, 'replay_' + sid, d.data_ptr())
 
-        if x0.shape == (M,):
-            for v in range(M):
-                prob.x0[v] = float(x0[v])
-        else:
+        if x0.shape != (M,):
             d_x0 = _lib.to_device(np.ascontiguousarray(x0, dtype=cdt))
             keep.append(d_x0)
             prob.x0_paths = d_x0.data_ptr()
@@ -433,28 +415,130 @@ class _gpu_sde:
                 info_rows = _lib.zeros((xx_dev.shape[0],), np.int64)
                 prob.info_rows = info_rows.data_ptr()
 
+        # one problem per component group: its own step table, Poisson table
+        # and initial values; the full-state device arrays are shared
+        probs = []
+        for g0, mg in groups:
+            gp = _lib.EulerProblem()
+            ctypes.memmove(ctypes.addressof(gp), ctypes.addressof(prob),
+                           ctypes.sizeof(prob))
+            if wsrc is not None:
+                wtab, _ = wsrc.step_table(t0s, ds, m=mg)
+            else:
+                wtab = np.stack((ds, np.sqrt(np.abs(ds))), axis=1) if nsteps \
+                    else np.zeros((0, 2))
+            gpar = self._group_params(params, g0, mg)
+            wtab = self._fold(wtab, gpar, has_corr)
+            tab = np.concatenate((wtab, gpar), axis=1)
+            if dj is not None:
+                extra = np.zeros((nsteps, 2 * mg)) if ab is None else \
+                    np.concatenate((ab[:, g0:g0 + mg, 0], ab[:, g0:g0 + mg, 1]),
+                                   axis=1)
+                tab = np.concatenate((tab, extra), axis=1)
+            gp.ncoef = tab.shape[1]
+            if nsteps:
+                d_tab = _lib.to_device(tab.astype(cdt))
+                keep.append(d_tab)
+                gp.coef = d_tab.data_ptr()
+                if pois is not None:
+                    d_pois = _lib.to_device(
+                        np.ascontiguousarray(pois[:, g0:g0 + mg, :]))
+                    keep.append(d_pois)
+                    gp.pois = d_pois.data_ptr()
+            if x0.shape == (M,):
+                for v in range(mg):
+                    gp.x0[v] = float(x0[g0 + v])
+            if len(groups) > 1:
+                gp.nvars, gp.comp0, gp.nvars_total = mg, g0, M
+                gp.nout, gp.out0, gp.exp_mask = self._group_output(
+                    g0, mg, nout, out_mode, out_group)
+                gp.out_stride = nout
+            probs.append(gp)
+
         def post():
             if self.getinfo:
                 self.info['computed_steps'] += nsteps
                 self.info['stored_steps'] += int((rows >= 0).sum())
                 self._pass_info(info_paths, info_rows, tt_pass, row_of,
                                 dn_host, rows)
-        return _Pass(prob, keep, post, nsteps)
+        return _Pass(probs, keep, post, nsteps)
 
     def _pass_info(self, info_paths, info_rows, tt_pass, row_of, dn_host, rows):
         pass
 
+    # component groups ----------------------------------------------------
+
+    def _groups(self, has_corr, out_mode, out_group):
+        """[(first component, size)] of the launches of a pass: the whole
+        working state when it has at most MAX_VARS components, else groups
+        of whole coupled blocks.  Components are coupled along the last axis
+        of wshape by a correlated dw, Heston's (log x, y) pairs and
+        Hull-White's summed factors; otherwise each is on its own."""
+        M = self._nvars()
+        if M <= _lib.MAX_VARS:
+            return [(0, M)]
+        coupled = has_corr or self.model == _lib.MODEL_HESTON or \
+            out_mode == _lib.OUT_SUM
+        unit = self.wshape[-1] if coupled else 1
+        if unit > _lib.MAX_VARS:
+            raise NotImplementedError(
+                'the GPU integrator steps at most {} coupled working-state '
+                'components per path; {} with vshape {} couples {}'.format(
+                    _lib.MAX_VARS, type(self).__name__, self.vshape, unit))
+        mg = (_lib.MAX_VARS // unit) * unit
+        return [(g0, min(mg, M - g0)) for g0 in range(0, M, mg)]
+
+    def _group_params(self, params, g0, mg):
+        """Columns of the parameter table for components g0 .. g0+mg-1: each
+        block of M (or of N log-x values for Heston) sliced."""
+        if self._nvars() <= _lib.MAX_VARS and g0 == 0:
+            return params
+        M = self._nvars()
+        if self.model == _lib.MODEL_HESTON:
+            B, lo, hi = M // 2, g0 // 2, (g0 + mg) // 2
+        else:
+            B, lo, hi = M, g0, g0 + mg
+        return np.concatenate([params[:, b * B + lo:b * B + hi]
+                               for b in range(params.shape[1] // B)], axis=1)
+
+    def _group_output(self, g0, mg, nout, out_mode, out_group):
+        """(nout, out0, exp_mask) of the group's stored values."""
+        M = self._nvars()
+        if out_mode == _lib.OUT_SUM:
+            return mg // out_group, g0 // out_group, 0
+        if self.model == _lib.MODEL_HESTON:
+            if nout == M // 2:
+                return mg // 2, g0 // 2, (1 << (mg // 2)) - 1
+            d = self._width()
+            return mg, g0, sum(1 << o for o in range(mg) if o % (2 * d) < d)
+        return mg, g0, ((1 << mg) - 1) if self.log else 0
+
 
 class _Pass:
-    """One integration pass ready to launch: the problem struct and the
-    device buffers it points to (kept alive with it)."""
+    """One integration pass ready to launch: the problem structs (one per
+    component group) and the device buffers they point to (kept alive with
+    it)."""
 
-    def __init__(self, prob, keep, post, nsteps):
-        self.prob, self.keep, self.post, self.nsteps = prob, keep, post, nsteps
+    def __init__(self, probs, keep, post, nsteps):
+        self.probs, self.keep, self.post, self.nsteps = probs, keep, post, nsteps
+
+    @property
+    def prob(self):
+        return self.probs[0]
+
+    def advance_epochs(self):
+        for p in self.probs:
+            p.epoch_w = (p.epoch_w + 1) & 0xFFFFFFFF
+            p.epoch_j = (p.epoch_j + 1) & 0xFFFFFFFF
+
+    def set_epoch_w(self, epoch):
+        for p in self.probs:
+            p.epoch_w = epoch
 
     def launch(self, stream=None):
-        _lib.check(_lib.lib().sdegpu_euler(
-            self.prob, stream if stream is not None else _lib.stream()))
+        st = stream if stream is not None else _lib.stream()
+        for p in self.probs:
+            _lib.check(_lib.lib().sdegpu_euler(p, st))
 
 
 class PreparedRun:
@@ -470,8 +554,7 @@ class PreparedRun:
     def launch(self, fresh=False):
         for p in self.passes:
             if fresh:
-                p.prob.epoch_w = (p.prob.epoch_w + 1) & 0xFFFFFFFF
-                p.prob.epoch_j = (p.prob.epoch_j + 1) & 0xFFFFFFFF
+                p.advance_epochs()
             p.launch()
 
     def result(self):
@@ -751,7 +834,7 @@ class ornstein_uhlenbeck_process(_gpu_sde):
         # correlated draw is the diffusion increment itself
         if not has_corr:
             return wtab
-        M = self._nvars()
+        M = params.shape[1] // 3
         wtab = wtab.copy()
         sigma = params[:, 2 * M:3 * M]
         sq = wtab[:, 1]

@@ -114,7 +114,17 @@ def test_entry_point_validation_maps_to_exceptions():
     pr.nout, pr.exp_mask = 8, 0x0F
     with pytest.raises(ValueError, match='0x33'):   # log x of groups [x x y y]
         _lib.check(lib.sdegpu_euler(pr, None))
-    pr.heston_width = 0
+    pr.heston_width, pr.exp_mask = 2, 0x33
+    pr.comp0, pr.nvars_total = 2, 16
+    with pytest.raises(ValueError):        # a group must start on a Heston block
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.comp0 = 12
+    with pytest.raises(ValueError):        # components 12..19 outside 16
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.comp0, pr.out0, pr.out_stride = 8, 10, 16
+    with pytest.raises(ValueError):        # values 10..17 outside rows of 16
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.heston_width = pr.comp0 = pr.nvars_total = pr.out0 = pr.out_stride = 0
     pr.model, pr.nvars, pr.paths = _lib.MODEL_OU, 1, 0
     with pytest.raises(ValueError):
         _lib.check(lib.sdegpu_euler(pr, None))
