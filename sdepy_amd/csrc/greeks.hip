@@ -304,6 +304,8 @@ struct LogArgs {
   uint32_t kw0, kw1, ew, kj0, kj1, ej;
 };
 
+constexpr int kLogEvents = 4;  // jump steps per path whose sizes are drawn after the loop
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
                                                           const T* __restrict__ st,
@@ -321,6 +323,32 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
     T X = (T)a.lx0;
     T sdw = (T)0, szeta = (T)0;
     int njump = 0;
+    // Sum of the cnt jump sizes of step s and of their normals: the jump
+    // normals of (step s, jump q) are lane q % 4 of block (s, path,
+    // jump stream | (q / 4) << 8), as in the Euler kernel.
+    auto jumps = [&](int s, int cnt, T& dj, T& zsum) {
+      T zz[4];
+      for (int q = 0; q < cnt; ++q) {
+        if ((q & 3) == 0)
+          normals4<T>((uint32_t)s, path, kStreamJump | ((uint32_t)(q >> 2) << 8), a.ej, a.kj0,
+                      a.kj1, zz);
+        const int l = q & 3;
+        const T zq = l == 0 ? zz[0] : l == 1 ? zz[1] : l == 2 ? zz[2] : zz[3];
+        dj = dj + (ja + jb * zq);
+        zsum = zsum + zq;
+      }
+    };
+    // The log-space recursion is linear with state-independent
+    // coefficients and only x_T is needed, so the jump sizes can be added
+    // after the step loop: up to kLogEvents jump steps per path are recorded
+    // as (s << 8 | count) and drawn afterwards, when every lane of the wave
+    // draws its own (at lambda ds = 1/252 a wave otherwise draws a Philox
+    // block at about one step in five for a single jumping lane).  Further
+    // jump steps, counts above 255 and steps beyond 2^24 are drawn in place.
+    uint32_t ev[kLogEvents];
+#pragma unroll
+    for (int k = 0; k < kLogEvents; ++k) ev[k] = 0u;
+    int nev = 0;
     for (int s0 = 0; s0 < n; s0 += 4) {
       T z[4];
       normals4<T>((uint32_t)(s0 >> 2), path, kStreamNormal, a.ew, a.kw0, a.kw1, z);
@@ -341,23 +369,32 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
           T dj = (T)0;
           if (a.pois) {
             const int cnt = poisson_inv(u01_f64(up[j]), pois[2 * s], pois[2 * s + 1], 4096);
-            T zsum = (T)0;
-            T zz[4];
-            for (int q = 0; q < cnt; ++q) {
-              if ((q & 3) == 0)
-                normals4<T>((uint32_t)s, path, kStreamJump | ((uint32_t)(q >> 2) << 8), a.ej,
-                            a.kj0, a.kj1, zz);
-              const int l = q & 3;
-              const T zq = l == 0 ? zz[0] : l == 1 ? zz[1] : l == 2 ? zz[2] : zz[3];
-              dj = dj + (ja + jb * zq);
-              zsum = zsum + zq;
+            if (cnt > 0) {
+              njump += cnt;
+              if (nev < kLogEvents && cnt < 256 && s < (1 << 24)) {
+                const uint32_t e = ((uint32_t)s << 8) | (uint32_t)cnt;
+#pragma unroll
+                for (int k = 0; k < kLogEvents; ++k) ev[k] = nev == k ? e : ev[k];
+                ++nev;
+              } else {
+                T zsum = (T)0;
+                jumps(s, cnt, dj, zsum);
+                szeta = szeta + zsum;
+              }
             }
-            njump += cnt;
-            szeta = szeta + zsum;
           }
           X = X + ((adt * ds + sig * dw) + dj);
           sdw = sdw + dw;
         }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kLogEvents; ++k) {
+      if (k < nev) {
+        T dj = (T)0, zsum = (T)0;
+        jumps((int)(ev[k] >> 8), (int)(ev[k] & 255u), dj, zsum);
+        X = X + dj;
+        szeta = szeta + zsum;
       }
     }
     const double S = exp((double)X);
