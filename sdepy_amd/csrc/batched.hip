@@ -25,16 +25,6 @@ namespace sdegpu {
 
 constexpr int kTile = 64;
 
-struct BatchArgs {
-  const sdegpu_batch_instance* inst;
-  const int32_t* work;   // [n_items][2] = (instance, tile), in claim order
-  const int64_t* slot0;  // [n_inst] first partial slot of each instance
-  const float* coef;     // concatenated step tables
-  double* part;          // [n_slots][2]
-  unsigned int* counter; // work counter (zeroed by k_batch_reset)
-  int32_t n_items;
-};
-
 template <int MODEL, int M>
 __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const float* __restrict__ coef,
                                           uint32_t path, bool live) {
@@ -119,31 +109,39 @@ __device__ __forceinline__ double wave_sum64(double v) {
 // block is avoided on purpose: the compiler may thread such lane-0 blocks
 // around the loop back-edge into a separate path, leaving the other lanes to
 // read the index of an inactive lane.
-__global__ __launch_bounds__(kBlock) void k_batch_persistent(const BatchArgs a) {
+//
+// The read-only arrays are separate restrict-qualified arguments: their
+// addresses are wave-uniform (the claimed index is in a scalar register), so
+// the instance descriptor and every step-table row are read with scalar
+// loads, not per-lane vector loads.
+__global__ __launch_bounds__(kBlock) void k_batch_persistent(
+    const sdegpu_batch_instance* __restrict__ insts, const int32_t* __restrict__ work,
+    const int64_t* __restrict__ slot0, const float* __restrict__ coef,
+    double* __restrict__ part, unsigned int* __restrict__ counter, int32_t n_items) {
   const int lane = threadIdx.x & 63;
   for (;;) {
-    const unsigned int old = atomicAdd(a.counter, lane == 0 ? 1u : 0u);
+    const unsigned int old = atomicAdd(counter, lane == 0 ? 1u : 0u);
     const int w = (int)__builtin_amdgcn_readfirstlane(old);
-    if (w >= a.n_items) break;
-    const int inst = a.work[2 * w], tile = a.work[2 * w + 1];
-    const sdegpu_batch_instance& d = a.inst[inst];
+    if (w >= n_items) break;
+    const int inst = work[2 * w], tile = work[2 * w + 1];
+    const sdegpu_batch_instance& d = insts[inst];
     const int64_t pidx = (int64_t)tile * kTile + lane;
     const bool live = pidx < d.paths;
     const uint32_t path = (uint32_t)pidx;
     float v;
     switch (d.model * 16 + d.nvars) {
-      case SDEGPU_MODEL_LOGNORM * 16 + 1: v = run_tile<SDEGPU_MODEL_LOGNORM, 1>(d, a.coef, path, live); break;
-      case SDEGPU_MODEL_WIENER * 16 + 1: v = run_tile<SDEGPU_MODEL_WIENER, 1>(d, a.coef, path, live); break;
-      case SDEGPU_MODEL_OU * 16 + 1: v = run_tile<SDEGPU_MODEL_OU, 1>(d, a.coef, path, live); break;
-      case SDEGPU_MODEL_OU * 16 + 2: v = run_tile<SDEGPU_MODEL_OU, 2>(d, a.coef, path, live); break;
-      case SDEGPU_MODEL_OU * 16 + 3: v = run_tile<SDEGPU_MODEL_OU, 3>(d, a.coef, path, live); break;
-      default: v = run_tile<SDEGPU_MODEL_OU, 4>(d, a.coef, path, live); break;
+      case SDEGPU_MODEL_LOGNORM * 16 + 1: v = run_tile<SDEGPU_MODEL_LOGNORM, 1>(d, coef, path, live); break;
+      case SDEGPU_MODEL_WIENER * 16 + 1: v = run_tile<SDEGPU_MODEL_WIENER, 1>(d, coef, path, live); break;
+      case SDEGPU_MODEL_OU * 16 + 1: v = run_tile<SDEGPU_MODEL_OU, 1>(d, coef, path, live); break;
+      case SDEGPU_MODEL_OU * 16 + 2: v = run_tile<SDEGPU_MODEL_OU, 2>(d, coef, path, live); break;
+      case SDEGPU_MODEL_OU * 16 + 3: v = run_tile<SDEGPU_MODEL_OU, 3>(d, coef, path, live); break;
+      default: v = run_tile<SDEGPU_MODEL_OU, 4>(d, coef, path, live); break;
     }
     const double dv = live ? (double)v - d.shift : 0.0;
     const double s1 = wave_sum64(dv), s2 = wave_sum64(dv * dv);
-    const int64_t slot = a.slot0[inst] + tile;
-    a.part[2 * slot] = s1;
-    a.part[2 * slot + 1] = s2;
+    const int64_t slot = slot0[inst] + tile;
+    part[2 * slot] = s1;
+    part[2 * slot + 1] = s2;
   }
 }
 
@@ -200,15 +198,8 @@ static int check_batch(const sdegpu_batch* b) {
 static int launch_batch(const sdegpu_batch* b, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->counter);
   if (int e = launch_status("k_batch_reset")) return e;
-  BatchArgs a;
-  a.inst = b->inst;
-  a.work = b->work;
-  a.slot0 = b->slot0;
-  a.coef = b->coef;
-  a.part = b->part;
-  a.counter = b->counter;
-  a.n_items = b->n_items;
-  hipLaunchKernelGGL(k_batch_persistent, dim3(b->blocks), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL(k_batch_persistent, dim3(b->blocks), dim3(kBlock), 0, st, b->inst, b->work,
+                     b->slot0, b->coef, b->part, b->counter, b->n_items);
   if (int e = launch_status("k_batch_persistent")) return e;
   hipLaunchKernelGGL(k_batch_finish, dim3(b->n_inst), dim3(kBlock), 0, st, b->inst, b->slot0,
                      (const double*)b->part, b->n_inst, b->mean, b->m2);
