@@ -111,7 +111,6 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
 
   EulerArgs a = {};
   a.coef = pr->coef;
-  a.pois = pr->pois;
   a.store_row = pr->store_row;
   a.x0_paths = pr->x0_paths;
   a.out = pr->out;
@@ -156,13 +155,13 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
         plain ? euler_plain_f32(pr->model, m, hd) : pick_kernel<true, float>(pr->model, m, hd);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
-                       (float*)pr->out);
+                       pr->pois, (float*)pr->out);
   } else {
     EulerKernel<double> k =
         plain ? euler_plain_f64(pr->model, m, hd) : pick_kernel<true, double>(pr->model, m, hd);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
-                       (double*)pr->out);
+                       pr->pois, (double*)pr->out);
   }
   return launch_status("k_euler");
 }

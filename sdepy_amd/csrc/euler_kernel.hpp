@@ -31,7 +31,6 @@ namespace sdegpu {
 
 struct EulerArgs {
   const void* coef;
-  const double* pois;
   const int32_t* store_row;
   const void* x0_paths;
   void* out;
@@ -146,6 +145,7 @@ __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs
 template <int MODEL, typename T, int M, int D, bool GEN>
 __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __restrict__ coef,
                                                   const int32_t* __restrict__ store_row,
+                                                  const double* __restrict__ pois,
                                                   T* __restrict__ out) {
   constexpr int CH = Chunking<M>::CH;
   constexpr int NZ = Chunking<M>::NZ;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 #pragma unroll
           for (int v = 0; v < M; ++v) dj[v] = rp[v * P];
         } else {
-          const double* pq = a.pois + (int64_t)s * M * 2;
+          const double* pq = pois + (int64_t)s * M * 2;
 #pragma unroll
           for (int v = 0; v < M; ++v) {
             const int n =
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 }
 
 template <typename T>
-using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, T*);
+using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, const double*, T*);
 
 template <bool GEN, int MODEL, typename T>
 EulerKernel<T> pick_m(int m) {
