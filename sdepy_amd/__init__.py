@@ -19,7 +19,7 @@ from .infrastructure import (process, source, wiener_source, poisson_source,
                              cpoisson_source, odd_wiener_source,
                              even_poisson_source, even_cpoisson_source,
                              norm_rv, uniform_rv, exp_rv, double_exp_rv,
-                             rvmap, path_moments)
+                             rvmap, path_moments, piecewise)
 from .montecarlo import montecarlo
 from .integration import (wiener_process, lognorm_process,
                           ornstein_uhlenbeck_process, hull_white_process,

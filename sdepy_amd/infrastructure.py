@@ -1160,6 +1160,40 @@ def path_cdf(y, x, device_tensor=None):
     return out
 
 
+def piecewise(t=0., *, x=None, v=None, dtype=None, mode='mid'):
+    """A process that interpolates to a piecewise constant function of time
+    (sdepy piecewise, infrastructure.py:1216-1281), for use as a time
+    dependent parameter: value x[i] (or v[i]) on the segment that has t[i] as
+    its midpoint ('mid'), start ('forward') or end ('backward').  The forward
+    and backward forms repeat every time point, so that 'nearest'
+    interpolation jumps there (at the point itself the earlier segment's
+    value is returned); the returned timeline and values are an
+    implementation detail, as in sdepy."""
+    p = process(t, x=x, v=v, dtype=dtype)
+    t, xv = p.t, p.view(np.ndarray)
+    if mode == 'mid':
+        s, y = t, xv
+    elif mode in ('forward', 'backward'):
+        s = np.repeat(t, 2)
+        y = np.empty((2 * xv.shape[0],) + xv.shape[1:], dtype=xv.dtype)
+        if mode == 'forward':
+            # [t_i, t_{i+1}) -> x_i: each time point ends the previous
+            # segment and starts its own
+            y[0::2] = np.concatenate((xv[:1], xv[:-1]))
+            y[1::2] = xv
+        else:
+            # (t_{i-1}, t_i] -> x_i
+            y[0::2] = xv
+            y[1::2] = np.concatenate((xv[1:], xv[-1:]))
+    else:
+        raise ValueError(
+            "mode should be one of 'mid', 'forward', 'backward', "
+            'but {} was given'.format(mode))
+    out = process(s, x=y, dtype=dtype)
+    out.interp_kind = 'nearest'
+    return out
+
+
 def path_moments(x, device_tensor=None, paths=None):
     """FP64 (n, mean, m2) over the last axis of ``x``, reduced on the GPU
     (``sdegpu_path_moments``).  ``device_tensor``, when given, is a device

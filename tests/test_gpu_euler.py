@@ -367,6 +367,24 @@ def test_backward_integration_i0_and_steps(gpu):
     assert info['stored_steps'] == ref_info['stored_steps'] == 5
 
 
+def test_piecewise_parameters_match_oracle(gpu):
+    """piecewise processes as time-dependent parameters (evaluated at each
+    step start, integration.py:1218-1221), per component, FP64 vs the
+    oracle driven by the same parameter objects."""
+    P, n = 800, 37
+    tt = np.linspace(0., 2., 5)
+    mu = sa.piecewise((0., .7, 1.4), v=np.array([[.1, -.2], [.3, .0], [-.1, .2]]))
+    sigma = sa.piecewise((0., 1.), v=np.array([[.2, .3], [.4, .1]]), mode='forward')
+    p = sa.lognorm_process(paths=P, vshape=2, dtype=np.float64, steps=n, x0=1.,
+                           mu=mu, sigma=sigma, rng=np.random.default_rng(61))
+    key = p.sources['dw'].key
+    x = p(tt)
+    ref = ora.integrate('lognorm', tt, paths=P, vshape=(2,), steps=n,
+                        sources={'dw': ora.PhiloxWiener(key, 0, P, (2,))},
+                        x0=1., mu=mu, sigma=sigma)
+    assert _rel(x, ref) < RTOL64
+
+
 def test_staged_transfers_round_trip(gpu):
     """Host <-> device copies through the pinned staging buffers return the
     same bytes at sizes around the direct-copy threshold and the chunk size,

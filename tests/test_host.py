@@ -331,3 +331,39 @@ def test_bench_refuses_more_than_one_gpu():
     res = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2'],
                          capture_output=True, text=True)
     assert res.returncode == 2 and 'one GPU' in res.stderr
+
+
+@pytest.mark.parametrize('mode', [None, 'mid', 'forward', 'backward'])
+@pytest.mark.parametrize('dtype,paths,vshape,shift', [
+    (float, None, (3,), 0), (float, 5, (2, 3), -2), (np.float32, None, (), 0),
+    (np.float16, 5, (), 0), (float, None, (2, 3), -4)])
+def test_piecewise_semantics(mode, dtype, paths, vshape, shift):
+    """piecewise (infrastructure.py:1216-1281): 'nearest' interpolation of
+    constant segments centred on, starting at or ending at the given times,
+    the earlier segment winning at a shared point; the semantics of the
+    reference's test_process.py:776-833."""
+    g = np.random.default_rng(7)
+    t = np.array((1, 2, 3), dtype=dtype) + shift
+    if paths is None:
+        v = (1 + g.random((3,) + vshape)).astype(dtype)
+        x, kw, npaths = v[..., None], dict(v=v), 1
+    else:
+        x = (1 + g.random((3,) + vshape + (paths,))).astype(dtype)
+        kw, npaths = dict(x=x), paths
+    p = sa.piecewise(t, **kw) if mode is None else sa.piecewise(t, mode=mode, **kw)
+    assert isinstance(p * p + 2, sa.process)
+    assert p.interp_kind == 'nearest'
+    assert p.paths == npaths and p.vshape == vshape
+    assert p(0).dtype == dtype
+    if mode in (None, 'mid'):
+        s, idx = (.5, 1, 1.49, 1.51, 2.49, 2.51, 3, 3.5), (0, 0, 0, 1, 1, 2, 2, 2)
+    elif mode == 'forward':
+        s, idx = (.5, 1, 1.5, 2, 2.01, 2.5, 3, 3.01, 3.5), (0, 0, 0, 0, 1, 1, 1, 2, 2)
+    else:
+        s, idx = (.5, 1, 1.01, 1.5, 2, 2.01, 2.5, 3, 3.5), (0, 0, 1, 1, 1, 2, 2, 2, 2)
+    got = p(np.array(s) + shift)
+    assert np.array_equal(got, np.stack([x[i] for i in idx]))
+    q = sa.piecewise(t=0., x=x[:1])
+    assert np.array_equal(q((-3, 0, 3)), np.stack((x[0], x[0], x[0])))
+    with pytest.raises(ValueError):
+        sa.piecewise((1, 2), v=(10, 20), mode='zzz')
