@@ -7,6 +7,9 @@ Greeks d/d(x0, r, k, theta, xi, rho, v0) with FP64 accumulation.
 
 Printed as one JSON line:
   value       path-steps/s of the FP32 forward+adjoint (4M*365 per step)
+  tangent     the same set by forward-mode derivatives carried along the one
+              forward pass (no checkpoints, no recomputation), its time and
+              its difference from the adjoint in SE units
   fp64_fd     the same set (price + 7 Greeks) from 15 FP64 forward runs with
               central finite differences on common random numbers, its
               throughput and the speedup of the adjoint over it
@@ -237,8 +240,9 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=10)
-    ap.add_argument('--legs', default='adjoint,fd,ou,merton,gbm,batched,cpu',
-                    help='comma list of: adjoint, fd, ou, merton, gbm, batched, '
+    ap.add_argument('--legs', default='adjoint,tangent,fd,ou,merton,gbm,batched,cpu',
+                    help='comma list of: adjoint, tangent (forward-mode Greeks '
+                         'on the same workload), fd, ou, merton, gbm, batched, '
                          'cpu, copyref (a 1 GiB device copy used to calibrate '
                          'rocprof byte counters), ou64 (FP64 twin of the OU '
                          'leg for its error figure; on by default with ou), '
@@ -320,6 +324,25 @@ def main():
             'achieved_per_s': rate, 'peak_per_s': VALU_ISSUE_PEAK_PER_S,
             'frac': rate / VALU_ISSUE_PEAK_PER_S}
 
+    # ---- the same set by forward-mode derivatives (method='tangent') ----
+    if 'tangent' in legs:
+        trun = G.HestonGreeks(paths=PATHS, steps=NSTEPS, strike=STRIKE,
+                              maturity=MATURITY, dtype=np.float32,
+                              key=(0x5eed, 0xc0ffee), epoch=0, method='tangent',
+                              **HPRM)
+        t_ms, _ = timed(torch, trun.launch, args.steps, args.warmup)
+        tres = trun.result()
+        out['tangent'] = {
+            'kernel': 'k_heston_tangent<float>', 'ms': t_ms,
+            'path_steps_per_s': trun.path_steps / (t_ms * 1e-3),
+            'greeks_per_s': 7 / (t_ms * 1e-3),
+            'speedup_over_adjoint': ms / t_ms,
+            'price': tres.price, 'greeks': tres.greeks,
+            'minus_adjoint_over_se': {
+                k: (tres.greeks[k] - res.greeks[k]) / res.greeks_se[k]
+                for k in G.HESTON_GREEKS}}
+        del trun
+
     # ---- FP64 forward + CRN finite differences, same paths ----
     if 'fd' in legs:
         fd = G.HestonFD(paths=PATHS, steps=NSTEPS, strike=STRIKE,
@@ -331,6 +354,13 @@ def main():
         acc = {'price': abs(res.price - price64) / res.price_se}
         for k in G.HESTON_GREEKS:
             acc[k] = abs(res.greeks[k] - fdg[k]) / res.greeks_se[k]
+        if 'tangent' in out:
+            out['tangent']['accuracy_max_se'] = max(
+                [abs(out['tangent']['price'] - price64) / res.price_se] +
+                [abs(out['tangent']['greeks'][k] - fdg[k]) / res.greeks_se[k]
+                 for k in G.HESTON_GREEKS])
+            out['tangent']['speedup_over_fp64_fd'] = \
+                out['tangent']['path_steps_per_s'] / fd_value
         out['fp64_fd'] = {'value': fd_value, 'unit': 'path-steps/s',
                           'ms_per_set': fd_ms, 'price': price64,
                           'greeks': fdg, 'speedup_of_adjoint': value / fd_value}

@@ -310,6 +310,10 @@ typedef struct sdegpu_heston_greeks {
   double strike, maturity;
   uint32_t key[2]; /* Philox key of the dw source */
   uint32_t epoch;
+  int32_t method;  /* 0: reverse-mode adjoint (variance checkpoints every 8 steps,
+                      recomputed segment by segment); 1: forward mode, the
+                      derivatives carried along the one forward pass (no
+                      checkpoints); the same estimator either way */
 } sdegpu_heston_greeks;
 
 size_t sdegpu_heston_greeks_workspace(const sdegpu_heston_greeks* g);

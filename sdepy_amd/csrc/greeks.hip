@@ -84,7 +84,7 @@ __device__ __forceinline__ double trcp<double>(double v) {
 // from the caller's (ds, sqrt|ds|) table, in FP64 and then rounded to T:
 //   [0] ds            [1] sq = sqrt|ds|   [2] r ds          [3] sigma^2/2 ds
 //   [4] k ds          [5] k theta ds      [6] sigma sq      [7] xi sq
-//   [8] xi sq / 2     [9] sigma sq / 2
+//   [8] xi sq / 2     [9] sigma sq / 2    [10] 1 - k ds
 // With them the forward step is
 //   X' = X + (sigma sq sv z0 + (r ds - sigma^2/2 ds y+))
 //   V' = V + (xi sq sv w    + (k theta ds - k ds y+)),   w = rho z0 + rhoc z1
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_heston_table(const T* __restrict__ s
     row[7] = (T)(xi * sq);
     row[8] = (T)(0.5 * xi * sq);
     row[9] = (T)(0.5 * sig * sq);
-    row[10] = (T)0;
+    row[10] = (T)(1.0 - k * ds);
     row[11] = (T)0;
     part += ds;
   }
@@ -276,6 +276,98 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoi
                                  (double)Sx,
                                  a.xi / (a.rhoc * a.rhoc) * (double)Sr,
                                  (double)Vbar};
+    acc[0] = pay;
+    acc[1] = pay * pay;
+#pragma unroll
+    for (int i = 0; i < kHestonNG; ++i) {
+      acc[2 + 2 * i] = g[i];
+      acc[3 + 2 * i] = g[i] * g[i];
+    }
+  }
+  block_reduce_store<kHestonNS>(acc, a.part);
+}
+
+// Forward-mode (tangent) differentiation of the same scheme.  One forward
+// pass carries, beside (X, V), the derivatives of (X, V) along five
+// directions, so it needs no checkpoints and no recomputation; with this few
+// parameters it issues fewer instructions than the reverse sweep.  Where
+// V_n > 0 (d y+/dV = 1; else a = 1, b = 0):
+//   dV' = a dV + c,  dX' = dX + b dV,
+//   a = 1 - k ds + xi sq w / (2 sv),  b = -sigma^2/2 ds + sigma sq z0 / (2 sv),
+// with the explicit partials c of the directions
+//   1: ds               k: theta * 1 - 2,  theta: k * 1
+//   2: y+ ds
+//   x: sv sq w          xi
+//   r: sv sq (z0 - rho w)   rho: times xi / rhoc^2
+//   v: none, dV_0 = 1   v0
+// and each Greek xbar dX_T along its direction: the same numbers as the
+// adjoint's sums (S1 = dX_1, S2 = dX_2, ...), in another order of operations.
+template <typename T>
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_tangent(
+    const HestonArgs a, const T* __restrict__ tb, const double* __restrict__ sum_ds) {
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = gid < a.paths;
+  const uint32_t path = (uint32_t)gid;
+  const int n = a.nsteps;
+  const T rho = (T)a.rho, rhoc = (T)a.rhoc;
+
+  double acc[kHestonNS];
+#pragma unroll
+  for (int i = 0; i < kHestonNS; ++i) acc[i] = 0.0;
+
+  if (live) {
+    T X = (T)a.lx0, V = (T)a.y0;
+    T x1 = (T)0, v1 = (T)0, x2 = (T)0, v2 = (T)0, xx = (T)0, vx = (T)0;
+    T xr = (T)0, vr = (T)0, xv = (T)0, vv = (T)1;
+    for (int s0 = 0; s0 < n; s0 += 2) {
+      T z[4];
+      normals4<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int s = s0 + j;
+        if (s < n) {
+          const T* r = tb + (int64_t)s * kHestonTab;
+          const T z0 = z[2 * j];
+          const T w = rho * z0 + rhoc * z[2 * j + 1];
+          const T vp = pos_part(V);
+          const T sv = tsqrt<T>(vp);
+          const T h = trcp<T>(sv);
+          const bool pos = sv > (T)0;
+          const T av = pos ? fma(w * r[8], h, r[10]) : (T)1;
+          const T bx = pos ? fma(z0 * r[9], h, -r[3]) : (T)0;
+          const T ssq = sv * r[1];
+          const T c2 = vp * r[0];
+          const T cx = ssq * w;
+          const T cr = ssq * fma(-rho, w, z0);
+          x1 = fma(v1, bx, x1);
+          v1 = fma(v1, av, r[0]);
+          x2 = fma(v2, bx, x2);
+          v2 = fma(v2, av, c2);
+          xx = fma(vx, bx, xx);
+          vx = fma(vx, av, cx);
+          xr = fma(vr, bx, xr);
+          vr = fma(vr, av, cr);
+          xv = fma(vv, bx, xv);
+          vv = vv * av;
+          X = X + fma(sv * r[6], z0, fma(-r[3], vp, r[2]));
+          V = V + fma(sv * r[7], w, fma(-r[4], vp, r[5]));
+        }
+      }
+    }
+    const double S = exp((double)X);
+    const bool call = a.payoff == 0;
+    const double intr = call ? S - a.strike : a.strike - S;
+    const bool itm = intr > 0.0;
+    const double pay = itm ? a.disc * intr : 0.0;
+    const double xbar = itm ? (call ? a.disc * S : -a.disc * S) : 0.0;
+    const double d1 = (double)x1;
+    const double g[kHestonNG] = {xbar / a.x0,
+                                 xbar * *sum_ds - a.maturity * pay,
+                                 xbar * (a.th * d1 - (double)x2),
+                                 xbar * a.k * d1,
+                                 xbar * (double)xx,
+                                 xbar * a.xi / (a.rhoc * a.rhoc) * (double)xr,
+                                 xbar * (double)xv};
     acc[0] = pay;
     acc[1] = pay * pay;
 #pragma unroll
@@ -479,6 +571,8 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
     return fail(SDEGPU_EINVAL, "heston_greeks: need x0 > 0 and |rho| < 1");
   if (g->payoff != 0 && g->payoff != 1)
     return fail(SDEGPU_EINVAL, "heston_greeks: payoff kind %d", g->payoff);
+  if (g->method != 0 && g->method != 1)
+    return fail(SDEGPU_EINVAL, "heston_greeks: method %d", g->method);
   const size_t need = sdegpu_heston_greeks_workspace(g);
   if (!ws || ws_bytes < need)
     return fail(SDEGPU_ENOMEM, "heston_greeks: workspace of %zu bytes, %zu needed", ws_bytes, need);
@@ -515,18 +609,26 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
     hipLaunchKernelGGL(k_heston_table<float>, dim3(1), dim3(kBlock), 0, st, (const float*)g->steps,
                        g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi, tab, sum_ds);
     if (int e = launch_status("k_heston_table")) return e;
-    hipLaunchKernelGGL((k_heston_adjoint<float, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st, a,
-                       (const float*)tab, (float*)a.ckpt, (const double*)sum_ds);
+    if (g->method == 1)
+      hipLaunchKernelGGL(k_heston_tangent<float>, dim3(nb), dim3(kBlock), 0, st, a,
+                         (const float*)tab, (const double*)sum_ds);
+    else
+      hipLaunchKernelGGL((k_heston_adjoint<float, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st, a,
+                         (const float*)tab, (float*)a.ckpt, (const double*)sum_ds);
   } else {
     double* tab = (double*)(base + lay.tab);
     hipLaunchKernelGGL(k_heston_table<double>, dim3(1), dim3(kBlock), 0, st,
                        (const double*)g->steps, g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi,
                        tab, sum_ds);
     if (int e = launch_status("k_heston_table")) return e;
-    hipLaunchKernelGGL((k_heston_adjoint<double, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st, a,
-                       (const double*)tab, (double*)a.ckpt, (const double*)sum_ds);
+    if (g->method == 1)
+      hipLaunchKernelGGL(k_heston_tangent<double>, dim3(nb), dim3(kBlock), 0, st, a,
+                         (const double*)tab, (const double*)sum_ds);
+    else
+      hipLaunchKernelGGL((k_heston_adjoint<double, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st,
+                         a, (const double*)tab, (double*)a.ckpt, (const double*)sum_ds);
   }
-  if (int e = launch_status("k_heston_adjoint")) return e;
+  if (int e = launch_status(g->method == 1 ? "k_heston_tangent" : "k_heston_adjoint")) return e;
   hipLaunchKernelGGL(k_sum_columns, dim3(kHestonNS), dim3(kBlock), 0, st, (const double*)a.part,
                      (int64_t)nb, kHestonNS, out);
   return launch_status("k_sum_columns");

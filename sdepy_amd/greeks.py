@@ -45,7 +45,8 @@ class HestonGreeksProblem(ctypes.Structure):
                 ('theta', ctypes.c_double), ('xi', ctypes.c_double),
                 ('rho', ctypes.c_double), ('y0', ctypes.c_double),
                 ('strike', ctypes.c_double), ('maturity', ctypes.c_double),
-                ('key', ctypes.c_uint32 * 2), ('epoch', ctypes.c_uint32)]
+                ('key', ctypes.c_uint32 * 2), ('epoch', ctypes.c_uint32),
+                ('method', ctypes.c_int32)]
 
 
 class LogGreeksProblem(ctypes.Structure):
@@ -116,13 +117,20 @@ class GreeksResult:
 
 
 class HestonGreeks:
-    """A prepared Heston adjoint run (step table, workspace and output on the
-    device); ``launch()`` issues the kernels, ``result()`` reads the sums."""
+    """A prepared Heston Greeks run (step table, workspace and output on the
+    device); ``launch()`` issues the kernels, ``result()`` reads the sums.
+    ``method='adjoint'`` sweeps the reverse-mode adjoint over recomputed
+    checkpoint segments; ``method='tangent'`` carries the five derivative
+    directions along the forward pass instead (fewer instructions for this
+    few parameters); both give the same pathwise estimator."""
 
     def __init__(self, *, x0=100., r=0.02, sigma=1., k=2., theta=.04, xi=.5,
                  rho=-.7, y0=.04, strike=100., maturity=1., steps=365,
                  paths=1_000_000, dtype=np.float32, payoff='call', key=None,
-                 epoch=0, rng=None):
+                 epoch=0, rng=None, method='adjoint'):
+        if method not in ('adjoint', 'tangent'):
+            raise ValueError("method should be 'adjoint' or 'tangent', not "
+                             '{!r}'.format(method))
         code, cdt, _ = _gpu_dtype(dtype)
         _lib.require_gpu()
         lib = _bind()
@@ -139,6 +147,7 @@ class HestonGreeks:
             if key is None else key
         pr.key[0], pr.key[1] = key
         pr.epoch = epoch
+        pr.method = 0 if method == 'adjoint' else 1
         self.key, self.prob = tuple(key), pr
         self.nbytes = lib.sdegpu_heston_greeks_workspace(pr)
         self.ws = _lib.workspace(self.nbytes)

@@ -40,6 +40,37 @@ def test_heston_adjoint_fp64_matches_oracle(gpu, K):
     assert np.isclose(res.price_se, pay.std(ddof=0) / np.sqrt(P - 1), rtol=1e-8)
 
 
+@pytest.mark.parametrize('K', [100., 0.])
+def test_heston_tangent_fp64_matches_oracle(gpu, K):
+    """Forward-mode kernel (method='tangent'): the same pathwise estimator as
+    the oracle's reverse-mode adjoint, FP64 to 1e-10 relative, and the same
+    per-path sums of squares (so the same standard errors)."""
+    P, n = 3000, 50
+    res = G.heston_greeks(paths=P, steps=n, strike=K, dtype=np.float64, key=KEY,
+                          epoch=2, method='tangent', **HPRM)
+    pay, g = og.heston_adjoint(KEY, 2, P, n, 1., strike=K, **HPRM)
+    got = np.concatenate(([res.price], [res.greeks[k] for k in og.HESTON_NAMES]))
+    ref = np.concatenate(([pay.mean()], g.mean(axis=1)))
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12 * max(1., abs(ref).max()))
+    se = np.array([res.greeks_se[k] for k in og.HESTON_NAMES])
+    assert np.allclose(se, g.std(axis=1) / np.sqrt(P - 1), rtol=1e-8)
+    with pytest.raises(ValueError):
+        G.heston_greeks(paths=P, steps=n, method='backward', **HPRM)
+
+
+def test_heston_tangent_fp32_equals_adjoint_fp32(gpu):
+    """FP32 forward mode and FP32 reverse mode on the same paths differ only
+    by FP32 rounding of two orders of operations: far below the SE."""
+    P, n = 200_000, 120
+    a = G.heston_greeks(paths=P, steps=n, strike=100., dtype=np.float32, key=KEY,
+                        epoch=4, **HPRM)
+    t = G.heston_greeks(paths=P, steps=n, strike=100., dtype=np.float32, key=KEY,
+                        epoch=4, method='tangent', **HPRM)
+    assert abs(t.price - a.price) < 1e-3 * a.price_se
+    for name in og.HESTON_NAMES:
+        assert abs(t.greeks[name] - a.greeks[name]) < .05 * a.greeks_se[name], name
+
+
 def test_heston_adjoint_fp32_vs_fp64_oracle(gpu):
     P, n = 8192, 100
     res = G.heston_greeks(paths=P, steps=n, strike=100., dtype=np.float32,
