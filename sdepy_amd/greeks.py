@@ -336,7 +336,9 @@ def _autograd_function():
         """price = HestonPrice.apply(params, config): params is a tensor of
         (x0, r, k, theta, xi, rho, y0); config a dict of the remaining
         HestonGreeks keywords (strike, maturity, steps, paths, dtype, key,
-        epoch, sigma, payoff).  The gradient is the pathwise adjoint."""
+        epoch, sigma, payoff, method).  The gradient is the pathwise
+        derivative, by the reverse-mode adjoint (or forward mode with
+        method='tangent')."""
 
         @staticmethod
         def forward(ctx, params, config):
