@@ -57,8 +57,9 @@ PATHS, NSTEPS, STRIKE, MATURITY = 4_000_000, 365, 100., 1.
 ADJ_FLOPS_PER_PATH_STEP = 74
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.
-# VALU issue ceiling: one wave64 vector instruction per clock per CU,
-# 256 CUs at 2.4 GHz
+# nominal VALU issue rate: one wave64 vector instruction per clock per CU,
+# 256 CUs at 2.4 GHz (FMA chains issue at 0.91-0.96 of it; simpler integer
+# and FP32 add/mul/mov instructions faster)
 VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
 
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per
