@@ -22,7 +22,8 @@ Printed as one JSON line:
               largest deviation from the FP64 run of the same paths
   merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
               mu, sigma, a, b), with the FP64 run of the same paths in SE units
-  summaries   moments, min/max, 100-bin histograms of every row of the OU
+  summaries   moments (pmean/pvar and montecarlo's 1..4), min/max, 100-bin
+              histograms of every row of the OU
               output, chf of its last row, tmax and FP64 tvar along the
               timeline and vmean across the 4 values, on the device (HBM
               read passes)
@@ -173,6 +174,15 @@ def _device_summaries(torch, run):
                                         _lib.ptr(mm), _lib.ptr(mws), mc_n, st))
     ms, _ = timed(torch, minmax, 5, 2)
     res['minmax'] = {'ms': ms, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9}
+    # montecarlo moments 1..4 about each row's mean (FP64 sums)
+    mom = _lib.empty((R, 5), np.float64)
+
+    def mc_moments():
+        _lib.check(lib.sdegpu_mc_moments(_lib.ptr(rows), _lib.F32, R, P, 0,
+                                         _lib.ptr(mean), _lib.ptr(mom), _lib.ptr(mws),
+                                         mc_n, st))
+    ms, _ = timed(torch, mc_moments, 5, 2)
+    res['mc_moments_1_4'] = {'ms': ms, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9}
     h = _lib.to_host(mm)
     # np.histogram's edges for 100 bins (a row of equal values widens to +-0.5)
     edges = np.stack([np.histogram_bin_edges(np.array([lo, hi]), bins=100)
