@@ -263,7 +263,7 @@ def main():
     ap.add_argument('--skip', default='',
                     help='comma list of legs to leave out (e.g. ou64,batched64 '
                          'under a profiler)')
-    ap.add_argument('--cpu-paths', type=int, default=30000)
+    ap.add_argument('--cpu-paths', type=int, default=64000)
     args = ap.parse_args()
     legs = set(args.legs.split(','))
     if 'ou' in legs:
