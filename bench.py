@@ -33,7 +33,8 @@ Printed as one JSON line:
               launch; per-instance FP32 moments vs FP64 runs of the same paths
   cpu_baseline the oracle's NumPy restatement of sdepy's loop (same
               operations as the reference, NumPy draws) on the host, one core,
-              price + 7 Greeks by 15 forward runs on a bounded path sample
+              price + 7 Greeks by 15 forward runs on a bounded path sample;
+              cpu_baseline_parallel the same path-sharded over up to 16 cores
 """
 import argparse
 import json
@@ -96,7 +97,7 @@ def timed(torch, fn, steps, warmup, stream=None):
     return e0.elapsed_time(e1) / steps, wall
 
 
-def cpu_baseline(paths, steps):
+def cpu_baseline(paths, steps, seed=1234):
     """The oracle (NumPy restatement of sdepy's Euler loop with NumPy
     multivariate-normal draws, as sdepy makes them) timed on one host core:
     15 forward runs give the price and 7 central-difference Greeks."""
@@ -111,7 +112,7 @@ def cpu_baseline(paths, steps):
             runs.append(d)
     t0 = time.perf_counter()
     for d in runs:
-        rng = np.random.RandomState(1234)   # common random numbers
+        rng = np.random.RandomState(seed)   # common random numbers
         src = ora.NumpyWiener(rng, paths, (2,), rho=d['rho'])
         x, _ = ora.integrate('full_heston', (0., MATURITY), paths=paths,
                              steps=steps + 1, sources={'dw': src},
@@ -138,6 +139,36 @@ def cpu_baseline_subprocess(paths, steps):
     res = subprocess.run([sys.executable, '-c', code], cwd=ROOT, env=env,
                          capture_output=True, text=True, check=True)
     return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline_parallel(paths, steps, workers):
+    """cpu_baseline in `workers` concurrent single-threaded child processes,
+    each on its own sample of `paths` paths (path-sharded, independent
+    seeds): the aggregate path-steps/s of the host's cores."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS='1', OPENBLAS_NUM_THREADS='1',
+               MKL_NUM_THREADS='1')
+    procs = []
+    t0 = time.perf_counter()
+    for i in range(workers):
+        code = ('import json, bench; print(json.dumps(bench.cpu_baseline({}, {}, {})))'
+                .format(int(paths), int(steps), 1234 + i))
+        procs.append(subprocess.Popen([sys.executable, '-c', code], cwd=ROOT, env=env,
+                                      stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                      text=True))
+    res = []
+    for pr in procs:
+        o, e = pr.communicate()
+        if pr.returncode != 0:
+            raise RuntimeError('CPU baseline worker failed: ' + e[-500:])
+        res.append(json.loads(o.strip().splitlines()[-1]))
+    wall = time.perf_counter() - t0
+    return {'value': workers * paths * steps / wall, 'unit': 'path-steps/s',
+            'cores': workers, 'kind': 'port',
+            'sample': ('{} processes x {} paths x {} steps, FP64, price + 7 Greeks '
+                       'by 15 CRN forward runs each, {:.1f} s wall including '
+                       'process start').format(workers, paths, steps, wall),
+            'per_process_value': [r['value'] for r in res]}
 
 
 def _device_summaries(torch, run):
@@ -526,6 +557,13 @@ def main():
     if 'cpu' in legs:
         out['cpu_baseline'] = cpu_baseline_subprocess(args.cpu_paths, NSTEPS)
         out['cpu_baseline']['gpu_over_cpu'] = value / out['cpu_baseline']['value']
+        # the same restatement path-sharded over the host's cores (at most 16,
+        # the CPU share of one GPU)
+        workers = max(1, min(16, os.cpu_count() or 1))
+        out['cpu_baseline_parallel'] = cpu_baseline_parallel(
+            args.cpu_paths // 2, NSTEPS, workers)
+        out['cpu_baseline_parallel']['gpu_over_cpu'] = \
+            value / out['cpu_baseline_parallel']['value']
     else:
         out['cpu_baseline'] = None
     print(json.dumps(out))
