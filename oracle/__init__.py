@@ -11,7 +11,10 @@ path of ``sdepy_amd`` can be checked against it on identical inputs:
   SDE.dZ / store / let / exit for the preset processes, fed either recorded
   increments (the reference's own replay hook: any source-protocol callable)
   or Philox draws;
-* ``oracle.moments``  -- process.pmean / pvar / psum / pstd semantics;
+* ``oracle.montecarlo`` -- sdepy's montecarlo accumulator (moments 1-4 and
+  histograms over cumulated samples, antithetic folding); the process
+  reductions pmean / pvar / psum / pstd are ndarray.mean / var / sum / std
+  over the paths axis, which the tests call on the oracle's arrays;
 * ``oracle.closed_forms`` -- the textbook closed forms used to pin the above
   (lognormal, discrete-Euler OU moments, Black-Scholes, Merton series, Heston
   log-mean);
