@@ -147,6 +147,13 @@ typedef struct sdegpu_euler_problem {
   int32_t nvars_total;
   int32_t out0;
   int32_t out_stride;
+  int32_t jump_defer;         /* JUMPDIFF, 1: the jump sizes drawn at a step that stores
+                                 no row are added to the state just before the next stored
+                                 row, (x + (drift + diffusion)) + dj instead of
+                                 x + (drift + diffusion + dj): the same draws and terms,
+                                 rounded in another order; faster when few steps store a
+                                 row.  0: every step sums its jumps with its increment, as
+                                 euler_next does (integration.py:707-723). */
 } sdegpu_euler_problem;
 
 /* Library and device queries. */

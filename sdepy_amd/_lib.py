@@ -52,7 +52,7 @@ class EulerProblem(ctypes.Structure):
         ('anti_w', c_int64), ('anti_j', c_int64),
         ('heston_width', c_int32),
         ('comp0', c_int32), ('nvars_total', c_int32),
-        ('out0', c_int32), ('out_stride', c_int32),
+        ('out0', c_int32), ('out_stride', c_int32), ('jump_defer', c_int32),
     ]
 
 

@@ -6,8 +6,8 @@ namespace sdegpu {
 
 // The instances without recorded increments and antithetic halves
 // (euler_plain.hip).
-EulerKernel<float> euler_plain_f32(int model, int m, int d);
-EulerKernel<double> euler_plain_f64(int model, int m, int d);
+EulerKernel<float> euler_plain_f32(int model, int m, int d, bool defer);
+EulerKernel<double> euler_plain_f64(int model, int m, int d, bool defer);
 
 // exp_mask the model implies for nout stored values (see OutRule and
 // store_state): every value for lognorm/jumpdiff, the log-x values for
@@ -148,17 +148,18 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   // no recorded increments and no antithetic halves: the plain instance
   const bool plain =
       !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 && !grouped;
+  const bool defer = jump && pr->jump_defer != 0;
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {
     EulerKernel<float> k =
-        plain ? euler_plain_f32(pr->model, m, hd) : pick_kernel<true, float>(pr->model, m, hd);
+        plain ? euler_plain_f32(pr->model, m, hd, defer) : pick_kernel<true, float>(pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
                        pr->pois, (float*)pr->out);
   } else {
     EulerKernel<double> k =
-        plain ? euler_plain_f64(pr->model, m, hd) : pick_kernel<true, double>(pr->model, m, hd);
+        plain ? euler_plain_f64(pr->model, m, hd, defer) : pick_kernel<true, double>(pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
                        pr->pois, (double*)pr->out);

@@ -142,7 +142,7 @@ __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs
   }
 }
 
-template <int MODEL, typename T, int M, int D, bool GEN>
+template <int MODEL, typename T, int M, int D, bool GEN, bool DEFER = false>
 __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __restrict__ coef,
                                                   const int32_t* __restrict__ store_row,
                                                   const double* __restrict__ pois,
@@ -150,6 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
   constexpr int CH = Chunking<M>::CH;
   constexpr int NZ = Chunking<M>::NZ;
   constexpr bool kJump = MODEL == SDEGPU_MODEL_JUMPDIFF;
+  constexpr bool kDefer = kJump && DEFER;
   constexpr bool kHeston = MODEL == SDEGPU_MODEL_HESTON;
   constexpr int N = kHeston ? M / 2 : M;
 
@@ -184,6 +185,52 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 #pragma unroll
   for (int v = 0; v < M; ++v) cnt[v] = 0;
   int interval_jumps = 0;
+
+  // jumpdiff with DEFER: x moves by (drift + diffusion) + dj with
+  // state-independent coefficients, so the jump sizes of a step only have to
+  // be in x when x is next stored.  On steps that are not stored, up to kJumpEvents jump draws
+  // per path are recorded as (step << 12 | backward << 11 | component << 8 |
+  // count) and their sizes drawn just before the next store, where all lanes
+  // of the wave draw together instead of the whole wave waiting on one
+  // jumping lane at each step.  Stored steps, draws beyond that, counts
+  // above 255 and steps beyond 2^20 are summed in place.  The sum is then
+  // (x + (drift + diffusion)) + dj instead of x + (drift + diffusion + dj):
+  // the same terms, rounded in another order.  The host asks for it when
+  // stored steps are sparse; with a store at most steps the bookkeeping
+  // costs more than it saves.
+  constexpr int kJumpEvents = 4;
+  uint32_t jev[kDefer ? kJumpEvents : 1];
+#pragma unroll
+  for (int k = 0; k < (kDefer ? kJumpEvents : 1); ++k) jev[k] = 0u;
+  int njev = 0;
+  // sum of the n jump sizes a + b z of component v at step s
+  auto jump_sum = [&](int s, int v, int n, T ja, T jb) -> T {
+    T acc = (T)0;
+    const uint32_t slot = (uint32_t)((int64_t)s * mt + c0 + v);
+    T zz[4];
+    for (int j = 0; j < n; ++j) {
+      if ((j & 3) == 0)
+        normals4<T>(slot, path_j, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0, a.kj1, zz);
+      const int lane = j & 3;
+      const T zj = lane == 0 ? zz[0] : lane == 1 ? zz[1] : lane == 2 ? zz[2] : zz[3];
+      acc = acc + (ja + jb * zj);
+    }
+    return acc;
+  };
+  auto flush_jumps = [&]() {
+    while (njev > 0) {
+      const uint32_t e = jev[0];
+      const int s = (int)(e >> 12), v = (int)((e >> 8) & 7u), n = (int)(e & 255u);
+      const T* pr = coef + (int64_t)s * a.ncoef + a.poff;
+      T acc = jump_sum(s, v, n, pr[2 * M + v], pr[3 * M + v]);
+      acc = (e >> 11) & 1u ? -acc : acc;
+#pragma unroll
+      for (int w = 0; w < M; ++w) x[w] = x[w] + (w == v ? acc : (T)0);
+#pragma unroll
+      for (int q = 0; q + 1 < kJumpEvents; ++q) jev[q] = jev[q + 1];
+      --njev;
+    }
+  };
 
   const bool replay = GEN && a.replay_dw != nullptr;
   const bool replay_j = GEN && a.replay_dj != nullptr;
@@ -331,25 +378,26 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
           for (int v = 0; v < M; ++v) dj[v] = rp[v * P];
         } else {
           const double* pq = pois + (int64_t)s * M * 2;
+          const bool neg = ds < (T)0;
+          // a stored step takes its own jumps in place (its state is written
+          // now); the steps in between defer theirs
+          const bool stored = store_row[s] >= 0;
 #pragma unroll
           for (int v = 0; v < M; ++v) {
             const int n =
                 poisson_inv(u01_f64(upois[k * M + v]), pq[2 * v], pq[2 * v + 1], kPoissonCap);
             T acc = (T)0;
             if (n > 0) {
-              const T ja = p[2 * M + v], jb = p[3 * M + v];
-              const uint32_t slot = (uint32_t)((int64_t)s * mt + c0 + v);
-              T zz[4];
-              for (int j = 0; j < n; ++j) {
-                if ((j & 3) == 0)
-                  normals4<T>(slot, path_j, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0,
-                              a.kj1, zz);
-                const int lane = j & 3;
-                const T zj = lane == 0 ? zz[0] : lane == 1 ? zz[1] : lane == 2 ? zz[2] : zz[3];
-                acc = acc + (ja + jb * zj);
+              if (kDefer && !stored && njev < kJumpEvents && n < 256 && s < (1 << 20)) {
+                const uint32_t e = ((uint32_t)s << 12) | ((neg ? 1u : 0u) << 11) |
+                                   ((uint32_t)v << 8) | (uint32_t)n;
+#pragma unroll
+                for (int q = 0; q < kJumpEvents; ++q) jev[q] = njev == q ? e : jev[q];
+                ++njev;
+              } else {
+                acc = jump_sum(s, v, n, p[2 * M + v], p[3 * M + v]);
               }
             }
-            const bool neg = ds < (T)0;
             dj[v] = neg ? -acc : acc;
             cnt[v] += neg ? -n : n;
             interval_jumps += neg ? -n : n;
@@ -362,6 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 
       const int row = store_row[s];
       if (row >= 0) {
+        if constexpr (kDefer) flush_jumps();
         store_state<MODEL, T, M, D, GEN>(out, a, row, x, gid);
         if constexpr (kJump) {
           if (a.info_rows && interval_jumps != 0)
@@ -384,17 +433,17 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 template <typename T>
 using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, const double*, T*);
 
-template <bool GEN, int MODEL, typename T>
+template <bool GEN, int MODEL, typename T, bool DEFER = false>
 EulerKernel<T> pick_m(int m) {
   switch (m) {
-    case 1: return k_euler<MODEL, T, 1, 1, GEN>;
-    case 2: return k_euler<MODEL, T, 2, 1, GEN>;
-    case 3: return k_euler<MODEL, T, 3, 1, GEN>;
-    case 4: return k_euler<MODEL, T, 4, 1, GEN>;
-    case 5: return k_euler<MODEL, T, 5, 1, GEN>;
-    case 6: return k_euler<MODEL, T, 6, 1, GEN>;
-    case 7: return k_euler<MODEL, T, 7, 1, GEN>;
-    case 8: return k_euler<MODEL, T, 8, 1, GEN>;
+    case 1: return k_euler<MODEL, T, 1, 1, GEN, DEFER>;
+    case 2: return k_euler<MODEL, T, 2, 1, GEN, DEFER>;
+    case 3: return k_euler<MODEL, T, 3, 1, GEN, DEFER>;
+    case 4: return k_euler<MODEL, T, 4, 1, GEN, DEFER>;
+    case 5: return k_euler<MODEL, T, 5, 1, GEN, DEFER>;
+    case 6: return k_euler<MODEL, T, 6, 1, GEN, DEFER>;
+    case 7: return k_euler<MODEL, T, 7, 1, GEN, DEFER>;
+    case 8: return k_euler<MODEL, T, 8, 1, GEN, DEFER>;
     default: return nullptr;
   }
 }
@@ -416,15 +465,18 @@ EulerKernel<T> pick_heston(int m, int d) {
   }
 }
 
+// defer: jumpdiff draws jump sizes at the next stored step (see k_euler)
 template <bool GEN, typename T>
-EulerKernel<T> pick_kernel(int model, int m, int d) {
+EulerKernel<T> pick_kernel(int model, int m, int d, bool defer) {
   switch (model) {
     case SDEGPU_MODEL_WIENER: return pick_m<GEN, SDEGPU_MODEL_WIENER, T>(m);
     case SDEGPU_MODEL_LOGNORM: return pick_m<GEN, SDEGPU_MODEL_LOGNORM, T>(m);
     case SDEGPU_MODEL_OU: return pick_m<GEN, SDEGPU_MODEL_OU, T>(m);
     case SDEGPU_MODEL_CIR: return pick_m<GEN, SDEGPU_MODEL_CIR, T>(m);
     case SDEGPU_MODEL_HESTON: return pick_heston<GEN, T>(m, d);
-    case SDEGPU_MODEL_JUMPDIFF: return pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T>(m);
+    case SDEGPU_MODEL_JUMPDIFF:
+      return defer ? pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, true>(m)
+                   : pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, false>(m);
     default: return nullptr;
   }
 }

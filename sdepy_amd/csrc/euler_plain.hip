@@ -5,12 +5,12 @@
 
 namespace sdegpu {
 
-EulerKernel<float> euler_plain_f32(int model, int m, int d) {
-  return pick_kernel<false, float>(model, m, d);
+EulerKernel<float> euler_plain_f32(int model, int m, int d, bool defer) {
+  return pick_kernel<false, float>(model, m, d, defer);
 }
 
-EulerKernel<double> euler_plain_f64(int model, int m, int d) {
-  return pick_kernel<false, double>(model, m, d);
+EulerKernel<double> euler_plain_f64(int model, int m, int d, bool defer) {
+  return pick_kernel<false, double>(model, m, d, defer);
 }
 
 }  // namespace sdegpu

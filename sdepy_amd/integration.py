@@ -379,6 +379,10 @@ class _gpu_sde:
             d_rows = _lib.to_device(rows)
             keep.append(d_rows)
             prob.store_row = d_rows.data_ptr()
+        if self.model == _lib.MODEL_JUMPDIFF:
+            # jump sizes drawn at the next stored step: pays when at most one
+            # step in 8 stores a row
+            prob.jump_defer = int(8 * np.count_nonzero(rows >= 0) <= nsteps)
 
         # host sources, drawn in sorted id order per step (SDE.dZ)
         recorded = {k: [] for k in host_sources}

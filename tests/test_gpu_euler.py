@@ -278,6 +278,41 @@ def test_merton_matches_twin(gpu, dtype):
     assert _rel(x, ref) < tol
 
 
+def test_merton_deferred_and_in_place_jumps(gpu):
+    """Both forms of the jump sum: with few stored rows the kernel adds the
+    jump sizes of unstored steps just before the next stored row (more than
+    4 jumping steps per path between rows overflow into in-place sums), with a
+    row at every step it sums them in the step.  Both equal the oracle's
+    restatement of sdepy's loop in FP64 and each other at the shared times."""
+    P, n, vs = 500, 200, (3,)
+    args = dict(x0=1., mu=.03, sigma=.2, lam=12., a=-.05, b=.1)
+    out = {}
+    # steps counts points (n + 1 of them, n intervals); .5 is on that grid, so
+    # both timelines integrate the same steps
+    for name, tt in (('sparse', np.array([0., .5, 1.])),
+                     ('dense', np.linspace(0., 1., n + 1))):
+        def make():
+            return sa.merton_jumpdiff_process(paths=P, vshape=vs, dtype=np.float64,
+                                              steps=n + 1, rng=np.random.default_rng(61),
+                                              **args)
+        # the launch's form (prepare advances the source epochs: its own process)
+        run = make().prepare(tt)
+        assert [q.jump_defer for ps in run.passes for q in ps.probs] == \
+            [int(name == 'sparse')]
+        del run
+        p = make()
+        kw, kj = p.sources['dw'].key, p.sources['dj'].key
+        x = p(tt)
+        ref = ora.integrate(
+            'jumpdiff', tt, paths=P, vshape=vs, steps=n + 1,
+            sources={'dw': ora.PhiloxWiener(kw, 0, P, vs),
+                     'dj': ora.PhiloxCPoisson(kj, 0, P, vs, lam=12., a=-.05, b=.1)},
+            x0=1., mu=.03, sigma=.2)
+        assert _rel(x, ref) < RTOL64, name
+        out[name] = x
+    assert _rel(out['sparse'], out['dense'][::n // 2]) < 1e-12
+
+
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_antithetic_sources_match_twin(gpu, dtype):
     """odd_wiener_source / even_cpoisson_source (infrastructure.py:2047-2156):
