@@ -111,3 +111,56 @@ def heston_log_mean(t, x0=1., mu=0., sigma=1., y0=1., theta=1., k=1.):
     alpha = np.exp(-k * t)
     return (np.log(x0) + sigma * sigma * (theta - y0) * (1 - alpha) / (2 * k)
             + (mu - theta * sigma * sigma / 2) * t)
+
+
+def wiener_mean(t, x0=0., mu=0., sigma=1.):
+    """analytical.py:24-38."""
+    return x0 + mu * t
+
+
+def wiener_var(t, x0=0., mu=0., sigma=1.):
+    """analytical.py:41-55."""
+    return sigma * sigma * t
+
+
+def wiener_cdf(t, x, x0=0., mu=0., sigma=1.):
+    """analytical.py:93-109."""
+    return norm.cdf(x, loc=x0 + mu * t, scale=sigma * np.sqrt(t))
+
+
+def wiener_chf(t, u, x0=0., mu=0., sigma=1.):
+    """analytical.py:112-130: the Gaussian exp(i u m - u^2 v / 2)."""
+    return np.exp(1j * u * x0 + t * (1j * mu * u - sigma * sigma * u * u / 2))
+
+
+def lognorm_log_chf(t, u, x0=1., mu=0., sigma=1.):
+    """analytical.py:218-237: chf of log x_t ~ Normal(log x0 + (mu - sigma^2/2) t,
+    sigma^2 t)."""
+    mu0 = mu - sigma * sigma / 2
+    return np.exp(1j * u * np.log(x0) + t * (1j * mu0 * u - sigma * sigma * u * u / 2))
+
+
+def mjd_log_chf(t, u, x0=1., mu=0., sigma=1., lam=1., a=0., b=1.):
+    """analytical.py:748-778 (no martingale correction): the Levy-Khintchine
+    form exp(i u log x0 + t (i u (mu - sigma^2/2) - sigma^2 u^2/2
+    + lam (exp(i a u - b^2 u^2/2) - 1)))."""
+    mu0 = mu - sigma * sigma / 2
+    return np.exp(1j * u * np.log(x0)
+                  + t * (1j * mu0 * u - sigma * sigma * u * u / 2
+                         + lam * (np.exp(1j * a * u - b * b * u * u / 2) - 1)))
+
+
+def heston_euler_log_mean(ds, x0=1., mu=0., sigma=1., y0=1., theta=1., k=1.):
+    """E[log x] after each step of full_heston_process's Euler recursion
+    (integration.py:2425-2433) for constant parameters, on paths whose y stays
+    non-negative (then y+ = y and the recursion is linear in the mean):
+    E[y_{n+1}] = E[y_n] + k (theta - E[y_n]) ds_n and
+    E[log x_{n+1}] = E[log x_n] + (mu - sigma^2 E[y_n] / 2) ds_n.
+    Returns the n+1 means, the first log x0."""
+    m, ly = float(y0), math.log(x0)
+    out = [ly]
+    for d in ds:
+        ly = ly + (mu - sigma * sigma * m / 2) * d
+        m = m + k * (theta - m) * d
+        out.append(ly)
+    return np.array(out)

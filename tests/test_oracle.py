@@ -311,6 +311,69 @@ def test_mjd_mean_and_heston_log_mean_limits():
                (math.log(3.) + (.05 - .02) * 2.)) < 1e-14
 
 
+def test_wiener_lognorm_and_mjd_chf_against_quadrature_and_mixture():
+    """Each characteristic function against an independent evaluation:
+    quadrature of scipy's densities for the Gaussian and lognormal forms, the
+    Poisson-weighted Gaussian mixture for Merton's Levy-Khintchine form, plus
+    the value 1 at u = 0 and the first moment from the slope there."""
+    from scipy import integrate, stats
+    t, x0, mu, s = 1.7, .4, .3, .6
+    m, sd = x0 + mu * t, s * math.sqrt(t)
+    for u in (.3, 1., 2.5):
+        re = integrate.quad(lambda x: math.cos(u * x) * stats.norm.pdf(x, m, sd), -np.inf, np.inf)[0]
+        im = integrate.quad(lambda x: math.sin(u * x) * stats.norm.pdf(x, m, sd), -np.inf, np.inf)[0]
+        assert abs(cf.wiener_chf(t, u, x0, mu, s) - (re + 1j * im)) < 1e-9
+    assert abs(cf.wiener_cdf(t, m + sd, x0, mu, s) - stats.norm.cdf(1.)) < 1e-15
+    assert abs(cf.wiener_mean(t, x0, mu, s) - m) < 1e-15
+    assert abs(cf.wiener_var(t, x0, mu, s) - sd * sd) < 1e-15
+    # lognormal: E[exp(i u log x)] by quadrature over x of scipy's lognormal
+    X0 = 10.
+    dist = stats.lognorm(s=sd, scale=X0 * math.exp((mu - s * s / 2) * t))
+    for u in (.5, 2.):
+        re = integrate.quad(lambda x: math.cos(u * math.log(x)) * dist.pdf(x), 0, np.inf,
+                            limit=200)[0]
+        im = integrate.quad(lambda x: math.sin(u * math.log(x)) * dist.pdf(x), 0, np.inf,
+                            limit=200)[0]
+        assert abs(cf.lognorm_log_chf(t, u, X0, mu, s) - (re + 1j * im)) < 1e-8
+    # Merton: conditional on n jumps log x_t ~ Normal(m + n a, s^2 t + n b^2)
+    lam, a, b = 2., -.2, .1
+    for u in (.4, 1.3, 3.):
+        mix = 0j
+        for n in range(60):
+            w = math.exp(-lam * t) * (lam * t) ** n / math.factorial(n)
+            mn = math.log(X0) + (mu - s * s / 2) * t + n * a
+            vn = s * s * t + n * b * b
+            mix += w * np.exp(1j * u * mn - u * u * vn / 2)
+        assert abs(cf.mjd_log_chf(t, u, X0, mu, s, lam, a, b) - mix) < 1e-13
+    assert cf.mjd_log_chf(t, 0., X0, mu, s, lam, a, b) == 1
+    h = 1e-6
+    slope = (cf.mjd_log_chf(t, h, X0, mu, s, lam, a, b) -
+             cf.mjd_log_chf(t, -h, X0, mu, s, lam, a, b)) / (2j * h)
+    assert abs(slope - (math.log(X0) + (mu - s * s / 2) * t + lam * t * a)) < 1e-8
+
+
+def test_heston_euler_log_mean_against_the_loop_and_the_limit():
+    """With zero increments the oracle's Heston loop is deterministic and its
+    log x is the recursion's mean exactly (drift evaluated at the step start);
+    with fine steps the recursion tends to heston_log_mean."""
+    prm = dict(x0=10., mu=.1, sigma=.2, y0=.5, theta=1.5, k=3.)
+    tt = np.array([1., 1.4, 3.])
+    steps = 41
+    _, _, grid = ora.consolidate(tt, steps)
+    ds = np.diff(grid)
+    zero = ora.ReplaySource(np.zeros((ds.size, 2, 1)), 1, (2,))
+    x, _ = ora.integrate('full_heston', tt, paths=1, steps=steps, sources={'dw': zero},
+                         xi=.7, **prm)
+    mean = cf.heston_euler_log_mean(ds, **prm)
+    assert abs(math.log(x[-1, 0]) - mean[-1]) < 1e-13
+    # first-order convergence: the error halves when the steps double
+    lim = cf.heston_log_mean(2., **prm)
+    err = [cf.heston_euler_log_mean(np.full(n, 2. / n), **prm)[-1] - lim
+           for n in (100, 200, 400)]
+    assert abs(err[0]) > 1e-7
+    assert abs(err[0] / err[1] - 2) < .1 and abs(err[1] / err[2] - 2) < .1
+
+
 def test_merton_series_vs_oracle_monte_carlo():
     P = 60_000
     kw = dict(x0=1., mu=.03, sigma=.2)
