@@ -21,7 +21,9 @@ Printed as one JSON line:
               HBM-bound case: stored bytes / kernel time vs 8 TB/s, and the
               largest deviation from the FP64 run of the same paths
   merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
-              mu, sigma, a, b), with the FP64 run of the same paths in SE units
+              mu, sigma, a, b), with the FP64 run of the same paths in SE units;
+              and the preset forward on the same configuration, terminal row
+              only and full path of 253 rows
   summaries   moments (pmean/pvar and montecarlo's 1..4), min/max, 100-bin
               histograms of every row of the OU
               output, chf of its last row, tmax and FP64 tvar along the
@@ -69,10 +71,10 @@ VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
 # (FETCH reads half the copied bytes: x2.0; WRITE is exact: x1.0), and the
 # kernel's VALU wave-instructions per launch (SQ_INSTS_VALU); the summaries
 # they come from are in profiles/
-ADJ_TRAFFIC_BYTES = 1382726630        # k_heston_adjoint<float, 8>, 4M x 365
+ADJ_TRAFFIC_BYTES = 1382725171        # k_heston_adjoint<float, 8>, 4M x 365
 ADJ_VALU_WAVE_INSTS = 2183953125      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
-OU_TRAFFIC_BYTES = 16023714462         # k_euler<OU, float, 4>, 1M x 1000
+OU_TRAFFIC_BYTES = 16023110048         # k_euler<OU, float, 4>, 1M x 1000
 
 
 def _events(torch):
@@ -491,6 +493,21 @@ def main():
             'greeks': mres.greeks, 'greeks_se': mres.greeks_se,
             'fp32_vs_fp64_same_paths_se': err}
         del mrun, m64
+        # the preset forward (k_euler, jumpdiff) on the same configuration:
+        # the terminal row only (jump sizes drawn at the stored row,
+        # jump_defer) and the full path of 253 rows (jumps summed in place)
+        import sdepy_amd as sa
+        fwd = {}
+        for name, tl in (('terminal', (0., 1.)), ('full_path', np.linspace(0., 1., 253))):
+            mp = sa.merton_jumpdiff_process(paths=2_000_000, steps=253, dtype=np.float32,
+                                            x0=1., mu=.03, sigma=.2, lam=1., a=-.1, b=.15,
+                                            rng=np.random.default_rng(7))
+            frun = mp.prepare(tl)
+            f_ms, _ = timed(torch, lambda: frun.launch(fresh=True), 10, 3)
+            fwd[name] = {'ms': f_ms, 'path_steps_per_s': frun.path_steps / (f_ms * 1e-3),
+                         'rows': len(tl), 'jump_defer': int(frun.passes[0].prob.jump_defer)}
+            del frun, mp
+        out['merton']['forward'] = fwd
 
     # ---- GBM 10k x 252 FP64 full path + pmean/pvar (C1, latency-bound) ----
     if 'gbm' in legs:
