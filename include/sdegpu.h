@@ -117,7 +117,8 @@ typedef struct sdegpu_euler_problem {
   int32_t* info_paths;        /* optional device [nvars][paths]: Heston y<0 counts (first N
                                  rows used) or jump counts (full_heston_SDE.info_next :2435-2439,
                                  jumpdiff_SDE.info_next :2610-2619) */
-  long long* info_rows;       /* optional device [nrows]: jump totals per output interval */
+  long long* info_rows;       /* optional device [nrows][slots]: jump totals per output
+                                 interval, slots = max(1, info_row_slots) (see below) */
   uint32_t key_w[2];          /* Philox key of the dw source */
   uint32_t epoch_w;           /* Philox epoch of the dw source */
   uint32_t key_j[2];          /* Philox key of the dj source (JUMPDIFF) */
@@ -154,6 +155,10 @@ typedef struct sdegpu_euler_problem {
                                  rounded in another order; faster when few steps store a
                                  row.  0: every step sums its jumps with its increment, as
                                  euler_next does (integration.py:707-723). */
+  int32_t info_row_slots;     /* 0 or 1: info_rows is [nrows]; S in 2..1024: info_rows is
+                                 [nrows][S] and each block of 256 paths adds its jumps to
+                                 slot blockIdx % S of its row, so the atomics of one row
+                                 spread over S addresses; the caller sums the slots */
 } sdegpu_euler_problem;
 
 /* Library and device queries. */

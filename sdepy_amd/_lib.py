@@ -53,6 +53,7 @@ class EulerProblem(ctypes.Structure):
         ('heston_width', c_int32),
         ('comp0', c_int32), ('nvars_total', c_int32),
         ('out0', c_int32), ('out_stride', c_int32), ('jump_defer', c_int32),
+        ('info_row_slots', c_int32),
     ]
 
 

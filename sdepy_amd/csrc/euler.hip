@@ -90,6 +90,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   if (pr->comp0 < 0 || m_total < m || pr->comp0 > m_total - m)
     return fail(SDEGPU_EINVAL, "component group %d..%d outside a state of %d components",
                 pr->comp0, pr->comp0 + m - 1, m_total);
+  if (pr->info_row_slots < 0 || pr->info_row_slots > 1024)
+    return fail(SDEGPU_EINVAL, "info_row_slots must be in [0, 1024], not %d", pr->info_row_slots);
   if (pr->out0 < 0 || out_stride < pr->nout || pr->out0 > out_stride - pr->nout)
     return fail(SDEGPU_EINVAL, "output values %d..%d outside rows of %d values", pr->out0,
                 pr->out0 + pr->nout - 1, out_stride);
@@ -118,6 +120,7 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.replay_dj = pr->replay_dj;
   a.info_paths = pr->info_paths;
   a.info_rows = pr->info_rows;
+  a.info_slots = pr->info_row_slots > 1 ? pr->info_row_slots : 1;
   for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
   a.paths = pr->paths;
   a.anti_w = pr->anti_w;

@@ -50,6 +50,7 @@ struct EulerArgs {
   // out0 .. out0+nout-1 of rows of out_stride values; for an ungrouped launch
   // comp0 = out0 = 0, m_total = M, out_stride = nout
   int32_t comp0, m_total, out0, out_stride;
+  int32_t info_slots;  // counters per row of info_rows (>= 1)
 };
 
 // Steps per chunk so that a chunk consumes a whole number of Philox blocks:
@@ -413,8 +414,27 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
         if constexpr (kDefer) flush_jumps();
         store_state<MODEL, T, M, D, GEN>(out, a, row, x, gid);
         if constexpr (kJump) {
-          if (a.info_rows && interval_jumps != 0)
-            atomicAdd((unsigned long long*)(a.info_rows + row), (unsigned long long)(long long)interval_jumps);
+          // the interval's jump total: one atomic per wave, into one of the
+          // row's info_slots counters (every wave reaches a row at about the
+          // same time; on one address their adds queue behind each other).
+          // The step and its row are the same for the whole wave, so its live
+          // lanes are all here; the lanes with jumps are summed by a scalar
+          // walk of their ballot.
+          if (a.info_rows) {
+            uint64_t todo = __ballot(interval_jumps != 0);
+            if (todo) {
+              long long sum = 0;
+              while (todo) {
+                const int l = __ffsll((unsigned long long)todo) - 1;
+                sum += __builtin_amdgcn_readlane(interval_jumps, l);
+                todo &= todo - 1;
+              }
+              if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
+                atomicAdd((unsigned long long*)(a.info_rows + (int64_t)row * a.info_slots +
+                                                (int)(blockIdx.x % (unsigned)a.info_slots)),
+                          (unsigned long long)sum);
+            }
+          }
           interval_jumps = 0;
         }
       }

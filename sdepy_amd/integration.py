@@ -86,6 +86,11 @@ def consolidate_timeline(timeline, steps, i0):
     return tt, tt_asis, steps_tt
 
 
+# counters per output row for the jump totals of jumpdiff's info (the kernel
+# spreads one row's atomic adds over them by block)
+INFO_SLOTS = 64
+
+
 def _store_rows(steps_pass, tt_pass, row_of):
     """Output row reached after each step of a pass, or -1
     (paths_generator._store_if_reached, integration.py:377-390).  A pass runs
@@ -416,8 +421,11 @@ class _gpu_sde:
             info_paths = _lib.zeros((M, P), np.int32)
             prob.info_paths = info_paths.data_ptr()
             if self.model == _lib.MODEL_JUMPDIFF:
-                info_rows = _lib.zeros((xx_dev.shape[0],), np.int64)
+                # jump totals per output row, spread over INFO_SLOTS counters
+                # (one address per row would queue every wave's atomic add)
+                info_rows = _lib.zeros((xx_dev.shape[0], INFO_SLOTS), np.int64)
                 prob.info_rows = info_rows.data_ptr()
+                prob.info_row_slots = INFO_SLOTS
 
         # one problem per component group: its own step table, Poisson table
         # and initial values; the full-state device arrays are shared
@@ -1032,7 +1040,7 @@ class jumpdiff_process(_gpu_sde):
             counts = np.zeros((self._nvars(), P), dtype=int)
         elif info_paths is not None:
             counts = _lib.to_host(info_paths)
-            tr = _lib.to_host(info_rows).astype(np.int64)
+            tr = _lib.to_host(info_rows).astype(np.int64).sum(axis=1)
             for j in range(1, n_t):
                 totals[j] = tr[int(row_of(j))]
         else:

@@ -124,6 +124,13 @@ def test_entry_point_validation_maps_to_exceptions():
     pr.comp0, pr.out0, pr.out_stride = 8, 10, 16
     with pytest.raises(ValueError):        # values 10..17 outside rows of 16
         _lib.check(lib.sdegpu_euler(pr, None))
+    pr.out0, pr.info_row_slots = 0, 1025
+    with pytest.raises(ValueError, match='info_row_slots'):
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.info_row_slots = -1
+    with pytest.raises(ValueError, match='info_row_slots'):
+        _lib.check(lib.sdegpu_euler(pr, None))
+    pr.info_row_slots = 0
     pr.heston_width = pr.comp0 = pr.nvars_total = pr.out0 = pr.out_stride = 0
     pr.model, pr.nvars, pr.paths = _lib.MODEL_OU, 1, 0
     with pytest.raises(ValueError):
