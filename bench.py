@@ -71,10 +71,10 @@ VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
 # (FETCH reads half the copied bytes: x2.0; WRITE is exact: x1.0), and the
 # kernel's VALU wave-instructions per launch (SQ_INSTS_VALU); the summaries
 # they come from are in profiles/
-ADJ_TRAFFIC_BYTES = 1382725171        # k_heston_adjoint<float, 8>, 4M x 365
+ADJ_TRAFFIC_BYTES = 1382829229        # k_heston_adjoint<float, 8>, 4M x 365
 ADJ_VALU_WAVE_INSTS = 2183953125      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
-OU_TRAFFIC_BYTES = 16023110048         # k_euler<OU, float, 4>, 1M x 1000
+OU_TRAFFIC_BYTES = 16022206495         # k_euler<OU, float, 4>, 1M x 1000
 
 
 def _events(torch):
