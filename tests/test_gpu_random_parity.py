@@ -5,7 +5,8 @@ in component groups), constant or time-dependent parameters, a correlation, an
 output timeline, `steps` and `i0`, integrates it on the GPU and compares every
 stored value with the oracle's restatement of sdepy's loop driven by the same
 Philox keys (the epochs advance once per pass, the backward pass of i0 != 0
-first, as the integrator draws them).  The case is printed before it runs.
+first, as the integrator draws them), and for Merton the jump info (counts
+per path, rates per output interval).  The case is printed before it runs.
 """
 import numpy as np
 import pytest
@@ -122,8 +123,11 @@ def _run(seed):
     if corr is None and vs and 2 <= last <= 8 and preset not in ('hull_white',) \
             and g.integers(2):
         corr = _corr(last, g)
+    info, ref_info = {}, {}
     kw = dict(paths=P, vshape=vshape, dtype=np.float64, steps=steps, i0=i0,
               rng=np.random.default_rng(seed))
+    if preset == 'merton':
+        kw['info'] = info
     if corr is not None:
         kw['corr'] = corr
     p = cls(**kw, **prm, **extra)
@@ -136,7 +140,7 @@ def _run(seed):
                                      lam=extra['lam'], a=extra['a'], b=extra['b'])
     opreset = {'merton': 'jumpdiff'}.get(preset, preset)
     ref = ora.integrate(opreset, tt, paths=P, vshape=vshape, steps=steps, i0=i0,
-                        sources=sources,
+                        sources=sources, **({'info': ref_info} if preset == 'merton' else {}),
                         **({'factors': extra['factors']} if preset == 'hull_white' else {}),
                         **oparams)
     xs = x if isinstance(x, tuple) else (x,)
@@ -146,6 +150,11 @@ def _run(seed):
         assert a.shape == b.shape, (a.shape, b.shape)
         assert np.allclose(a, b, rtol=1e-10, atol=1e-12), \
             (seed, preset, float(np.max(np.abs(np.asarray(a) - b))))
+    if preset == 'merton':
+        # jump counts per path and jump rates per output interval (the
+        # per-row totals summed on the device), both passes of i0 != 0
+        assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+        assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize('seed', range(48))
