@@ -278,6 +278,32 @@ def test_merton_matches_twin(gpu, dtype):
     assert _rel(x, ref) < tol
 
 
+def test_process_as_dw_source_matches_oracle(gpu):
+    """A process is itself a source (infrastructure.py:615-633; the
+    reference's tests drive presets with Wiener processes Z2, Z23,
+    test_processes.py:92-97): its increment over (s, s + ds) is the linear
+    interpolation difference.  Here a Wiener process on 100 points drives a
+    2-component OU on an 11-point timeline; the oracle replays increments
+    interpolated path by path with np.interp."""
+    P = 11
+    zt = np.linspace(0, 1, 100)
+    Z = sa.wiener_process(sigma=.1, paths=P, vshape=2, dtype=np.float64,
+                          rng=np.random.default_rng(71))(zt)
+    tt = np.linspace(0, 1, 11)
+    prm = dict(x0=.2, theta=np.array([[.1], [-.3]]), k=np.array([[1.], [2.5]]),
+               sigma=np.array([[.3], [.2]]))
+    x = sa.ornstein_uhlenbeck_process(paths=P, vshape=2, dtype=np.float64, dw=Z,
+                                      **prm)(tt)
+    zc = np.asarray(Z)
+    interp = np.stack([np.stack([np.interp(tt, zt, zc[:, v, p]) for p in range(P)], axis=-1)
+                       for v in range(2)], axis=1)          # (11, 2, P)
+    inc = np.diff(interp, axis=0)
+    ref = ora.integrate('ou', tt, paths=P, vshape=(2,),
+                        sources={'dw': ora.ReplaySource(inc, P, (2,))}, **prm)
+    assert x.shape == (11, 2, P)
+    assert np.max(np.abs(x - ref)) < 1e-14
+
+
 def test_merton_deferred_and_in_place_jumps(gpu):
     """Both forms of the jump sum: with few stored rows the kernel adds the
     jump sizes of unstored steps just before the next stored row (more than
