@@ -164,3 +164,21 @@ def heston_euler_log_mean(ds, x0=1., mu=0., sigma=1., y0=1., theta=1., k=1.):
         m = m + k * (theta - m) * d
         out.append(ly)
     return np.array(out)
+
+
+def kou_mean(t, x0=1., mu=0., sigma=1., lam=1., a=.5, b=.5, pa=.5):
+    """analytical.py:813-836 (no martingale correction): jumps Y are +Exp(a)
+    with probability pa and -Exp(b) otherwise, E[e^Y] = pa/(1-a) + pb/(1+b)."""
+    pb = 1 - pa
+    return x0 * np.exp((mu + lam * (pa / (1 - a) + pb / (1 + b) - 1)) * t)
+
+
+def kou_log_chf(t, u, x0=1., mu=0., sigma=1., lam=1., a=.5, b=.5, pa=.5):
+    """analytical.py:839-869: exp(i u log x0 + t (i u (mu - sigma^2/2)
+    - sigma^2 u^2/2 + lam (phi_Y(u) - 1))) with the double-exponential
+    phi_Y(u) = pa/(1 - i u a) + pb/(1 + i u b)."""
+    pb = 1 - pa
+    phi = pa / (1 - 1j * u * a) + pb / (1 + 1j * u * b)
+    mu0 = mu - sigma * sigma / 2
+    return np.exp(1j * u * np.log(x0)
+                  + t * (1j * mu0 * u - sigma * sigma * u * u / 2 + lam * (phi - 1)))

@@ -11,8 +11,9 @@ the reference's parameter sets (test_quant.py:198-216) and step counts.
 
 Where Euler's own moments are known exactly the comparison is with them, so
 the discretisation bias is not mistaken for an error:
-  * wiener, lognorm (log-Euler) and Merton (log-Euler with compound Poisson
-    increments) are exact in distribution for constant parameters;
+  * wiener, lognorm (log-Euler), Merton and Kou (log-Euler with compound
+    Poisson increments; Kou's double-exponential sizes drawn on the host and
+    replayed) are exact in distribution for constant parameters;
   * OU and Hull-White are linear recursions: oracle.closed_forms.
     ou_euler_moments;
   * CIR and Heston's variance follow y' = y + k (theta - y+) ds + xi sqrt(y+) dw,
@@ -165,3 +166,21 @@ def test_merton_exact_in_distribution(gpu):
            'merton t=%g mean' % S[i])
     _chf(np.log(np.asarray(x[2], np.float64)), 1.,
          complex(cf.mjd_log_chf(2., 1., **prm)), 'merton log')
+
+
+def test_kou_exact_in_distribution(gpu):
+    """Kou's double-exponential jumps are drawn on the host (through the
+    source protocol) and replayed in the kernel; the log-Euler scheme with
+    compound Poisson increments is exact in distribution.  2e5 paths (the
+    host draws each step's jumps)."""
+    Pk = 200_000
+    prm = dict(x0=6., mu=.1, sigma=.3, lam=2., pa=.8, a=.2, b=.3)
+    x = sa.kou_jumpdiff_process(paths=Pk, dtype=np.float32, steps=200,
+                                rng=np.random.default_rng(108), **prm)(S)
+    pm = x.pmean()
+    for i in (1, 2):
+        xs = np.asarray(x[i], np.float64)
+        _z(float(pm[i, 0]), cf.kou_mean(S[i] - S[0], **prm), float(xs.std()) / math.sqrt(Pk),
+           'kou t=%g mean' % S[i])
+    _chf(np.log(np.asarray(x[2], np.float64)), 1.,
+         complex(cf.kou_log_chf(2., 1., **prm)), 'kou log')

@@ -352,6 +352,32 @@ def test_wiener_lognorm_and_mjd_chf_against_quadrature_and_mixture():
     assert abs(slope - (math.log(X0) + (mu - s * s / 2) * t + lam * t * a)) < 1e-8
 
 
+def test_kou_chf_against_quadrature_and_the_mean():
+    """Kou's double-exponential jump chf against quadrature of its density,
+    the slope at 0 against the mean log, and chf(-i) = E[x] against kou_mean
+    (the same identity pins mjd_mean with mjd_log_chf)."""
+    from scipy import integrate
+    t, x0, mu, s, lam, a, b, pa = 2., 6., .1, .3, 2., .2, .3, .8
+    pb = 1 - pa
+    dens = (lambda y: pa / a * math.exp(-y / a), lambda y: pb / b * math.exp(y / b))
+    for u in (.5, 1.7):
+        phi = (integrate.quad(lambda y: math.cos(u * y) * dens[0](y), 0, np.inf)[0]
+               + integrate.quad(lambda y: math.cos(u * y) * dens[1](y), -np.inf, 0)[0]
+               + 1j * (integrate.quad(lambda y: math.sin(u * y) * dens[0](y), 0, np.inf)[0]
+                       + integrate.quad(lambda y: math.sin(u * y) * dens[1](y), -np.inf, 0)[0]))
+        want = np.exp(1j * u * math.log(x0) + t * (1j * u * (mu - s * s / 2)
+                                                   - s * s * u * u / 2 + lam * (phi - 1)))
+        assert abs(cf.kou_log_chf(t, u, x0, mu, s, lam, a, b, pa) - want) < 1e-9
+    h = 1e-6
+    slope = (cf.kou_log_chf(t, h, x0, mu, s, lam, a, b, pa) -
+             cf.kou_log_chf(t, -h, x0, mu, s, lam, a, b, pa)) / (2j * h)
+    assert abs(slope - (math.log(x0) + (mu - s * s / 2) * t + lam * t * (pa * a - pb * b))) < 1e-8
+    assert abs(cf.kou_log_chf(t, -1j, x0, mu, s, lam, a, b, pa)
+               - cf.kou_mean(t, x0, mu, s, lam, a, b, pa)) < 1e-12 * x0 * 10
+    assert abs(cf.mjd_log_chf(t, -1j, x0, mu, s, lam, -.2, .1)
+               - cf.mjd_mean(t, x0, mu, s, lam, -.2, .1)) < 1e-12 * x0 * 10
+
+
 def test_heston_euler_log_mean_against_the_loop_and_the_limit():
     """With zero increments the oracle's Heston loop is deterministic and its
     log x is the recursion's mean exactly (drift evaluated at the step start);
