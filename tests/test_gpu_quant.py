@@ -184,3 +184,71 @@ def test_kou_exact_in_distribution(gpu):
            'kou t=%g mean' % S[i])
     _chf(np.log(np.asarray(x[2], np.float64)), 1.,
          complex(cf.kou_log_chf(2., 1., **prm)), 'kou log')
+
+
+def _chi2(sample, cdf, lo, hi, what, nbins=100):
+    """Chi-square goodness of fit of a GPU montecarlo histogram on fixed edges
+    (lo..hi) against the exact bin probabilities F(e[b+1]) - F(e[b]); values
+    outside the edges form one more cell, cells expecting fewer than 20 are
+    lumped into it.  Passes unless p < 1e-3."""
+    import scipy.stats
+    edges = np.linspace(lo, hi, nbins + 1)
+    mc = sa.montecarlo(sample, bins=edges)
+    counts, _ = mc.histogram()
+    n = sample.size
+    E = n * np.diff(cdf(edges))
+    O = np.asarray(counts, float)
+    keep = E >= 20
+    out_O, out_E = n - O[keep].sum(), n - E[keep].sum()
+    chi = float(((O[keep] - E[keep]) ** 2 / E[keep]).sum() + (out_O - out_E) ** 2 / out_E)
+    dof = int(keep.sum())
+    p = float(scipy.stats.chi2.sf(chi, dof))
+    print('%-28s chi2 %.1f dof %d p %.3g' % (what, chi, dof, p), flush=True)
+    assert p > 1e-3, what
+
+
+def test_terminal_distributions_chi_square(gpu):
+    """Whole terminal distributions where they are known exactly: lognormal
+    (log-Euler), Wiener and OU (Gaussian; OU with the Euler recursion's mean
+    and variance) and Merton's log (Poisson-weighted Gaussian mixture), each
+    a 100-bin histogram on the device against the closed-form bin
+    probabilities (the pdf/cdf checks of test_quant.py:290-330).  The edges
+    span the closed form's 0.05%..99.95% quantiles (Merton's mixture: the
+    sample's, as it has no closed-form quantile)."""
+    from scipy import stats
+    T = 2.
+    prm = dict(x0=10., mu=.1, sigma=.3)
+    x = sa.lognorm_process(paths=P, dtype=np.float32, steps=50,
+                           rng=np.random.default_rng(111), **prm)(S)
+    d = stats.lognorm(s=.3 * math.sqrt(T), scale=10. * math.exp((.1 - .045) * T))
+    _chi2(np.asarray(x[2]), d.cdf, d.ppf(5e-4), d.ppf(1 - 5e-4), 'lognorm terminal')
+
+    w = sa.wiener_process(paths=P, dtype=np.float32, steps=50, x0=.5, mu=-.2, sigma=.4,
+                          rng=np.random.default_rng(112))(S)
+    d = stats.norm(.5 - .2 * T, .4 * math.sqrt(T))
+    _chi2(np.asarray(w[2]), d.cdf, d.ppf(5e-4), d.ppf(1 - 5e-4), 'wiener terminal')
+
+    ds, idx = _grid(200)
+    o = sa.ornstein_uhlenbeck_process(paths=P, dtype=np.float32, steps=200, x0=.2,
+                                      theta=.1, k=1., sigma=.03,
+                                      rng=np.random.default_rng(113))(S)
+    means, covs = cf.ou_euler_moments(ds, .2, .1, 1., .03)
+    d = stats.norm(means[idx[2], 0], math.sqrt(covs[idx[2], 0, 0]))
+    _chi2(np.asarray(o[2]), d.cdf, d.ppf(5e-4), d.ppf(1 - 5e-4), 'ou terminal')
+
+    mp = dict(x0=6., mu=.1, sigma=.3, lam=2., a=-.2, b=.1)
+    m = sa.merton_jumpdiff_process(paths=P, dtype=np.float32, steps=200,
+                                   rng=np.random.default_rng(114), **mp)(S)
+    lm = np.log(np.asarray(m[2], np.float64))
+
+    def mix_cdf(y):
+        y = np.asarray(y, float)
+        tot = np.zeros_like(y)
+        for k in range(40):
+            wk = stats.poisson.pmf(k, 2. * T)
+            mk = math.log(6.) + (.1 - .045) * T + k * -.2
+            sk = math.sqrt(.09 * T + k * .01)
+            tot += wk * stats.norm.cdf((y - mk) / sk)
+        return tot
+    _chi2(lm, mix_cdf, float(np.quantile(lm, 5e-4)), float(np.quantile(lm, 1 - 5e-4)),
+          'merton log terminal')
