@@ -75,10 +75,10 @@ def _case(seed):
     return g, preset, vshape, tdep, tt, steps, i0
 
 
-def _run(seed):
+def _run(seed, dtype=np.float64):
     g, preset, vshape, tdep, tt, steps, i0 = _case(seed)
-    print('case', seed, preset, vshape, 'tdep' if tdep else 'const', tt, steps, i0,
-          flush=True)
+    print('case', seed, np.dtype(dtype).name, preset, vshape, 'tdep' if tdep else 'const',
+          tt, steps, i0, flush=True)
     vs = vshape
     last = vs[-1] if vs else 1
     prm, wshape, corr, extra = {}, vs, None, {}
@@ -124,7 +124,7 @@ def _run(seed):
             and g.integers(2):
         corr = _corr(last, g)
     info, ref_info = {}, {}
-    kw = dict(paths=P, vshape=vshape, dtype=np.float64, steps=steps, i0=i0,
+    kw = dict(paths=P, vshape=vshape, dtype=dtype, steps=steps, i0=i0,
               rng=np.random.default_rng(seed))
     if preset == 'merton':
         kw['info'] = info
@@ -148,8 +148,15 @@ def _run(seed):
     assert len(xs) == len(rs)
     for a, b in zip(xs, rs):
         assert a.shape == b.shape, (a.shape, b.shape)
-        assert np.allclose(a, b, rtol=1e-10, atol=1e-12), \
-            (seed, preset, float(np.max(np.abs(np.asarray(a) - b))))
+        if dtype == np.float64:
+            assert np.allclose(a, b, rtol=1e-10, atol=1e-12), \
+                (seed, preset, float(np.max(np.abs(np.asarray(a) - b))))
+        else:
+            # FP32 state against the FP64 oracle: rounding of a few dozen
+            # steps, relative to the size of the values
+            scale = float(np.max(np.abs(b)))
+            assert np.allclose(np.asarray(a, np.float64), b, rtol=2e-5, atol=2e-5 * scale), \
+                (seed, preset, float(np.max(np.abs(np.asarray(a, np.float64) - b))), scale)
     if preset == 'merton':
         # jump counts per path and jump rates per output interval (the
         # per-row totals summed on the device), both passes of i0 != 0
@@ -160,3 +167,10 @@ def _run(seed):
 @pytest.mark.parametrize('seed', range(48))
 def test_random_fp64_parity(gpu, seed):
     _run(seed)
+
+
+@pytest.mark.parametrize('seed', range(48))
+def test_random_fp32_against_fp64_oracle(gpu, seed):
+    """The same cases stepped in FP32 (the FP32 kernel instances) against the
+    FP64 oracle on the same draws, within 2e-5 relative to the values' size."""
+    _run(seed, np.float32)
