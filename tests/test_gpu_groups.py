@@ -133,6 +133,25 @@ def test_heston_hull_white_and_merton_groups_match_oracle(gpu):
     assert _rel(w, ref) < RTOL64
     assert np.array_equal(info['jump_count'], ref_info['jump_count'])
     assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12)
+    # terminal row only: the grouped launches add the jump sizes at the row
+    # (jump_defer), in the general kernel instance
+    info = {}
+    m2 = sa.merton_jumpdiff_process(paths=P, vshape=9, dtype=np.float64, steps=n,
+                                    info=info, x0=1., mu=.03, sigma=.2, lam=4., a=-.1,
+                                    b=.15, rng=np.random.default_rng(12))
+    kw, kj = m2.sources['dw'].key, m2.sources['dj'].key
+    assert all(q.jump_defer == 1 for ps in sa.merton_jumpdiff_process(
+        paths=P, vshape=9, steps=n, lam=4.).prepare((0., 1.)).passes for q in ps.probs)
+    w2 = m2((0., 1.))
+    ref_info = {}
+    ref2 = ora.integrate('jumpdiff', (0., 1.), paths=P, vshape=(9,), steps=n, info=ref_info,
+                         sources={'dw': ora.PhiloxWiener(kw, 0, P, (9,)),
+                                  'dj': ora.PhiloxCPoisson(kj, 0, P, (9,), lam=4.,
+                                                           a=-.1, b=.15)},
+                         x0=1., mu=.03, sigma=.2)
+    assert _rel(w2, ref2) < RTOL64
+    assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+    assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12)
 
 
 def test_group_sources_replay_and_antithetics(gpu):
