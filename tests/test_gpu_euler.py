@@ -339,6 +339,50 @@ def test_merton_deferred_and_in_place_jumps(gpu):
     assert _rel(out['sparse'], out['dense'][::n // 2]) < 1e-12
 
 
+def test_merton_deferred_jumps_backward_pass(gpu):
+    """i0 = 1 on (0, .5, 1): a backward pass to t = 0 and a forward pass to
+    t = 1, each storing one row over 20 steps, so both defer their jump
+    sizes; the backward pass negates them (dj = sign(ds) * sum)."""
+    P, n = 400, 41
+    args = dict(x0=1., mu=.03, sigma=.2, lam=6., a=-.05, b=.1)
+    tt = np.array([0., .5, 1.])
+
+    def make():
+        return sa.merton_jumpdiff_process(paths=P, dtype=np.float64, steps=n, i0=1,
+                                          rng=np.random.default_rng(62), **args)
+    run = make().prepare(tt)
+    assert len(run.passes) == 2
+    assert all(q.jump_defer == 1 for ps in run.passes for q in ps.probs)
+    del run
+    info, ref_info = {}, {}
+    kw, kj = make().sources['dw'].key, make().sources['dj'].key   # the run's keys (same seed)
+
+    class _Seq(ora.PhiloxWiener):
+        passes = 0
+
+        def begin_pass(self, tag):
+            super().begin_pass(tag)
+            self.epoch = self.passes
+            self.passes += 1
+
+    class _SeqJ(ora.PhiloxCPoisson):
+        passes = 0
+
+        def begin_pass(self, tag):
+            super().begin_pass(tag)
+            self.epoch = self.passes
+            self.passes += 1
+    x = sa.merton_jumpdiff_process(paths=P, dtype=np.float64, steps=n, i0=1, info=info,
+                                   rng=np.random.default_rng(62), **args)(tt)
+    ref = ora.integrate('jumpdiff', tt, paths=P, steps=n, i0=1, info=ref_info,
+                        sources={'dw': _Seq(kw, 0, P, ()),
+                                 'dj': _SeqJ(kj, 0, P, (), lam=6., a=-.05, b=.1)},
+                        x0=1., mu=.03, sigma=.2)
+    assert _rel(x, ref) < RTOL64
+    assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+    assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12)
+
+
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_antithetic_sources_match_twin(gpu, dtype):
     """odd_wiener_source / even_cpoisson_source (infrastructure.py:2047-2156):
