@@ -32,7 +32,9 @@ Printed as one JSON line:
   gbm_c1      GBM 10k x 252 FP64 full path + pmean/pvar (latency-bound):
               kernel time and the end-to-end preset call
   batched     1024 heterogeneous GBM / HW2F instances in one persistent
-              launch; per-instance FP32 moments vs FP64 runs of the same paths
+              launch; per-instance FP32 moments vs FP64 runs of the same paths;
+              VALU issue rate vs the nominal ceiling (SQ_INSTS_VALU per launch
+              from rocprof)
   cpu_baseline the oracle's NumPy restatement of sdepy's loop (same
               operations as the reference, NumPy draws) on the host, one core,
               price + 7 Greeks by 15 forward runs on a bounded path sample;
@@ -75,6 +77,7 @@ ADJ_TRAFFIC_BYTES = 1382829229        # k_heston_adjoint<float, 8>, 4M x 365
 ADJ_VALU_WAVE_INSTS = 2183953125      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
 OU_TRAFFIC_BYTES = 16022206495         # k_euler<OU, float, 4>, 1M x 1000
+BATCH_VALU_WAVE_INSTS = 32935748358    # k_batch_persistent, the 1024-instance set
 
 
 def _events(torch):
@@ -567,7 +570,13 @@ def main():
             'blocks': bt.blocks, 'cu': bt.cu,
             'autotune_s': {str(k): v for k, v in bt.tuning.items()},
             'prepare_s': prep, 'graph': bt.graph is not None,
-            'fp32_vs_fp64_same_paths': inst_err}
+            'fp32_vs_fp64_same_paths': inst_err,
+            'valu_issue': {
+                'kernel': 'k_batch_persistent', 'bound': 'valu',
+                'wave_insts_per_launch': BATCH_VALU_WAVE_INSTS,
+                'achieved_per_s': BATCH_VALU_WAVE_INSTS / (bms * 1e-3),
+                'peak_per_s': VALU_ISSUE_PEAK_PER_S,
+                'frac': BATCH_VALU_WAVE_INSTS / (bms * 1e-3) / VALU_ISSUE_PEAK_PER_S}}
         del bt
         torch.cuda.empty_cache()
 
