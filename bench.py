@@ -21,7 +21,8 @@ Printed as one JSON line:
               HBM-bound case: stored bytes / kernel time vs 8 TB/s, and the
               largest deviation from the FP64 run of the same paths
   merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
-              mu, sigma, a, b), with the FP64 run of the same paths in SE units;
+              mu, sigma, a, b), with the FP64 run of the same paths in SE units
+              and its VALU issue rate;
               and the preset forward on the same configuration, terminal row
               only and full path of 253 rows
   summaries   moments (pmean/pvar and montecarlo's 1..4), min/max, 100-bin
@@ -78,6 +79,7 @@ ADJ_VALU_WAVE_INSTS = 2183953125      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
 OU_TRAFFIC_BYTES = 16022206495         # k_euler<OU, float, 4>, 1M x 1000
 BATCH_VALU_WAVE_INSTS = 32935748358    # k_batch_persistent, the 1024-instance set
+MERTON_VALU_WAVE_INSTS = 362572745     # k_loglin_greeks<float>, 2M x 252
 
 
 def _events(torch):
@@ -494,7 +496,13 @@ def main():
             'greeks_per_s': len(G.LOG_GREEKS) / (m_ms * 1e-3),
             'price': mres.price, 'price_se': mres.price_se,
             'greeks': mres.greeks, 'greeks_se': mres.greeks_se,
-            'fp32_vs_fp64_same_paths_se': err}
+            'fp32_vs_fp64_same_paths_se': err,
+            'valu_issue': {
+                'kernel': 'k_loglin_greeks<float>', 'bound': 'valu',
+                'wave_insts_per_launch': MERTON_VALU_WAVE_INSTS,
+                'achieved_per_s': MERTON_VALU_WAVE_INSTS / (m_ms * 1e-3),
+                'peak_per_s': VALU_ISSUE_PEAK_PER_S,
+                'frac': MERTON_VALU_WAVE_INSTS / (m_ms * 1e-3) / VALU_ISSUE_PEAK_PER_S}}
         del mrun, m64
         # the preset forward (k_euler, jumpdiff) on the same configuration:
         # the terminal row only (jump sizes drawn at the stored row,
