@@ -36,6 +36,11 @@ Printed as one JSON line:
               launch; per-instance FP32 moments vs FP64 runs of the same paths;
               VALU issue rate vs the nominal ceiling (SQ_INSTS_VALU per launch
               from rocprof)
+  valu_issue  (tangent, merton, batched) VALU wave-instructions per launch
+              (rocprof SQ_INSTS_VALU) per second of kernel time, against the
+              nominal one wave64 instruction per CU per clock: the measured
+              rate of dependent FMA chains; FP32 add/mul and integer
+              instructions issue faster, so a mix can exceed it
   cpu_baseline the oracle's NumPy restatement of sdepy's loop (same
               operations as the reference, NumPy draws) on the host, one core,
               price + 7 Greeks by 15 forward runs on a bounded path sample;
@@ -80,6 +85,7 @@ ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
 OU_TRAFFIC_BYTES = 16022206495         # k_euler<OU, float, 4>, 1M x 1000
 BATCH_VALU_WAVE_INSTS = 32935748358    # k_batch_persistent, the 1024-instance set
 MERTON_VALU_WAVE_INSTS = 362572745     # k_loglin_greeks<float>, 2M x 252
+TANGENT_VALU_WAVE_INSTS = 1523390625   # k_heston_tangent<float>, 4M x 365
 
 
 def _events(torch):
@@ -389,7 +395,13 @@ def main():
             'price': tres.price, 'greeks': tres.greeks,
             'minus_adjoint_over_se': {
                 k: (tres.greeks[k] - res.greeks[k]) / res.greeks_se[k]
-                for k in G.HESTON_GREEKS}}
+                for k in G.HESTON_GREEKS},
+            'valu_issue': {
+                'wave_insts_per_launch': TANGENT_VALU_WAVE_INSTS,
+                'achieved_per_s': TANGENT_VALU_WAVE_INSTS / (t_ms * 1e-3),
+                'nominal_per_s': VALU_ISSUE_PEAK_PER_S,
+                'frac_of_nominal': TANGENT_VALU_WAVE_INSTS / (t_ms * 1e-3)
+                / VALU_ISSUE_PEAK_PER_S}}
         del trun
 
     # ---- FP64 forward + CRN finite differences, same paths ----
@@ -501,8 +513,9 @@ def main():
                 'kernel': 'k_loglin_greeks<float>', 'bound': 'valu',
                 'wave_insts_per_launch': MERTON_VALU_WAVE_INSTS,
                 'achieved_per_s': MERTON_VALU_WAVE_INSTS / (m_ms * 1e-3),
-                'peak_per_s': VALU_ISSUE_PEAK_PER_S,
-                'frac': MERTON_VALU_WAVE_INSTS / (m_ms * 1e-3) / VALU_ISSUE_PEAK_PER_S}}
+                'nominal_per_s': VALU_ISSUE_PEAK_PER_S,
+                'frac_of_nominal': MERTON_VALU_WAVE_INSTS / (m_ms * 1e-3)
+                / VALU_ISSUE_PEAK_PER_S}}
         del mrun, m64
         # the preset forward (k_euler, jumpdiff) on the same configuration:
         # the terminal row only (jump sizes drawn at the stored row,
@@ -583,8 +596,9 @@ def main():
                 'kernel': 'k_batch_persistent', 'bound': 'valu',
                 'wave_insts_per_launch': BATCH_VALU_WAVE_INSTS,
                 'achieved_per_s': BATCH_VALU_WAVE_INSTS / (bms * 1e-3),
-                'peak_per_s': VALU_ISSUE_PEAK_PER_S,
-                'frac': BATCH_VALU_WAVE_INSTS / (bms * 1e-3) / VALU_ISSUE_PEAK_PER_S}}
+                'nominal_per_s': VALU_ISSUE_PEAK_PER_S,
+                'frac_of_nominal': BATCH_VALU_WAVE_INSTS / (bms * 1e-3)
+                / VALU_ISSUE_PEAK_PER_S}}
         del bt
         torch.cuda.empty_cache()
 
