@@ -339,6 +339,46 @@ def test_merton_deferred_and_in_place_jumps(gpu):
     assert _rel(out['sparse'], out['dense'][::n // 2]) < 1e-12
 
 
+@pytest.mark.parametrize('P', [1, 63, 65])
+def test_partial_waves_match_oracle(gpu, P):
+    """Path counts that leave lanes of the last wave without a path (they
+    return at once): Merton with its jump info (the per-wave ballot sum of
+    the interval counts) in both jump forms, Heston and correlated OU, FP64."""
+    for tt, n in (((0., 1.), 41), (np.linspace(0., 1., 21), 21)):
+        info, ref_info = {}, {}
+        p = sa.merton_jumpdiff_process(paths=P, dtype=np.float64, steps=n, info=info,
+                                       x0=1., mu=.03, sigma=.2, lam=9., a=-.05, b=.1,
+                                       rng=np.random.default_rng(81))
+        kw, kj = p.sources['dw'].key, p.sources['dj'].key
+        x = p(tt)
+        ref = ora.integrate('jumpdiff', tt, paths=P, steps=n, info=ref_info,
+                            sources={'dw': ora.PhiloxWiener(kw, 0, P, ()),
+                                     'dj': ora.PhiloxCPoisson(kj, 0, P, (), lam=9., a=-.05,
+                                                              b=.1)},
+                            x0=1., mu=.03, sigma=.2)
+        assert _rel(x, ref) < RTOL64
+        assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+        assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12)
+    args = dict(x0=100., mu=.02, sigma=1., y0=.04, theta=.04, k=2., xi=.5)
+    h = sa.full_heston_process(paths=P, dtype=np.float64, steps=30, rho=-.7,
+                               rng=np.random.default_rng(82), **args)
+    key = h.sources['dw'].key
+    hx, hy = h((0., .5, 1.))
+    rx, ry = ora.integrate('full_heston', (0., .5, 1.), paths=P, steps=30,
+                           sources={'dw': ora.PhiloxWiener(key, 0, P, (2,), rho=-.7)},
+                           **args)
+    assert _rel(hx, rx) < 1e-10 and np.max(np.abs(hy - ry)) < 1e-12
+    C = np.array([[1., .3, -.2], [.3, 1., .1], [-.2, .1, 1.]])
+    o = sa.ornstein_uhlenbeck_process(paths=P, vshape=3, dtype=np.float64, corr=C,
+                                      theta=.1, k=2., sigma=.3, rng=np.random.default_rng(83))
+    key = o.sources['dw'].key
+    y = o(np.linspace(0., 1., 6))
+    ry = ora.integrate('ou', np.linspace(0., 1., 6), paths=P, vshape=(3,),
+                       sources={'dw': ora.PhiloxWiener(key, 0, P, (3,), corr=C)},
+                       theta=.1, k=2., sigma=.3)
+    assert np.max(np.abs(y - ry)) < 1e-12
+
+
 def test_merton_deferred_jumps_backward_pass(gpu):
     """i0 = 1 on (0, .5, 1): a backward pass to t = 0 and a forward pass to
     t = 1, each storing one row over 20 steps, so both defer their jump
