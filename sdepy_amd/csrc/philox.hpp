@@ -137,8 +137,10 @@ struct Gauss<double> {
     const double ua = u01_f64(a);
     const double ub = u01_f64(b);
     const double r = sqrt(-2.0 * log(ua));
+    // the angle 2 pi ub in half-turns: 2 ub is exact and needs no reduction
+    // by pi (within an ulp of cos/sin of the rounded product 2 pi * ub)
     double s, c;
-    sincos(6.283185307179586 * ub, &s, &c);
+    sincospi(2.0 * ub, &s, &c);
     z0 = r * c;
     z1 = r * s;
   }
