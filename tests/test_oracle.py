@@ -278,6 +278,256 @@ def test_merton_reductions():
 
 
 # ---------------------------------------------------------------------------
+# backward pass (i0 != 0, integration.py:573-580)
+# ---------------------------------------------------------------------------
+
+class _Recorder(ora._PhiloxSource):
+    """Source-protocol wrapper that records every call (t, dt, increment,
+    the wrapped source's dn_value) in call order and forwards begin_pass."""
+
+    def __init__(self, inner):
+        self.inner = inner
+        self.paths, self.vshape = inner.paths, inner.vshape
+        self.calls = []
+
+    def begin_pass(self, tag):
+        if isinstance(self.inner, ora._PhiloxSource):
+            self.inner.begin_pass(tag)
+
+    @property
+    def dn_value(self):
+        return getattr(self.inner, 'dn_value', None)
+
+    def __call__(self, t, dt):
+        z = self.inner(t, dt)
+        dn = getattr(self.inner, 'dn_value', None)
+        self.calls.append((t, dt, np.array(z), None if dn is None
+                           else np.array(dn)))
+        return z
+
+
+def _step_grid(t, steps, i0):
+    """The consolidated integration grid and the position of t[i0] in it,
+    computed here from the definition (the union of the timeline with
+    linspace(t0, tN, steps)), not through the oracle's consolidate()."""
+    t = np.asarray(t, float)
+    grid = t if steps is None else np.union1d(
+        t, np.linspace(t[0], t[-1], steps))
+    j0 = int(np.searchsorted(grid, t[i0]))
+    assert grid[j0] == t[i0]
+    return grid, j0
+
+
+def _calls_to_row(grid, j0, t_row):
+    """Number of steps the loop takes from t[i0] to t_row (either side)."""
+    return abs(int(np.searchsorted(grid, t_row)) - j0)
+
+
+@pytest.mark.parametrize('i0', [3, -2, 7])
+@pytest.mark.parametrize('steps', [None, 37])
+def test_backward_pass_one_plus_shifted_t(i0, steps):
+    """x += ds from x(t[i0]) = 1 gives 1 + (t - t[i0]) on every row, on both
+    sides of t[i0] (sdepy runs the loop backwards with negative ds down to
+    t[0], then forwards; integration.py:573-580).  A wrong sign of ds on the
+    backward side, a shifted start row or a mis-ordered store fails."""
+    t = np.array([0., .3, .7, 1.2, 2., 2.5, 3.1, 4., 5.5])
+    info = {}
+    zero = ora.ReplaySource(np.zeros((400, 1)), 1, ())
+    x = ora.integrate('wiener', t, x0=1., mu=1., sigma=0., steps=steps,
+                      i0=i0, info=info, sources={'dw': zero})
+    assert np.allclose(x[:, 0], 1. + (t - t[i0]), rtol=0, atol=1e-13)
+    grid, j0 = _step_grid(t, steps, i0)
+    back = j0 if i0 % t.size != 0 else 0
+    fwd = (grid.size - 1 - j0) if i0 % t.size != t.size - 1 else 0
+    assert info['computed_steps'] == back + fwd
+    assert info['t0'] == t[i0]
+
+
+@pytest.mark.parametrize('i0,steps', [(4, None), (-3, 29), (8, 17)])
+def test_backward_pass_log_euler_exact(i0, steps):
+    """Log-Euler lognorm with constant parameters is exact on both sides of
+    t[i0]: x(t_j) = x0 exp((mu - s^2/2)(t_j - t[i0]) + s * sum of the dw the
+    loop drew between t[i0] and t_j), whatever the sign of the steps.  The
+    increments are replayed in call order (backward pass first)."""
+    P = 23
+    t = np.array([.5, .9, 1.3, 1.4, 2.2, 2.9, 3.5, 3.6, 4.8])
+    grid, j0 = _step_grid(t, steps, i0)
+    n_back = j0 if i0 % t.size != 0 else 0
+    n_fwd = grid.size - 1 - j0 if i0 % t.size != t.size - 1 else 0
+    rng = np.random.default_rng(11)
+    inc = rng.normal(size=(n_back + n_fwd, P))
+    mu, s, x0 = .15, .4, 3.
+    x = ora.integrate('lognorm', t, paths=P, x0=x0, mu=mu, sigma=s,
+                      steps=steps, i0=i0,
+                      sources={'dw': ora.ReplaySource(inc, P, ())})
+    for j, tj in enumerate(t):
+        k = _calls_to_row(grid, j0, tj)
+        used = inc[:k] if tj <= t[i0] else inc[n_back:n_back + k]
+        exact = x0 * np.exp((mu - s * s / 2) * (tj - t[i0]) + s * used.sum(axis=0))
+        assert np.allclose(x[j], exact, rtol=1e-13), j
+
+
+@pytest.mark.parametrize('i0', [5, -1])
+def test_backward_pass_merton_b0_jump_counts(i0):
+    """Merton with b = 0 on both sides of t[i0]: every jump is exactly a and
+    the compound-Poisson source returns sign(dt) * n * a with
+    dn_value = sign(dt) * n (infrastructure.py:2017-2040, :1617-1633), so
+    x(t_j) = x0 exp((mu - s^2/2)(t_j - t[i0]) + s sum dw + a sum dn_value)
+    over the calls between t[i0] and t_j; jump_count is the signed total."""
+    P, lam, a = 400, 3., -.25
+    t = np.linspace(0., 2., 11)
+    mu, s, x0 = .05, .3, 2.
+    dw = _Recorder(ora.PhiloxWiener((7, 9), 0, P, ()))
+    dj = _Recorder(ora.PhiloxCPoisson((8, 10), 0, P, (), lam=lam, a=a, b=0.))
+    info = {}
+    x = ora.integrate('jumpdiff', t, paths=P, x0=x0, mu=mu, sigma=s, i0=i0,
+                      steps=41, info=info, sources={'dw': dw, 'dj': dj})
+    assert len(dw.calls) == len(dj.calls)
+    back = [c for c in dj.calls if c[1] < 0]
+    assert len(back) > 0
+    for _, dt, z, dn in dj.calls:
+        assert np.array_equal(np.sign(dn[dn != 0]), np.full((dn != 0).sum(),
+                                                           int(np.sign(dt))))
+        assert np.allclose(z, a * dn, rtol=1e-15, atol=0)
+    n_jumps = sum(int(np.abs(c[3]).sum()) for c in back)
+    assert n_jumps > 100
+    grid, j0 = _step_grid(t, 41, i0)
+    nb = sum(1 for c in dw.calls if c[1] < 0)
+    for j, tj in enumerate(t):
+        k = _calls_to_row(grid, j0, tj)
+        sl = slice(0, k) if tj <= t[i0] else slice(nb, nb + k)
+        w = sum((c[2] for c in dw.calls[sl]), np.zeros(P))
+        n = sum((c[3] for c in dj.calls[sl]), np.zeros(P, int))
+        exact = x0 * np.exp((mu - s * s / 2) * (tj - t[i0]) + s * w + a * n)
+        assert np.allclose(x[j], exact, rtol=1e-12), j
+    # sdepy resets the info at the start of each pass (SDE.begin ->
+    # jumpdiff_SDE.info_begin, integration.py:1166-1167, :2587-2591): the
+    # returned counts and rates are those of the last pass run
+    last = [c for c in dj.calls if (c[1] > 0) == (i0 % t.size != t.size - 1)]
+    total = sum((c[3] for c in last), np.zeros(P, int))
+    assert np.array_equal(info['jump_count'], total)
+    tp = t[i0:] if i0 % t.size != t.size - 1 else t[i0::-1]
+    assert info['jump_rate'].shape == tp.shape
+    rows = [_calls_to_row(grid, j0, tj) for tj in tp]
+    for r in range(1, tp.size):
+        dn = sum((c[3] for c in last[rows[r - 1]:rows[r]]), np.zeros(P, int))
+        want = dn.sum() / (tp[r] - tp[r - 1]) / P
+        assert abs(info['jump_rate'][r - 1] - want) <= 1e-12 * abs(want)
+    assert info['jump_rate'][-1] == info['jump_rate'][-2]
+
+
+# ---------------------------------------------------------------------------
+# SDEs packing with leading vshape axes (integration.py:1735-1783)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize('vshape', [(2, 2), (2, 1, 2), (3,)])
+def test_full_heston_packing_closed_form(vshape):
+    """full_heston with vshape != () packs its working state along the last
+    axis as [log x_0..log x_{d-1}, y_0..y_{d-1}] (SDEs.pack, addaxis=False,
+    integration.py:1759-1783).  With y0 = theta and xi = 0 each variance stays
+    at its own theta, so each log x component is an exact lognormal driven by
+    the dw component at the same position of the first half of the packed
+    axis: x = x0 exp((mu - s^2 theta/2) T + s sqrt(theta) sum dw[..., j]).
+    Distinct per-component x0, mu, sigma, theta make an interleaved
+    [x0, y0, x1, y1] layout, a swapped half or a transposed leading axis
+    fail."""
+    P, n = 7, 12
+    d = vshape[-1]
+    wshape = vshape[:-1] + (2 * d,)
+    rng = np.random.default_rng(5)
+    pshape = vshape + (1,)
+    x0 = rng.uniform(1., 5., pshape)
+    mu = rng.uniform(-.1, .2, pshape)
+    sig = rng.uniform(.2, 1.5, pshape)
+    theta = rng.uniform(.01, .2, pshape)
+    k = rng.uniform(.5, 3., pshape)
+    dt = 1. / n
+    inc = rng.normal(size=(n,) + wshape + (P,)) * math.sqrt(dt)
+    x, y = ora.integrate('full_heston', (0., 1.), paths=P, vshape=vshape,
+                         steps=n + 1, x0=x0, y0=theta, mu=mu, sigma=sig,
+                         theta=theta, k=k, xi=0.,
+                         sources={'dw': ora.ReplaySource(inc, P, wshape)})
+    assert x.shape == (2,) + vshape + (P,) and y.shape == x.shape
+    assert np.array_equal(y, np.broadcast_to(theta, y.shape))
+    wx = inc[..., :d, :].sum(axis=0)
+    exact = x0 * np.exp((mu - sig * sig * theta / 2) + sig * np.sqrt(theta) * wx)
+    assert np.allclose(x[-1], exact, rtol=1e-13)
+    # heston_SDE (integration.py:2517-2541) stores the x half only
+    xh = ora.integrate('heston', (0., 1.), paths=P, vshape=vshape, steps=n + 1,
+                       x0=x0, y0=theta, mu=mu, sigma=sig, theta=theta, k=k,
+                       xi=0., sources={'dw': ora.ReplaySource(inc, P, wshape)})
+    assert np.array_equal(xh, x)
+
+
+@pytest.mark.parametrize('vshape', [(2, 2), (2, 1, 2)])
+def test_full_heston_packing_variance_half(vshape):
+    """The variance half of the packed axis: with sigma = 0 the log price is
+    deterministic (log x0 + mu t), and each variance y_j follows the scalar
+    full-truncation recursion y' = y + k(theta - y+) dt + xi sqrt(y+) dw
+    driven by dw[..., d + j] only.  Brute force: a scalar Python loop per
+    component on 3 paths x 9 steps, written from the scalar SDE
+    (integration.py:2425-2433), against the oracle's packed arrays."""
+    P, n = 3, 9
+    d = vshape[-1]
+    wshape = vshape[:-1] + (2 * d,)
+    rng = np.random.default_rng(8)
+    pshape = vshape + (1,)
+    y0 = rng.uniform(.02, .1, pshape)
+    theta = rng.uniform(.02, .1, pshape)
+    k = rng.uniform(.5, 3., pshape)
+    xi = rng.uniform(.3, .9, pshape)
+    dt = 1. / n
+    inc = rng.normal(size=(n,) + wshape + (P,)) * math.sqrt(dt)
+    x, y = ora.integrate('full_heston', (0., 1.), paths=P, vshape=vshape,
+                         steps=n + 1, x0=2., y0=y0, mu=.07, sigma=0.,
+                         theta=theta, k=k, xi=xi,
+                         sources={'dw': ora.ReplaySource(inc, P, wshape)})
+    assert np.allclose(x[-1], 2. * math.exp(.07), rtol=1e-14)
+    for lead in np.ndindex(vshape[:-1]):
+        for j in range(d):
+            ix = lead + (j,)
+            c = ix + (0,)
+            for p in range(P):
+                yy = float(y0[c])
+                for s in range(n):
+                    yp = max(yy, 0.)
+                    yy = yy + (float(k[c]) * (float(theta[c]) - yp) * dt +
+                               float(xi[c]) * math.sqrt(yp) *
+                               float(inc[(s,) + lead + (d + j, p)]))
+                assert abs(y[(-1,) + ix + (p,)] - yy) < 1e-14
+
+
+def test_hull_white_factor_layout_closed_forms():
+    """hull_white_SDE (integration.py:2247-2272): wshape = vshape + (factors,)
+    and the stored value sums the factor axis.  sigma = 0: the sum over
+    factors of the geometric recursions theta + (x0 - theta)(1 - k dt)^n with
+    per-(component, factor) parameters.  k = 0: x = sum_f (x0_f + sigma_f
+    sum dw[..., v, f]).  A transposed (factor, component) layout fails both."""
+    P, n, F = 4, 20, 2
+    vshape = (5,)
+    rng = np.random.default_rng(9)
+    pshape = vshape + (F, 1)
+    x0 = rng.uniform(-1., 1., pshape)
+    theta = rng.uniform(-.5, .5, pshape)
+    k = rng.uniform(.1, 4., pshape)
+    sig = rng.uniform(.1, .9, pshape)
+    dt = 1. / n
+    zero = ora.ReplaySource(np.zeros((n,) + vshape + (F, P)), P, vshape + (F,))
+    x = ora.integrate('hull_white', (0., 1.), paths=P, vshape=vshape,
+                      factors=F, steps=n + 1, x0=x0, theta=theta, k=k,
+                      sigma=0., sources={'dw': zero})
+    closed = (theta + (x0 - theta) * (1 - k * dt) ** n).sum(axis=-2)
+    assert np.allclose(x[-1], np.broadcast_to(closed, (5, P)), rtol=1e-13)
+    inc = rng.normal(size=(n,) + vshape + (F, P)) * math.sqrt(dt)
+    x = ora.integrate('hull_white', (0., 1.), paths=P, vshape=vshape,
+                      factors=F, steps=n + 1, x0=x0, theta=theta, k=0.,
+                      sigma=sig, sources={'dw': ora.ReplaySource(
+                          inc, P, vshape + (F,))})
+    exact = (x0 + sig * inc.sum(axis=0)).sum(axis=-2)
+    assert np.allclose(x[-1], exact, rtol=1e-13, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
 # closed forms
 # ---------------------------------------------------------------------------
 
@@ -507,3 +757,63 @@ def test_montecarlo_oracle_cumulation_and_moments():
     odd.update(t, axis=-1)
     assert odd.paths == 30
     assert np.allclose(odd.mean, (t[:, :30] - t[:, 30:]).mean(axis=-1) / 2, rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# the pins above catch the mistakes they are meant to catch
+# ---------------------------------------------------------------------------
+
+def _mutated_run_pass(old, new):
+    import inspect
+    src = inspect.getsource(ora._run_pass)
+    assert old in src
+    ns = dict(vars(ora))
+    exec(compile(src.replace(old, new), ora.__file__, 'exec'), ns)
+    return ns['_run_pass']
+
+
+def _interleaved_unpack(self, X):
+    if self.addaxis:
+        return tuple(X[..., k, :] for k in range(2))
+    return tuple(X[..., k::2, :] for k in range(2))
+
+
+def _interleaved_pack(self, xs, paths):
+    target = self.vshape + (paths,)
+    if self.addaxis:
+        return np.concatenate(tuple(np.broadcast_to(x, target)[..., None, :]
+                                    for x in xs), axis=-2)
+    a, b = (np.broadcast_to(x, target) for x in xs)
+    return np.stack((a, b), axis=-2).reshape(
+        self.vshape[:-1] + (2 * self.vshape[-1], paths))
+
+
+@pytest.mark.parametrize('mistake', ['abs_ds', 'interleaved_pack',
+                                     'backward_counts_unsigned'])
+def test_planted_oracle_mistakes_fail_the_pins(monkeypatch, mistake):
+    """Plant a plausible mistake in the oracle (on the CPU) and check that
+    the closed-form pins above fail: a backward pass stepping with |ds|, the
+    [x0, y0, x1, y1] packing in place of [x0, x1, y0, y1], and unsigned jump
+    counts on the backward side."""
+    if mistake == 'abs_ds':
+        monkeypatch.setattr(ora, '_run_pass', _mutated_run_pass(
+            'ds = s_next - s', 'ds = abs(s_next - s)'))
+        checks = [lambda: test_backward_pass_one_plus_shifted_t(3, None),
+                  lambda: test_backward_pass_log_euler_exact(4, None)]
+    elif mistake == 'interleaved_pack':
+        monkeypatch.setattr(ora.FullHeston, 'unpack', _interleaved_unpack)
+        monkeypatch.setattr(ora.FullHeston, 'pack', _interleaved_pack)
+        checks = [lambda: test_full_heston_packing_closed_form((2, 2)),
+                  lambda: test_full_heston_packing_variance_half((2, 1, 2))]
+    else:
+        orig = ora.PhiloxCPoisson.__call__
+
+        def unsigned(self, t, dt):
+            z = orig(self, t, dt)
+            self.dn_value = np.abs(self.dn_value)
+            return np.abs(z) * np.sign(self.a) if self.b == 0 else z
+        monkeypatch.setattr(ora.PhiloxCPoisson, '__call__', unsigned)
+        checks = [lambda: test_backward_pass_merton_b0_jump_counts(5)]
+    for check in checks:
+        with pytest.raises((AssertionError, ValueError)):
+            check()
