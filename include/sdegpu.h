@@ -207,6 +207,20 @@ int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t path
                         double* m2, void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * sdegpu_path_moments without a workspace: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The chunk
+ * partials live in a 4 MiB device array of the library that each GPU holds
+ * until the process ends, which limits a call to
+ * rows * ceil(paths / 65536) <= 262144 (beyond that: SDEGPU_ENOMEM, and the
+ * workspace form serves).  Results equal sdegpu_path_moments' bit for bit.
+ * All calls on one GPU share its array: calls that overlap on one GPU (on
+ * two streams or from two threads) race.
+ */
+int sdegpu_path_moments_noalloc(const void* x, int32_t dtype, int64_t rows, int64_t paths,
+                                double* mean, double* m2, void* stream);
+
+/*
  * FP64 sums of a European payoff of the values x[paths]:
  *   kind 0: scale*max(x-strike, 0), 1: scale*max(strike-x, 0), 2: scale*x.
  * out (device double[2]) = (sum, sum of squares).  Replaces the

@@ -7,6 +7,8 @@
 // by the row's first element, and a second kernel adds the chunk partials of
 // each row in chunk order.  No atomics: the result does not depend on
 // scheduling.
+#include <mutex>
+
 #include "common.hpp"
 #include "span.hpp"
 
@@ -131,21 +133,14 @@ extern "C" size_t sdegpu_moments_workspace(int64_t rows, int64_t paths) {
   return (size_t)(rows * nchunks * 2 * sizeof(double));
 }
 
-extern "C" int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths,
-                                   double* mean, double* m2, void* ws, size_t ws_bytes,
-                                   void* stream) {
-  if (rows < 0 || paths < 1) return fail(SDEGPU_EINVAL, "path_moments: rows=%lld paths=%lld",
-                                         (long long)rows, (long long)paths);
-  if (rows == 0) return SDEGPU_OK;
-  if (!x || !mean || !m2) return fail(SDEGPU_EINVAL, "path_moments: null pointer");
+namespace sdegpu {
+
+// Both moments entry points: validation done, `part` holds at least
+// rows * nchunks * 2 doubles.
+static int launch_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths,
+                          double* mean, double* m2, double* part, hipStream_t st) {
   const int64_t nchunks = (paths + kChunk - 1) / kChunk;
   const int64_t nblocks = rows * nchunks;
-  if (nblocks > 0x7FFFFFFFll) return fail(SDEGPU_EINVAL, "path_moments: too many rows");
-  if (ws_bytes < sdegpu_moments_workspace(rows, paths) || !ws)
-    return fail(SDEGPU_ENOMEM, "path_moments: workspace of %zu bytes, %zu needed", ws_bytes,
-                sdegpu_moments_workspace(rows, paths));
-  hipStream_t st = (hipStream_t)stream;
-  double* part = (double*)ws;
   const unsigned gfin = blocks_for(rows, 64);
   if (dtype == SDEGPU_F32) {
     hipLaunchKernelGGL(k_moments_partial<float>, dim3((unsigned)nblocks), dim3(kBlock), 0, st,
@@ -153,16 +148,103 @@ extern "C" int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, i
     if (int e = launch_status("k_moments_partial")) return e;
     hipLaunchKernelGGL(k_moments_final<float>, dim3(gfin), dim3(64), 0, st, (const float*)x, rows,
                        paths, nchunks, (const double*)part, mean, m2);
-  } else if (dtype == SDEGPU_F64) {
+  } else {
     hipLaunchKernelGGL(k_moments_partial<double>, dim3((unsigned)nblocks), dim3(kBlock), 0, st,
                        (const double*)x, paths, nchunks, part);
     if (int e = launch_status("k_moments_partial")) return e;
     hipLaunchKernelGGL(k_moments_final<double>, dim3(gfin), dim3(64), 0, st, (const double*)x,
                        rows, paths, nchunks, (const double*)part, mean, m2);
-  } else {
-    return fail(SDEGPU_EUNSUPPORTED, "path_moments: dtype code %d", dtype);
   }
   return launch_status("k_moments_final");
+}
+
+static int check_moments_args(const void* x, int32_t dtype, int64_t rows, int64_t paths,
+                              const double* mean, const double* m2) {
+  if (rows < 0 || paths < 1)
+    return fail(SDEGPU_EINVAL, "path_moments: rows=%lld paths=%lld", (long long)rows,
+                (long long)paths);
+  if (rows == 0) return SDEGPU_OK;
+  if (!x || !mean || !m2) return fail(SDEGPU_EINVAL, "path_moments: null pointer");
+  if (dtype != SDEGPU_F32 && dtype != SDEGPU_F64)
+    return fail(SDEGPU_EUNSUPPORTED, "path_moments: dtype code %d", dtype);
+  const int64_t nchunks = (paths + kChunk - 1) / kChunk;
+  if (rows > 0x7FFFFFFFll / nchunks) return fail(SDEGPU_EINVAL, "path_moments: too many rows");
+  return SDEGPU_OK;
+}
+
+}  // namespace sdegpu
+
+extern "C" int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t paths,
+                                   double* mean, double* m2, void* ws, size_t ws_bytes,
+                                   void* stream) {
+  if (int e = check_moments_args(x, dtype, rows, paths, mean, m2)) return e;
+  if (rows == 0) return SDEGPU_OK;
+  if (ws_bytes < sdegpu_moments_workspace(rows, paths) || !ws)
+    return fail(SDEGPU_ENOMEM, "path_moments: workspace of %zu bytes, %zu needed", ws_bytes,
+                sdegpu_moments_workspace(rows, paths));
+  return launch_moments(x, dtype, rows, paths, mean, m2, (double*)ws, (hipStream_t)stream);
+}
+
+// ---- allocation-free form ------------------------------------------------
+//
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops, harnesses that forbid hipMalloc): the chunk
+// partials live in `scratch`, a device array of the library that each GPU
+// holds, at its own address, until the process ends.  In host code the
+// array's name only stands in for the symbol: its device address comes from
+// hipGetSymbolAddress, asked once per device (with that device current) on
+// the first call there, and only that address is passed to kernels.
+// Every call on one GPU shares that GPU's array; k_moments_partial writes
+// every slot the merge reads, so nothing needs clearing between calls, but
+// calls that overlap on one GPU (two streams, two threads) race.
+
+namespace sdegpu {
+
+constexpr size_t kScratchBytes = size_t(4) << 20;  // 4 MiB per GPU
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kScratchDevices];
+
+// The device address of the current GPU's `scratch`.
+static int scratch_address(double** out) {
+  int dev = -1;
+  if (int e = hip_status(hipGetDevice(&dev), "hipGetDevice")) return e;
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail(SDEGPU_EUNSUPPORTED, "path_moments_noalloc: device %d beyond the %d supported",
+                dev, kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (int e = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                           "hipGetSymbolAddress(scratch)"))
+      return e;
+    if (!p) return fail(SDEGPU_ERUNTIME, "path_moments_noalloc: no address for scratch");
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return SDEGPU_OK;
+}
+
+}  // namespace sdegpu
+
+extern "C" int sdegpu_path_moments_noalloc(const void* x, int32_t dtype, int64_t rows,
+                                           int64_t paths, double* mean, double* m2,
+                                           void* stream) {
+  if (int e = check_moments_args(x, dtype, rows, paths, mean, m2)) return e;
+  if (rows == 0) return SDEGPU_OK;
+  const size_t need = sdegpu_moments_workspace(rows, paths);
+  if (need > kScratchBytes)
+    return fail(SDEGPU_ENOMEM,
+                "path_moments_noalloc: %lld rows x %lld paths need %zu bytes of partials, beyond "
+                "the %zu-byte scratch (rows * ceil(paths / 65536) <= %zu); use "
+                "sdegpu_path_moments with a workspace",
+                (long long)rows, (long long)paths, need, kScratchBytes,
+                kScratchBytes / (2 * sizeof(double)));
+  double* part = nullptr;
+  if (int e = scratch_address(&part)) return e;
+  return launch_moments(x, dtype, rows, paths, mean, m2, part, (hipStream_t)stream);
 }
 
 extern "C" size_t sdegpu_payoff_workspace(int64_t paths) {
