@@ -138,7 +138,7 @@ def cpu_baseline(paths, steps, seed=1234):
             'sample': ('{} paths x {} steps, FP64, price + 7 Greeks by 15 CRN '
                        'forward runs of the oracle, {:.1f} s').format(
                            paths, steps, dt),
-            'host_cpu': _cpu_name(),
+            'host_cpu': _cpu_info(),
             'forward_path_steps_per_s': 15 * paths * steps / dt}
 
 
@@ -181,7 +181,8 @@ def cpu_baseline_parallel(paths, steps, workers):
             'sample': ('{} processes x {} paths x {} steps, FP64, price + 7 Greeks '
                        'by 15 CRN forward runs each, {:.1f} s wall including '
                        'process start').format(workers, paths, steps, wall),
-            'per_process_value': [r['value'] for r in res]}
+            'per_process_value': [r['value'] for r in res],
+            'host_cpu': _cpu_info()}
 
 
 def _device_summaries(torch, run):
@@ -288,6 +289,32 @@ def _cpu_name():
     except OSError:
         pass
     return 'unknown'
+
+
+def _cpu_info():
+    """The host CPU as lscpu describes it, and the CPU counts this process
+    sees (os.cpu_count and its affinity mask)."""
+    import subprocess
+    info = {'model': _cpu_name(), 'os_cpu_count': os.cpu_count()}
+    try:
+        info['affinity_cpus'] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True,
+                             timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {'Model name': 'lscpu_model', 'Vendor ID': 'vendor',
+            'CPU(s)': 'lscpu_cpus', 'Socket(s)': 'sockets',
+            'Core(s) per socket': 'cores_per_socket',
+            'Thread(s) per core': 'threads_per_core',
+            'CPU max MHz': 'max_mhz', 'Hypervisor vendor': 'hypervisor'}
+    for line in out.splitlines():
+        k, _, v = line.partition(':')
+        if k.strip() in keys and v.strip():
+            info[keys[k.strip()]] = v.strip()
+    return info
 
 
 def main():

@@ -11,6 +11,10 @@ Each instance draws exactly the Philox numbers its own ``process(timeline)``
 call would draw (same key and epoch), so a batched result equals the
 stand-alone one up to the order of the FP64 sums.
 
+A prepared batch is a fixed computation: every ``run()`` replays the same
+Philox epochs and returns the same numbers (build a new batch from the
+processes for fresh draws).  Only float32 processes are accepted.
+
 The persistent grid size is chosen by a small autotuner: the first run of a
 batch shape times each candidate (4..32 wavefronts per CU) and the winner is
 cached on disk (``SDEPY_AMD_TUNE_CACHE``, default
@@ -26,7 +30,7 @@ import time
 import numpy as np
 
 from . import _lib
-from .infrastructure import wiener_source
+from .infrastructure import wiener_source, _gpu_dtype
 from .integration import (consolidate_timeline, lognorm_process,
                           wiener_process, ornstein_uhlenbeck_process,
                           hull_white_process)
@@ -94,6 +98,11 @@ def _instance(proc, timeline):
             'hull_white processes, not {}'.format(type(proc).__name__))
     if proc.vshape != ():
         raise NotImplementedError('batched simulations take vshape == ()')
+    if _gpu_dtype(proc.dtype)[0] != _lib.F32:
+        raise NotImplementedError(
+            'batched simulations step float32 processes, not {} (give the '
+            'process dtype=np.float32, or run it on its own)'.format(
+                np.dtype(float if proc.dtype is None else proc.dtype).name))
     M = proc._nvars()
     if M > 4 or (model != _lib.MODEL_OU and M != 1):
         raise NotImplementedError('batched simulations take at most 4 OU '

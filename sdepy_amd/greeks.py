@@ -244,6 +244,29 @@ def merton_greeks(**kw):
 # FP64 forward + finite differences (common random numbers)
 # --------------------------------------------------------------------------
 
+def fd_points(base, names, h):
+    """Parameter sets of central differences: the base point, then for each
+    name the point bumped up and the point bumped down by h relative to the
+    parameter (by h itself where the parameter is 0, so that r = 0 or
+    rho = 0 still differentiate).  Also returns, per name, the difference of
+    the two bumped values as they are rounded, the divisor of the
+    difference quotient."""
+    pts, spans = [dict(base)], {}
+    for name in names:
+        step = h * abs(base[name]) if base[name] != 0 else h
+        up, dn = dict(base), dict(base)
+        up[name], dn[name] = base[name] + step, base[name] - step
+        spans[name] = up[name] - dn[name]
+        pts += [up, dn]
+    return pts, spans
+
+
+def fd_quotients(prices, names, spans):
+    """Central difference quotients from prices ordered as fd_points' sets."""
+    return {name: (prices[1 + 2 * j] - prices[2 + 2 * j]) / spans[name]
+            for j, name in enumerate(names)}
+
+
 class HestonFD:
     """Prepared FP64 (or FP32) forward runs of full_heston_process for the
     base point and the +-h bumps of each parameter, all on the same Philox
@@ -263,12 +286,7 @@ class HestonFD:
         key = philox_key(_get_default_rng() if rng is None else rng) \
             if key is None else key
         self.key = tuple(key)
-        pts = [dict(base)]
-        for name in (HESTON_GREEKS if bumps else ()):
-            for sgn in (1, -1):
-                d = dict(base)
-                d[name] = base[name] * (1 + sgn * h)
-                pts.append(d)
+        pts, self.spans = fd_points(base, HESTON_GREEKS if bumps else (), h)
         self.points = pts
         self.kind = _payoff_code(payoff)
         self.strike, self.maturity = strike, maturity
@@ -306,11 +324,8 @@ class HestonFD:
         prices = sums[:, 0] / n
         price = prices[0]
         price_se = math.sqrt(max(sums[0, 1] / n - price ** 2, 0.) / (n - 1))
-        greeks = {}
-        for j, name in enumerate(HESTON_GREEKS if len(self.points) > 1 else ()):
-            up, dn = prices[1 + 2 * j], prices[2 + 2 * j]
-            greeks[name] = (up - dn) / (2 * self.h * self.base[name])
-        return price, price_se, greeks
+        names = HESTON_GREEKS if len(self.points) > 1 else ()
+        return price, price_se, fd_quotients(prices, names, self.spans)
 
     @property
     def path_steps(self):
@@ -345,6 +360,11 @@ def _autograd_function():
             vals = [float(v) for v in params.detach().cpu().tolist()]
             kw = dict(config)
             kw.update(zip(HESTON_GREEKS, vals))
+            # one Philox key for either path below, so the price does not
+            # depend on whether a gradient is asked for
+            rng = kw.pop('rng', None)
+            if kw.get('key') is None:
+                kw['key'] = philox_key(_get_default_rng() if rng is None else rng)
             if params.requires_grad:
                 res = heston_greeks(**kw)
                 g = torch.tensor([res.greeks[n] for n in HESTON_GREEKS],
@@ -374,7 +394,60 @@ def _heston_price_only(kw, dtype):
     return price, se, None
 
 
+def _loglin_autograd_function():
+    torch = _lib.torch_mod()
+
+    class LogLinPrice(torch.autograd.Function):
+        """price = LogLinPrice.apply(params, config): params is a tensor of
+        (x0, mu, sigma, a, b) of merton_jumpdiff_process (lognorm_process
+        when config's lam is 0); config a dict of the remaining LogGreeks
+        keywords (lam, r, strike, maturity, steps, paths, dtype, payoff,
+        key_w, key_j, epoch, rng).  Price and pathwise gradient come from
+        one launch of the log-space adjoint kernel, whether or not a
+        gradient is asked for; backward scales the gradient by the incoming
+        one."""
+
+        @staticmethod
+        def forward(ctx, params, config):
+            vals = [float(v) for v in params.detach().cpu().tolist()]
+            kw = dict(config)
+            kw.update(zip(LOG_GREEKS, vals))
+            res = merton_greeks(**kw)
+            g = torch.tensor([res.greeks[n] for n in LOG_GREEKS],
+                             dtype=params.dtype, device=params.device)
+            ctx.save_for_backward(g)
+            return torch.tensor(res.price, dtype=params.dtype, device=params.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            g, = ctx.saved_tensors
+            return grad_out * g, None
+
+    return LogLinPrice
+
+
 _HP = None
+_LP = None
+
+
+def loglin_price(params, **config):
+    """Differentiable European option price on merton_jumpdiff_process
+    paths (lognorm_process with lam=0): ``params`` is a torch tensor of
+    (x0, mu, sigma, a, b); ``config`` the other LogGreeks keywords.
+    ``loglin_price(p, ...).backward()`` fills ``p.grad`` with the pathwise
+    Greeks of the adjoint kernel.  Without ``key_w``/``key_j`` the keys are
+    drawn from ``rng`` (or the default generator) once per call."""
+    global _LP
+    if _LP is None:
+        _LP = _loglin_autograd_function()
+    cfg = dict(config)
+    rng = cfg.pop('rng', None)
+    rng = _get_default_rng() if rng is None else rng
+    if cfg.get('key_w') is None:
+        cfg['key_w'] = philox_key(rng)
+    if cfg.get('key_j') is None:
+        cfg['key_j'] = philox_key(rng)
+    return _LP.apply(params, cfg)
 
 
 def heston_price(params, **config):

@@ -149,3 +149,57 @@ def test_lognorm_delta_matches_black_scholes(gpu):
     d = cf.bscall_delta(1.1, 2., 1., .05, 0., .3)
     assert abs(res.price - bs) < 3 * res.price_se
     assert abs(res.greeks['x0'] - d) < 3 * res.greeks_se['x0']
+
+
+def test_heston_fd_differentiates_zero_parameters(gpu):
+    """r = 0 and rho = 0: the bump falls back to h itself (a relative bump of
+    0 would divide 0 by 0); the FD Greeks are finite and equal the adjoint's
+    on the same paths for the smooth directions."""
+    prm = dict(HPRM, r=0., rho=0.)
+    P, n = 4096, 24
+    price, _, fd = G.heston_price_fd(paths=P, steps=n, strike=100., key=KEY, epoch=5,
+                                     dtype=np.float64, h=1e-4, **prm)
+    adj = G.heston_greeks(paths=P, steps=n, strike=100., dtype=np.float64, key=KEY,
+                          epoch=5, **prm)
+    assert abs(price - adj.price) < 1e-10 * price
+    for name in G.HESTON_GREEKS:
+        assert np.isfinite(fd[name]), name
+    for name in ('x0', 'r'):
+        assert abs(fd[name] - adj.greeks[name]) < 1e-3 * abs(adj.greeks[name]), name
+    assert abs(fd['rho'] - adj.greeks['rho']) < 3 * adj.greeks_se['rho']
+
+
+def test_heston_autograd_price_does_not_depend_on_requires_grad(gpu):
+    """With an rng and no key, the price is computed on one key whether or
+    not a gradient is asked for (the adjoint kernel and the forward kernel
+    then price the same paths)."""
+    torch = gpu
+    cfg = dict(paths=4096, steps=30, strike=100., dtype=np.float64, epoch=0)
+    vals = [HPRM[k] for k in G.HESTON_GREEKS]
+    p1 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    p2 = torch.tensor(vals, dtype=torch.float64)
+    a = G.heston_price(p1, rng=np.random.default_rng(11), **cfg)
+    b = G.heston_price(p2, rng=np.random.default_rng(11), **cfg)
+    assert abs(a.item() - b.item()) < 1e-10 * abs(a.item())
+
+
+def test_loglin_autograd(gpu):
+    """The Merton adjoint behind torch.autograd: price and gradient equal
+    merton_greeks on the same keys, and the oracle's adjoint."""
+    torch = gpu
+    kw, kj = (21, 22), (23, 24)
+    cfg = dict(paths=4096, steps=50, lam=2., r=.03, strike=1., maturity=1.,
+               dtype=np.float64, key_w=kw, key_j=kj, epoch=1)
+    vals = dict(x0=1., mu=.03, sigma=.2, a=-.1, b=.15)
+    p = torch.tensor([vals[k] for k in G.LOG_GREEKS], dtype=torch.float64,
+                     requires_grad=True)
+    price = G.loglin_price(p, **cfg)
+    (2. * price).backward()
+    res = G.merton_greeks(**cfg, **vals)
+    assert abs(price.item() - res.price) < 1e-12 * res.price
+    assert np.allclose(p.grad.numpy(), [2. * res.greeks[k] for k in G.LOG_GREEKS],
+                       rtol=1e-12)
+    pay, g = og.loglin_adjoint(kw, kj, 1, 4096, 50, 1., x0=1., mu=.03, sigma=.2,
+                               strike=1., r=.03, lam=2., a=-.1, b=.15)
+    assert abs(price.item() - pay.mean()) < 1e-10 * pay.mean()
+    assert np.allclose(p.grad.numpy() / 2., g.mean(axis=1), rtol=1e-10, atol=1e-13)

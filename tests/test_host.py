@@ -374,3 +374,35 @@ def test_piecewise_semantics(mode, dtype, paths, vshape, shift):
     assert np.array_equal(q((-3, 0, 3)), np.stack((x[0], x[0], x[0])))
     with pytest.raises(ValueError):
         sa.piecewise((1, 2), v=(10, 20), mode='zzz')
+
+
+def test_fd_points_with_zero_parameters():
+    """Central-difference sets bump a zero parameter by h itself and divide
+    by the difference of the bumped values (no 0/0)."""
+    base = dict(x0=100., r=0., k=2., theta=.04, xi=.5, rho=0., y0=.04)
+    pts, spans = G.fd_points(base, G.HESTON_GREEKS, 1e-4)
+    assert len(pts) == 15 and pts[0] == base
+    assert spans['r'] == 2e-4 and spans['rho'] == 2e-4
+    assert abs(spans['x0'] - 2e-2) < 1e-13   # 100.01 - 99.99 as rounded
+    # a price linear in the parameters: the quotients are its slopes
+    slope = dict(x0=.5, r=30., k=-.2, theta=50., xi=-2., rho=.3, y0=40.)
+
+    def price(d):
+        return sum(slope[n] * d[n] for n in G.HESTON_GREEKS)
+    q = G.fd_quotients([price(d) for d in pts], G.HESTON_GREEKS, spans)
+    for n in G.HESTON_GREEKS:
+        assert np.isfinite(q[n]) and abs(q[n] - slope[n]) < 1e-8 * max(1., abs(slope[n])), n
+
+
+def test_batched_takes_float32_processes_only():
+    """A default-dtype (float64) process in a batch is refused, not stepped
+    in FP32 with other normals."""
+    from sdepy_amd import batched
+    p = sa.lognorm_process(paths=100, steps=5, x0=1., mu=.05, sigma=.2,
+                           rng=np.random.default_rng(1))
+    with pytest.raises(NotImplementedError):
+        batched._instance(p, (0., 1.))
+    p32 = sa.lognorm_process(paths=100, steps=5, x0=1., mu=.05, sigma=.2,
+                             dtype=np.float32, rng=np.random.default_rng(1))
+    d, tab = batched._instance(p32, (0., 1.))
+    assert d.nsteps == 4 and tab.shape[0] == 4
