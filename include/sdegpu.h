@@ -61,6 +61,14 @@ extern "C" {
 
 #define SDEGPU_MAX_VARS 8
 
+/* ---- compound Poisson jump-size laws (cpoisson_source's y, infrastructure.py:
+   2017-2040, :1653-1796), three parameters (p0, p1, p2) per component ---- */
+#define SDEGPU_JUMP_NORMAL 0     /* p0 + p1 z: norm_rv(a, b) = (a, b, -)  :1653-1664 */
+#define SDEGPU_JUMP_UNIFORM 1    /* p0 + p1 u: uniform_rv(a, b) = (a, b - a, -)  :1667-1677 */
+#define SDEGPU_JUMP_DOUBLE_EXP 2 /* (v <= p2 ? p0 : -p1) E, E = -ln(u'):
+                                    double_exp_rv(a, b, pa) = (a, b, pa)  :1696-1712,
+                                    exp_rv(a) = (a, -, 1)  :1680-1693 */
+
 /*
  * One Euler-Maruyama pass over a consolidated step timeline.
  *
@@ -84,7 +92,8 @@ extern "C" {
  *     CIR      theta[M], k[M], xi[M]
  *     HESTON   mu[N], sigma[N], theta[N], k[N], xi[N]   (N log-x values in vshape order;
  *              state layout: see heston_width)
- *     JUMPDIFF mu - sigma^2/2 [M], sigma[M], a[M], b[M]  (jumps a + b*N(0,1))
+ *     JUMPDIFF mu - sigma^2/2 [M], sigma[M], then the jump law's parameters p0[M], p1[M]
+             and, unless jump_law is SDEGPU_JUMP_NORMAL, p2[M] (SDEGPU_JUMP_*)
  * SDE parameters are evaluated at the step start, correlation and intensity at
  * the step midpoint, as sdepy does (integration.py:1218-1221,
  * infrastructure.py:1540, :1629).
@@ -159,6 +168,8 @@ typedef struct sdegpu_euler_problem {
                                  [nrows][S] and each block of 256 paths adds its jumps to
                                  slot blockIdx % S of its row, so the atomics of one row
                                  spread over S addresses; the caller sums the slots */
+  int32_t jump_law;           /* JUMPDIFF: SDEGPU_JUMP_* law of the jump sizes drawn in the
+                                 kernel (0, the normal law, for merton_jumpdiff_process) */
 } sdegpu_euler_problem;
 
 /* Library and device queries. */
@@ -288,9 +299,11 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  * sdegpu_poisson_fill: poisson_source.__call__ (infrastructure.py:1617-1633).
  *   pois [nslots][nvars][2] = (lam*|ds|, exp(-lam*|ds|)); sign [nslots] = sign(ds);
  *   out int64 [nslots][nvars][paths].
- * sdegpu_cpoisson_fill: cpoisson_source.__call__ with y = norm_rv(a, b)
- *   (infrastructure.py:2017-2040, :1653-1664).  ab [nslots][nvars][2] dtype;
- *   out [nslots][nvars][paths] dtype; dn_out optional int64 like poisson_fill.
+ * sdegpu_cpoisson_fill: cpoisson_source.__call__ (infrastructure.py:2017-2040)
+ *   with y of law jump_law (SDEGPU_JUMP_*; norm_rv, uniform_rv, exp_rv,
+ *   double_exp_rv, :1653-1796).  law_params [nslots][nvars][3] dtype = (p0, p1,
+ *   p2); out [nslots][nvars][paths] dtype; dn_out optional int64 like
+ *   poisson_fill.
  */
 int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
                        int32_t nvars, int64_t paths, int64_t nslots, const void* coef,
@@ -301,8 +314,8 @@ int sdegpu_poisson_fill(const uint32_t key[2], uint32_t epoch, uint32_t stream_t
                         long long* out, void* stream);
 int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                          uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
-                         const double* pois, const int32_t* sign, const void* ab, void* out,
-                         long long* dn_out, void* stream);
+                         const double* pois, const int32_t* sign, int32_t jump_law,
+                         const void* law_params, void* out, long long* dn_out, void* stream);
 
 /*
  * Pathwise Greeks by reverse-mode differentiation of the Euler scheme.

@@ -195,17 +195,25 @@ def _lower_matvec(L, z):
 
 
 class PhiloxCPoisson(_PhiloxSource):
-    """cpoisson_source.__call__ (infrastructure.py:2017-2040) with
-    y = norm_rv(a, b) (:1653-1664, :1715-1720): dn ~ Poisson(lam(t+dt/2) |dt|)
-    by inversion of Philox uniforms, jump sizes a + b z from the jump stream;
-    dz = sign(dt) * sum of the dn jump sizes."""
+    """cpoisson_source.__call__ (infrastructure.py:2017-2040):
+    dn ~ Poisson(lam(t+dt/2) |dt|) by inversion of Philox uniforms, jump
+    sizes from the jump stream, dz = sign(dt) * sum of the dn jump sizes.
+    ``law`` names y: 'normal' norm_rv(a, b) (:1653-1664, :1715-1720, the
+    default), 'uniform' uniform_rv(a, b) (:1667-1677, :1723-1728), 'exp'
+    exp_rv(a) (:1680-1693, :1731-1754), 'double_exp' double_exp_rv(a, b, pa)
+    (:1696-1712, :1757-1796); the parameters may be callables of t,
+    evaluated at t + dt/2 (:2029)."""
+
+    LAWS = ('normal', 'uniform', 'exp', 'double_exp')
 
     def __init__(self, key, epoch, paths, vshape, dtype=None, lam=1., a=0.,
-                 b=1., path_index=None):
+                 b=1., path_index=None, law='normal', pa=.5):
+        if law not in self.LAWS:
+            raise ValueError('unknown jump law {!r}'.format(law))
         self.key, self.epoch = key, epoch
         self.paths, self.vshape = paths, tuple(vshape)
         self.dtype = float if dtype is None else dtype
-        self.lam, self.a, self.b = lam, a, b
+        self.lam, self.a, self.b, self.pa, self.law = lam, a, b, pa, law
         self.path_index = (np.arange(paths) if path_index is None
                            else np.asarray(path_index))
         self.dn_value = None
@@ -224,18 +232,24 @@ class PhiloxCPoisson(_PhiloxSource):
                               self.pass_tag | philox.STREAM_POISSON,
                               s, M, self.path_index)              # (M, P)
         n = philox.poisson_inv(philox.u01(w), lamdt[:, None], p0[:, None])
-        a = np.broadcast_to(np.asarray(self.a(t + dt / 2) if callable(self.a)
-                                       else self.a, float).reshape(-1)
-                            if np.ndim(self.a) else self.a, (M,))
-        b = np.broadcast_to(np.asarray(self.b(t + dt / 2) if callable(self.b)
-                                       else self.b, float).reshape(-1)
-                            if np.ndim(self.b) else self.b, (M,))
+        def per_comp(z):
+            z = z(t + dt / 2) if callable(z) else z
+            return np.broadcast_to(np.asarray(z, float).reshape(-1)
+                                   if np.ndim(z) else z, (M,))
+        a, b, pa = per_comp(self.a), per_comp(self.b), per_comp(self.pa)
+        # the kernels' three parameters of each law
+        code, p0, p1, p2 = {
+            'normal': (philox.JUMP_NORMAL, a, b, 0. * a),
+            'uniform': (philox.JUMP_UNIFORM, a, b - a, 0. * a),
+            'exp': (philox.JUMP_DOUBLE_EXP, a, 0. * a + 1., 0. * a + 1.),
+            'double_exp': (philox.JUMP_DOUBLE_EXP, a, b, pa),
+        }[self.law]
         dz = np.zeros((M, P))
         for v in range(M):
             for i in np.nonzero(n[v])[0]:
-                dz[v, i] = philox.jump_sum(self.key, self.epoch, self.pass_tag,
-                                           s * M + v, self.path_index[i],
-                                           n[v, i], a[v], b[v])
+                dz[v, i] = philox.jump_total(self.key, self.epoch, self.pass_tag,
+                                             s * M + v, self.path_index[i],
+                                             n[v, i], code, p0[v], p1[v], p2[v])
         sign = int(np.sign(dt))
         self.dn_value = (sign * n).reshape(self.vshape + (P,))
         return (sign * dz).astype(self.dtype).reshape(self.vshape + (P,))

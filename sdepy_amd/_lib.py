@@ -23,6 +23,7 @@ F32, F64 = 0, 1
 MODEL_WIENER, MODEL_LOGNORM, MODEL_OU, MODEL_CIR, MODEL_HESTON, \
     MODEL_JUMPDIFF = range(6)
 OUT_IDENTITY, OUT_SUM, OUT_FIRST = range(3)
+JUMP_NORMAL, JUMP_UNIFORM, JUMP_DOUBLE_EXP = range(3)
 MAX_VARS = 8
 STREAM_BACKWARD = 16
 REDUCE_MIN, REDUCE_MAX, REDUCE_SUM, REDUCE_MEAN, REDUCE_VAR, REDUCE_STD = range(6)
@@ -53,7 +54,7 @@ class EulerProblem(ctypes.Structure):
         ('heston_width', c_int32),
         ('comp0', c_int32), ('nvars_total', c_int32),
         ('out0', c_int32), ('out_stride', c_int32), ('jump_defer', c_int32),
-        ('info_row_slots', c_int32),
+        ('info_row_slots', c_int32), ('jump_law', c_int32),
     ]
 
 
@@ -100,7 +101,7 @@ SIGNATURES = [
       c_void_p, c_void_p, c_void_p]),
     ('sdegpu_cpoisson_fill', ctypes.c_int,
      [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
-      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+      c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     ('sdegpu_philox_host', ctypes.c_int,
      [ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
       ctypes.POINTER(c_uint32)]),

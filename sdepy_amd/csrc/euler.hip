@@ -31,7 +31,7 @@ static int param_blocks(int model, int m) {
     case SDEGPU_MODEL_OU:
     case SDEGPU_MODEL_CIR: return 3 * m;
     case SDEGPU_MODEL_HESTON: return 5 * (m / 2);
-    case SDEGPU_MODEL_JUMPDIFF: return 4 * m;
+    case SDEGPU_MODEL_JUMPDIFF: return 4 * m;  // + m for a jump law other than the normal
     default: return -1;
   }
 }
@@ -64,8 +64,13 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   if (heston && pr->nout != m && pr->nout != m / 2)
     return fail(SDEGPU_EINVAL, "Heston stores %d (whole state) or %d (log x) values, not %d", m,
                 m / 2, pr->nout);
-  const int nparam = param_blocks(pr->model, m);
+  int nparam = param_blocks(pr->model, m);
   if (nparam < 0) return fail(SDEGPU_EUNSUPPORTED, "unknown model code %d", pr->model);
+  if (pr->model == SDEGPU_MODEL_JUMPDIFF) {
+    if (pr->jump_law < SDEGPU_JUMP_NORMAL || pr->jump_law > SDEGPU_JUMP_DOUBLE_EXP)
+      return fail(SDEGPU_EUNSUPPORTED, "unknown jump law %d", pr->jump_law);
+    if (pr->jump_law != SDEGPU_JUMP_NORMAL) nparam += m;
+  }
   const int ncorr = pr->has_corr ? m * (m + 1) / 2 : 0;
   const int poff = 2 + ncorr;
   if (pr->ncoef < poff + nparam)
@@ -121,6 +126,7 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.info_paths = pr->info_paths;
   a.info_rows = pr->info_rows;
   a.info_slots = pr->info_row_slots > 1 ? pr->info_row_slots : 1;
+  a.jump_law = pr->model == SDEGPU_MODEL_JUMPDIFF ? pr->jump_law : SDEGPU_JUMP_NORMAL;
   for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
   a.paths = pr->paths;
   a.anti_w = pr->anti_w;

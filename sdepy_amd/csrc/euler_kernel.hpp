@@ -25,6 +25,7 @@
 // SIMD instead of 7.
 #pragma once
 #include "common.hpp"
+#include "jumps.hpp"
 #include "philox.hpp"
 
 namespace sdegpu {
@@ -51,6 +52,7 @@ struct EulerArgs {
   // comp0 = out0 = 0, m_total = M, out_stride = nout
   int32_t comp0, m_total, out0, out_stride;
   int32_t info_slots;  // counters per row of info_rows (>= 1)
+  int32_t jump_law;    // SDEGPU_JUMP_*: the law of the jump sizes
 };
 
 // Steps per chunk so that a chunk consumes a whole number of Philox blocks:
@@ -204,26 +206,20 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 #pragma unroll
   for (int k = 0; k < (kDefer ? kJumpEvents : 1); ++k) jev[k] = 0u;
   int njev = 0;
-  // sum of the n jump sizes a + b z of component v at step s
-  auto jump_sum = [&](int s, int v, int n, T ja, T jb) -> T {
-    T acc = (T)0;
+  // sum of the n jump sizes of component v at step s, of the law whose
+  // parameters are the step row's blocks 2, 3 (and 4 but for the normal law)
+  auto jump_sum = [&](int s, int v, int n, const T* pr) -> T {
     const uint32_t slot = (uint32_t)((int64_t)s * mt + c0 + v);
-    T zz[4];
-    for (int j = 0; j < n; ++j) {
-      if ((j & 3) == 0)
-        normals4<T>(slot, path_j, a.c2j | ((uint32_t)(j >> 2) << 8), a.ej, a.kj0, a.kj1, zz);
-      const int lane = j & 3;
-      const T zj = lane == 0 ? zz[0] : lane == 1 ? zz[1] : lane == 2 ? zz[2] : zz[3];
-      acc = acc + (ja + jb * zj);
-    }
-    return acc;
+    const T p2 = a.jump_law == SDEGPU_JUMP_NORMAL ? (T)0 : pr[4 * M + v];
+    return jump_total<T>(a.jump_law, n, slot, path_j, a.c2j, a.ej, a.kj0, a.kj1, pr[2 * M + v],
+                         pr[3 * M + v], p2);
   };
   auto flush_jumps = [&]() {
     while (njev > 0) {
       const uint32_t e = jev[0];
       const int s = (int)(e >> 12), v = (int)((e >> 8) & 7u), n = (int)(e & 255u);
       const T* pr = coef + (int64_t)s * a.ncoef + a.poff;
-      T acc = jump_sum(s, v, n, pr[2 * M + v], pr[3 * M + v]);
+      T acc = jump_sum(s, v, n, pr);
       acc = (e >> 11) & 1u ? -acc : acc;
 #pragma unroll
       for (int w = 0; w < M; ++w) x[w] = x[w] + (w == v ? acc : (T)0);
@@ -370,8 +366,8 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
         }
       } else if constexpr (kJump) {
         // jumpdiff_SDE: {'dt': mu - sigma^2/2, 'dw': sigma, 'dj': 1}, no martingale correction;
-        // dj = sign(ds) * sum of n ~ Poisson(lam*|ds|) variates a + b*N(0,1)
-        // (cpoisson_source.__call__ with y = norm_rv(a, b)).
+        // dj = sign(ds) * sum of n ~ Poisson(lam*|ds|) jump sizes of the law
+        // jump_law (cpoisson_source.__call__; norm_rv(a, b) for Merton).
         T dj[M];
         if (replay_j) {
           const T* rp = (const T*)a.replay_dj + ((int64_t)s * mt + c0) * P + gid;
@@ -396,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
                 for (int q = 0; q < kJumpEvents; ++q) jev[q] = njev == q ? e : jev[q];
                 ++njev;
               } else {
-                acc = jump_sum(s, v, n, p[2 * M + v], p[3 * M + v]);
+                acc = jump_sum(s, v, n, p);
               }
             }
             dj[v] = neg ? -acc : acc;

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include "common.hpp"
+#include "jumps.hpp"
 #include "philox.hpp"
 
 namespace sdegpu {
@@ -80,15 +81,15 @@ __global__ __launch_bounds__(kBlock) void k_wiener_fill(uint32_t k0, uint32_t k1
   }
 }
 
-// poisson_source.__call__: dn = sign(dt) * Poisson(lam |dt|); with `ab` given,
-// cpoisson_source.__call__ with y = norm_rv(a, b): dz = sign(dt) * sum of dn
-// variates a + b N(0,1).
+// poisson_source.__call__: dn = sign(dt) * Poisson(lam |dt|); with `law`
+// parameters given, cpoisson_source.__call__: dz = sign(dt) * sum of dn jump
+// sizes of that law (jumps.hpp).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k1, uint32_t epoch,
                                                          uint32_t tag, int nvars, int64_t paths,
                                                          int64_t nslots, const double* __restrict__ pois,
-                                                         const int32_t* __restrict__ sign,
-                                                         const T* __restrict__ ab, T* __restrict__ out,
+                                                         const int32_t* __restrict__ sign, int law,
+                                                         const T* __restrict__ lp, T* __restrict__ out,
                                                          long long* __restrict__ dn_out) {
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (gid >= paths) return;
@@ -107,14 +108,8 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
       const int sg = sign[s];
       if (dn_out) dn_out[g * paths + gid] = (long long)(sg * n);
       if (out) {
-        const T a = ab[2 * g], b = ab[2 * g + 1];
-        T acc = (T)0;
-        T zz[4];
-        for (int j = 0; j < n; ++j) {
-          if ((j & 3) == 0)
-            normals4<T>((uint32_t)g, path, c2j | ((uint32_t)(j >> 2) << 8), epoch, k0, k1, zz);
-          acc = acc + (a + b * zz[j & 3]);
-        }
+        const T acc = jump_total<T>(law, n, (uint32_t)g, path, c2j, epoch, k0, k1, lp[3 * g],
+                                    lp[3 * g + 1], lp[3 * g + 2]);
         out[g * paths + gid] = sg < 0 ? -acc : acc;
       }
     }
@@ -230,8 +225,8 @@ extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t
 
 static int poisson_common(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t tag,
                           int32_t nvars, int64_t paths, int64_t nslots, const double* pois,
-                          const int32_t* sign, const void* ab, void* out, long long* dn_out,
-                          void* stream) {
+                          const int32_t* sign, int law, const void* lp, void* out,
+                          long long* dn_out, void* stream) {
   if (!key || nvars < 1 || paths < 1 || paths > kMaxPaths || nslots < 0)
     return fail(SDEGPU_EINVAL, "poisson_fill: nvars=%d paths=%lld slots=%lld", nvars,
                 (long long)paths, (long long)nslots);
@@ -243,10 +238,10 @@ static int poisson_common(int32_t dtype, const uint32_t key[2], uint32_t epoch, 
   const dim3 grid(blocks_for(paths)), block(kBlock);
   if (dtype == SDEGPU_F32)
     hipLaunchKernelGGL(k_poisson_fill<float>, grid, block, 0, st, key[0], key[1], epoch, t, nvars,
-                       paths, nslots, pois, sign, (const float*)ab, (float*)out, dn_out);
+                       paths, nslots, pois, sign, law, (const float*)lp, (float*)out, dn_out);
   else if (dtype == SDEGPU_F64)
     hipLaunchKernelGGL(k_poisson_fill<double>, grid, block, 0, st, key[0], key[1], epoch, t, nvars,
-                       paths, nslots, pois, sign, (const double*)ab, (double*)out, dn_out);
+                       paths, nslots, pois, sign, law, (const double*)lp, (double*)out, dn_out);
   else
     return fail(SDEGPU_EUNSUPPORTED, "poisson_fill: dtype code %d", dtype);
   return launch_status("k_poisson_fill");
@@ -257,14 +252,17 @@ extern "C" int sdegpu_poisson_fill(const uint32_t key[2], uint32_t epoch, uint32
                                    const int32_t* sign, long long* out, void* stream) {
   if (!out) return fail(SDEGPU_EINVAL, "poisson_fill: null output");
   return poisson_common(SDEGPU_F64, key, epoch, stream_tag, nvars, paths, nslots, pois, sign,
-                        nullptr, nullptr, out, stream);
+                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, stream);
 }
 
 extern "C" int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                                     uint32_t stream_tag, int32_t nvars, int64_t paths,
                                     int64_t nslots, const double* pois, const int32_t* sign,
-                                    const void* ab, void* out, long long* dn_out, void* stream) {
-  if (!out || !ab) return fail(SDEGPU_EINVAL, "cpoisson_fill: null pointer");
-  return poisson_common(dtype, key, epoch, stream_tag, nvars, paths, nslots, pois, sign, ab, out,
-                        dn_out, stream);
+                                    int32_t jump_law, const void* law_params, void* out,
+                                    long long* dn_out, void* stream) {
+  if (!out || !law_params) return fail(SDEGPU_EINVAL, "cpoisson_fill: null pointer");
+  if (jump_law < SDEGPU_JUMP_NORMAL || jump_law > SDEGPU_JUMP_DOUBLE_EXP)
+    return fail(SDEGPU_EUNSUPPORTED, "cpoisson_fill: unknown jump law %d", jump_law);
+  return poisson_common(dtype, key, epoch, stream_tag, nvars, paths, nslots, pois, sign, jump_law,
+                        law_params, out, dn_out, stream);
 }

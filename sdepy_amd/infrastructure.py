@@ -337,58 +337,260 @@ def per_component(value, vshape, name):
     return np.ascontiguousarray(b[..., 0]).reshape(-1)
 
 
-# random variables for compound Poisson sources --------------------------
+# --------------------------------------------------------------------------
+# jump-size laws of compound Poisson sources
+# --------------------------------------------------------------------------
 
-class _norm_rv:
-    """Normal distribution of mean a and standard deviation b
-    (sdepy/infrastructure.py:1653-1664, :1715-1720); the GPU compound
-    Poisson sampler draws it in-kernel."""
+def _host_generator(random_state):
+    """The NumPy generator a host-side ``rvs`` draws from: None gives this
+    module's default generator, an integer a new generator seeded with it; a
+    Generator or a legacy RandomState is used as it is."""
+    if random_state is None:
+        return default_rng
+    if isinstance(random_state, numbers.Integral):
+        return np.random.default_rng(int(random_state))
+    return random_state
 
-    def __init__(self, a=0., b=1.):
-        self.a, self.b = a, b
 
-    def _ab(self, t=None):
-        a = self.a(t) if callable(self.a) else self.a
-        b = self.b(t) if callable(self.b) else self.b
-        return a, b
+class jump_law:
+    """A distribution of compound Poisson jump sizes that the GPU kernels
+    sample themselves: the laws behind sdepy's ``norm_rv``, ``uniform_rv``,
+    ``exp_rv`` and ``double_exp_rv`` (infrastructure.py:1653-1796).
 
-    def at(self, t):
-        """The distribution at time t (callable parameters evaluated)."""
-        return _norm_rv(*self._ab(t))
+    A law is its parameters (``params`` names them, each an array; they
+    broadcast together) and three numbers per component for the kernels
+    (``kernel_params``, the SDEGPU_JUMP_* parametrisation of sdegpu.h).
+    ``rvs(size, random_state)`` samples on the host, for code that uses the
+    law outside a simulation; ``mean``, ``var``, ``std`` and ``exp_mean``
+    (the mean of exp(y), for martingale corrections) are closed forms.
+    """
+    params = ()
+    kernel_law = None
 
-    def rvs(self, size, random_state=None):
-        a, b = self._ab()
-        return scipy.stats.norm(loc=a, scale=b).rvs(size=size,
-                                                    random_state=random_state)
+    def __init__(self, *values):
+        arrays = np.broadcast_arrays(*(np.asarray(v, dtype=float) for v in values))
+        for name, arr in zip(self.params, arrays):
+            setattr(self, name, arr)
+        if not self._valid():
+            raise ValueError('domain error in arguments')
 
-    def mean(self):
-        return np.asarray(self.a) + 0.
+    def _valid(self):
+        return True
+
+    def _triple(self):
+        raise NotImplementedError
+
+    def kernel_params(self, vshape):
+        """(p0, p1, p2) of every component of ``vshape``: FP64, shape (M, 3).
+        Raises NotImplementedError for parameters that vary across paths."""
+        return np.stack([per_component(c, vshape, 'jump law')
+                         for c in self._triple()], axis=1)
 
     def std(self):
-        return np.asarray(self.b) + 0.
+        return np.sqrt(self.var())
+
+    def __repr__(self):
+        return '{}({})'.format(type(self).__name__, ', '.join(
+            '{}={}'.format(n, getattr(self, n)) for n in self.params))
+
+
+class normal_jumps(jump_law):
+    """Normal jump sizes of mean a and standard deviation b."""
+    params = ('a', 'b')
+    kernel_law = _lib.JUMP_NORMAL
+
+    def _triple(self):
+        return self.a, self.b, 0.
+
+    def rvs(self, size, random_state=None):
+        return self.a + self.b * _host_generator(random_state).standard_normal(size)
+
+    def mean(self):
+        return self.a + 0.
 
     def var(self):
-        return np.asarray(self.b) ** 2 + 0.
+        return self.b * self.b + 0.
 
     def exp_mean(self):
-        a, b = np.asarray(self.a), np.asarray(self.b)
-        return np.exp(a + b * b / 2) + 0
+        return np.exp(self.a + self.b * self.b / 2) + 0.
+
+
+class uniform_jumps(jump_law):
+    """Jump sizes uniform between a and b."""
+    params = ('a', 'b')
+    kernel_law = _lib.JUMP_UNIFORM
+
+    def _triple(self):
+        return self.a, self.b - self.a, 0.
+
+    def rvs(self, size, random_state=None):
+        u = _host_generator(random_state).random(size)
+        return self.a + (self.b - self.a) * u
+
+    def mean(self):
+        return (self.a + self.b) / 2 + 0.
+
+    def var(self):
+        return (self.b - self.a) ** 2 / 12 + 0.
+
+    def exp_mean(self):
+        return (np.exp(self.b) - np.exp(self.a)) / (self.b - self.a) + 0.
+
+
+class exponential_jumps(jump_law):
+    """Exponential jump sizes of scale |a| on the side of the sign of a,
+    i.e. a E with E a standard exponential (a != 0)."""
+    params = ('a',)
+    kernel_law = _lib.JUMP_DOUBLE_EXP
+
+    def _valid(self):
+        return not (self.a == 0).any()
+
+    def _triple(self):
+        # the double exponential whose positive side is always taken
+        return self.a, 1., 1.
+
+    def rvs(self, size, random_state=None):
+        return self.a * _host_generator(random_state).standard_exponential(size)
+
+    def mean(self):
+        return self.a + 0.
+
+    def var(self):
+        return self.a * self.a + 0.
+
+    def exp_mean(self):
+        # E exp(a E) = 1 / (1 - a), finite for a < 1
+        with np.errstate(divide='ignore', invalid='ignore'):
+            return np.where(self.a < 1, 1 / (1 - self.a), np.inf) + 0.
+
+
+class double_exponential_jumps(jump_law):
+    """Kou's double exponential jump sizes: a E with probability pa, -b E
+    with probability 1 - pa (a, b > 0, 0 <= pa <= 1; E a standard
+    exponential)."""
+    params = ('a', 'b', 'pa')
+    kernel_law = _lib.JUMP_DOUBLE_EXP
+
+    def _valid(self):
+        return bool((self.a > 0).all() and (self.b > 0).all()
+                    and ((self.pa >= 0) & (self.pa <= 1)).all())
+
+    def _triple(self):
+        return self.a, self.b, self.pa
+
+    def rvs(self, size, random_state=None):
+        g = _host_generator(random_state)
+        side = np.where(g.random(size) <= self.pa, self.a, -self.b)
+        return side * g.standard_exponential(size)
+
+    def mean(self):
+        return self.pa * self.a - (1 - self.pa) * self.b + 0.
+
+    def var(self):
+        # E y^2 = 2 (pa a^2 + (1 - pa) b^2), an exponential's second moment
+        # being twice its squared scale
+        second = 2 * (self.pa * self.a ** 2 + (1 - self.pa) * self.b ** 2)
+        return second - self.mean() ** 2 + 0.
+
+    def exp_mean(self):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            up = np.where(self.a < 1, self.pa / (1 - self.a), np.inf)
+        return up + (1 - self.pa) / (1 + self.b) + 0.
+
+
+class timed_jump_law:
+    """A jump law whose parameters depend on time: ``y(t)`` is the law with
+    every callable parameter evaluated at t.  cpoisson_source evaluates a
+    callable ``y`` at each step's midpoint (infrastructure.py:2029), and so
+    does the kernel path, one table row per step."""
+
+    def __init__(self, law_class, values):
+        self.law_class, self.values = law_class, tuple(values)
+
+    def __call__(self, t):
+        return self.law_class(*[v(t) if callable(v) else v for v in self.values])
+
+
+def _jump_law_or_timed(law_class, *values):
+    if any(callable(v) for v in values):
+        return timed_jump_law(law_class, values)
+    return law_class(*values)
 
 
 def norm_rv(a=0., b=1.):
-    """Normal jump-size distribution with mean a and standard deviation b,
-    either possibly callable in t (sdepy/infrastructure.py:1653-1664)."""
-    return _norm_rv(a, b)
+    """Normal jump sizes of mean a and standard deviation b, either possibly
+    callable in t (sdepy norm_rv, infrastructure.py:1653-1664); drawn in the
+    kernels."""
+    return _jump_law_or_timed(normal_jumps, a, b)
+
+
+def uniform_rv(a=0., b=1.):
+    """Jump sizes uniform between a and b, possibly callable in t (sdepy
+    uniform_rv, infrastructure.py:1667-1677); drawn in the kernels."""
+    return _jump_law_or_timed(uniform_jumps, a, b)
+
+
+def exp_rv(a=1.):
+    """Exponential jump sizes of scale |a| on the side of the sign of a,
+    possibly callable in t (sdepy exp_rv, infrastructure.py:1680-1693);
+    drawn in the kernels."""
+    return _jump_law_or_timed(exponential_jumps, a)
+
+
+def double_exp_rv(a=1., b=1., pa=.5):
+    """Double exponential (Kou) jump sizes: scale a with probability pa,
+    -b with probability 1 - pa, possibly callable in t (sdepy double_exp_rv,
+    infrastructure.py:1696-1712); drawn in the kernels."""
+    return _jump_law_or_timed(double_exponential_jumps, a, b, pa)
+
+
+class _mapped_variates:
+    """``rvs`` of the variates of ``law`` passed through ``f`` (with the
+    time ``t`` first when f takes it)."""
+
+    def __init__(self, f, law, t=None):
+        self.f, self.law, self.t = f, law, t
+
+    def rvs(self, size, random_state=None):
+        v = self.law.rvs(size=size, random_state=random_state)
+        return self.f(v) if self.t is None else self.f(self.t, v)
+
+
+def _first_parameter(f):
+    import inspect
+    try:
+        return next(iter(inspect.signature(f).parameters), None)
+    except (TypeError, ValueError):
+        return None
+
+
+def rvmap(f, y):
+    """Jump sizes f(y), or f(t, y) when f's first parameter is named t or s,
+    for a law y possibly callable in t (sdepy rvmap, infrastructure.py:
+    1799-1864).  The result offers ``rvs`` only, so compound Poisson sources
+    draw its sizes on the host and replay them."""
+    with_time = _first_parameter(f) in ('t', 's')
+    if not (with_time or callable(y)):
+        return _mapped_variates(f, y)
+
+    def at(t):
+        return _mapped_variates(f, y(t) if callable(y) else y,
+                                t if with_time else None)
+    return at
 
 
 class cpoisson_source(_philox_source):
     """dj, compound Poisson increments (sdepy/infrastructure.py:1881-2040).
 
-    With ``y = norm_rv(a, b)`` (the Merton case) both the Poisson counts and
-    the jump sizes are drawn on the GPU.  Any other ``y`` exposing
-    ``rvs(size, random_state)`` keeps sdepy's semantics: counts from the GPU,
-    jump sizes from ``y.rvs`` with this source's ``rng``.  ``dn_value`` holds
-    the signed counts of the last call.
+    With ``y`` one of the jump laws above (the default is uniform_rv(0, 1),
+    as in sdepy), possibly callable in t, both the Poisson counts and the
+    jump sizes are drawn on the GPU.  Any other ``y`` exposing
+    ``rvs(size, random_state)``, or a user ``dn`` source, keeps sdepy's
+    semantics on the host: counts from ``dn``, jump sizes from ``y.rvs`` with
+    this source's ``rng``.  ``dn_value`` holds the signed counts of the last
+    call, ``y_value`` the host-drawn sizes of each step (none when the GPU
+    draws them).
     """
 
     def __init__(self, *, paths=1, vshape=(), dtype=None, rng=None,
@@ -407,286 +609,206 @@ class cpoisson_source(_philox_source):
             self.dn = dn
         self.ptype = getattr(self.dn, 'dtype', ptype)
         self.lam = getattr(self.dn, 'lam', lam)
-        self.y = _uniform_default() if y is None else y
+        self.y = uniform_rv(0., 1.) if y is None else y
         self.dn_value = None
         self.y_value = None
 
-    @property
-    def gpu_jumps(self):
-        """True when counts and sizes can be drawn inside the Euler kernel."""
-        return isinstance(self.y, _norm_rv) and isinstance(self.dn, poisson_source) \
-            and self.dn.key == self.key
+    def _law_at(self, t):
+        """y at time t (a callable y is evaluated, as sdepy does)."""
+        return self.y(t) if callable(self.y) else self.y
 
-    def ab_table(self, tt, dtt):
+    def kernel_laws(self, tt, dtt):
+        """(law code, FP64 parameters [slots][M][3]) of the jump sizes the
+        kernels draw at the steps (tt, dtt), y evaluated at each step's
+        midpoint; None when the host must draw them: a user ``dn`` source, a
+        ``y`` that is not a jump law (or changes law between steps), or law
+        parameters that vary across paths."""
+        if not (isinstance(self.dn, poisson_source) and self.dn.key == self.key):
+            return None
         M = int(np.prod(self.vshape, dtype=int))
-        tab = np.zeros((tt.size, M, 2))
+        tab = np.zeros((tt.size, M, 3))
+        if not callable(self.y):
+            if not isinstance(self.y, jump_law):
+                return None
+            try:
+                tab[:] = self.y.kernel_params(self.vshape)
+            except NotImplementedError:
+                return None
+            return self.y.kernel_law, tab
+        code = None
         for i in range(tt.size):
-            a, b = self.y._ab(tt[i] + dtt[i] / 2)
-            tab[i, :, 0] = per_component(a, self.vshape, 'a')
-            tab[i, :, 1] = per_component(b, self.vshape, 'b')
-        return tab
+            law = self._law_at(tt[i] + dtt[i] / 2)
+            if not isinstance(law, jump_law) or code not in (None, law.kernel_law):
+                return None
+            code = law.kernel_law
+            try:
+                tab[i] = law.kernel_params(self.vshape)
+            except NotImplementedError:
+                return None
+        return (_lib.JUMP_NORMAL if code is None else code), tab
 
     def __call__(self, t, dt):
         t, dt = np.broadcast_arrays(np.asarray(t, dtype=float),
                                     np.asarray(dt, dtype=float))
         tshape = dt.shape
         shape = self.vshape + (self.paths,)
-        if self.gpu_jumps:
-            code, cdt, odt = _gpu_dtype(self.dtype)
-            M = int(np.prod(self.vshape, dtype=int))
-            n = int(np.prod(tshape, dtype=int))
-            tf, dtf = t.reshape(-1), dt.reshape(-1)
-            pois = self.dn.lam_table(tf, dtf)
-            ab = self.ab_table(tf, dtf).astype(cdt)
-            sign = np.sign(dtf).astype(np.int32)
-            _lib.require_gpu()
-            dpois, dab, dsign = (_lib.to_device(pois), _lib.to_device(ab),
-                                 _lib.to_device(sign))
-            out = _lib.empty((n, M, self.paths), cdt)
-            dn = _lib.empty((n, M, self.paths), np.int64)
-            epoch = self.next_epoch()
-            self.dn.epoch = self.epoch
-            _lib.check(_lib.lib().sdegpu_cpoisson_fill(
-                code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
-                _lib.ptr(dpois), _lib.ptr(dsign), _lib.ptr(dab),
-                _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
-            self.dn_value = _lib.to_host(dn).astype(self.ptype).reshape(
-                tshape + shape)
-            self.y_value = None
-            return _lib.to_host(out).astype(odt, copy=False).reshape(
-                tshape + shape)
-        # generic y: counts from dn, sizes from y.rvs (sdepy's grouping by
-        # count, infrastructure.py:2026-2037)
-        sign_dt = np.sign(dt).astype(int)
+        tf, dtf = t.reshape(-1), dt.reshape(-1)
+        laws = self.kernel_laws(tf, dtf)
+        if laws is not None:
+            return self._gpu_draw(tf, dtf, laws, tshape)
+        # host sizes: every step draws all its jumps at once from y at the
+        # step's midpoint and adds them up per path and component
+        sgn = np.sign(dt).astype(int)
         dn = np.asarray(self.dn(t, dt))
-        dz = np.zeros(dt.shape + shape, dtype=self.dtype)
-        y_value = []
-        for i in np.ndindex(dt.shape):
-            dnp = sign_dt[i] * dn[i]
-            ti = t[i] + dt[i] / 2
-            if isinstance(self.y, _norm_rv):
-                rv = self.y.at(ti)
-            elif callable(self.y) and not hasattr(self.y, 'rvs'):
-                rv = self.y(ti)
-            else:
-                rv = self.y
-            for j in range(1, int(dnp.max(initial=0)) + 1):
-                index = dnp == j
-                if index.any():
-                    ys = rv.rvs(size=(index.sum(), j), random_state=self.rng)
-                    dz[i][index] = sign_dt[i] * ys.sum(axis=-1)
-                    y_value.append(ys)
+        dz = np.zeros(tshape + shape, dtype=self.dtype)
+        drawn = []
+        for i in np.ndindex(tshape):
+            counts = (sgn[i] * dn[i]).reshape(-1)
+            total = int(counts.sum())
+            if total == 0:
+                continue
+            law = self._law_at(t[i] + dt[i] / 2)
+            try:
+                sizes = law.rvs(size=total, random_state=self.rng)
+            except TypeError:       # an rvs without random_state
+                sizes = law.rvs(size=total)
+            sizes = np.asarray(sizes, dtype=float).reshape(-1)
+            owner = np.repeat(np.arange(counts.size), counts)
+            sums = np.bincount(owner, weights=sizes, minlength=counts.size)
+            dz[i] = (sgn[i] * sums).reshape(shape)
+            drawn.append(sizes)
         self.dn_value = dn
-        self.y_value = y_value
+        self.y_value = drawn
         return dz
+
+    def _gpu_draw(self, tf, dtf, laws, tshape):
+        code, cdt, odt = _gpu_dtype(self.dtype)
+        law, params = laws
+        M = int(np.prod(self.vshape, dtype=int))
+        n = tf.size
+        shape = self.vshape + (self.paths,)
+        pois = self.dn.lam_table(tf, dtf)
+        sign = np.sign(dtf).astype(np.int32)
+        _lib.require_gpu()
+        dpois, dsign = _lib.to_device(pois), _lib.to_device(sign)
+        dlaw = _lib.to_device(np.ascontiguousarray(params, dtype=cdt))
+        out = _lib.empty((n, M, self.paths), cdt)
+        dn = _lib.empty((n, M, self.paths), np.int64)
+        epoch = self.next_epoch()
+        self.dn.epoch = self.epoch
+        _lib.check(_lib.lib().sdegpu_cpoisson_fill(
+            code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
+            _lib.ptr(dpois), _lib.ptr(dsign), law, _lib.ptr(dlaw),
+            _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
+        self.dn_value = _lib.to_host(dn).astype(self.ptype).reshape(tshape + shape)
+        self.y_value = None
+        return _lib.to_host(out).astype(odt, copy=False).reshape(tshape + shape)
 
 
 # --------------------------------------------------------------------------
 # sources with antithetic paths
 # --------------------------------------------------------------------------
 
-def _antithetics(source_class, sign):
-    """A source class whose ``2K`` paths are ``K`` paths of ``source_class``
-    followed by the same ``K`` paths multiplied by ``sign`` (sdepy's
-    ``_antithetics``, infrastructure.py:2047-2098; like there, the class is
-    not a subclass of ``source_class``).  Inside the Euler kernel the
-    trailing half re-reads the Philox counters of the leading half."""
+class _antithetic_pairs(source):
+    """Base of the sources with antithetic paths (sdepy's _antithetics,
+    infrastructure.py:2047-2098): ``2K`` paths, the ``K`` paths of a
+    ``half_class`` source followed by the same ``K`` paths times
+    ``antithetic_sign``.  As in sdepy, the classes are not subclasses of the
+    source they wrap.  Inside the Euler kernel the trailing half re-reads the
+    Philox counters of the leading half, so nothing is stored."""
+    half_class = None
+    antithetic_sign = 1
 
-    class antithetics_class(source):
-        def __init__(self, *, paths=2, vshape=(), dtype=None, rng=None,
-                     **args):
-            if paths % 2:
-                raise ValueError(
-                    'the number of paths for sources with antithetics '
-                    'should be even, not {}'.format(paths))
-            self._dz = source_class(paths=paths // 2, vshape=vshape,
-                                    dtype=dtype, rng=rng, **args)
+    def __init__(self, *, paths=2, vshape=(), dtype=None, rng=None, **args):
+        if paths % 2:
+            raise ValueError(
+                'the number of paths for sources with antithetics '
+                'should be even, not {}'.format(paths))
+        self._half = self.half_class(paths=paths // 2, vshape=vshape,
+                                     dtype=dtype, rng=rng, **args)
 
-        def __call__(self, t, dt=None):
-            dz = self._dz(t, dt)
-            return np.concatenate((dz, -dz if sign < 0 else dz), axis=-1)
+    def __call__(self, t, dt=None):
+        lead = self._half(t, dt)
+        return np.concatenate(
+            (lead, lead if self.antithetic_sign > 0 else -lead), axis=-1)
 
-        @property
-        def paths(self):
-            return 2 * self._dz.paths
+    @property
+    def half(self):
+        """The source drawing the leading half of the paths."""
+        return self._half
 
-        @property
-        def vshape(self):
-            return self._dz.vshape
+    @property
+    def paths(self):
+        return 2 * self._half.paths
 
-        @property
-        def dtype(self):
-            return self._dz.dtype
+    @property
+    def vshape(self):
+        return self._half.vshape
 
-        @property
-        def rng(self):
-            return self._dz.rng
+    @property
+    def dtype(self):
+        return self._half.dtype
 
-        @property
-        def half(self):
-            """The source drawing the leading half of the paths."""
-            return self._dz
-
-        antithetic_sign = sign
-
-    antithetics_class.__init__.__doc__ = (
-        'See {} class documentation'.format(source_class.__name__))
-    return antithetics_class
+    @property
+    def rng(self):
+        return self._half.rng
 
 
-class odd_wiener_source(_antithetics(wiener_source, -1)):
+class odd_wiener_source(_antithetic_pairs):
     """dw with antithetic paths (sdepy infrastructure.py:2102-2117): with
     ``paths=2K``, ``x[..., i] == -x[..., K + i]``."""
+    half_class = wiener_source
+    antithetic_sign = -1
 
 
-class even_poisson_source(_antithetics(poisson_source, 1)):
+class even_poisson_source(_antithetic_pairs):
     """dn with antithetic paths exposing identical increments (sdepy
     infrastructure.py:2120-2136): ``x[..., i] == x[..., K + i]``."""
+    half_class = poisson_source
 
 
-class even_cpoisson_source(_antithetics(cpoisson_source, 1)):
+class even_cpoisson_source(_antithetic_pairs):
     """dj with antithetic paths exposing identical increments (sdepy
     infrastructure.py:2139-2156): ``x[..., i] == x[..., K + i]``."""
-
-
-def _rv_params_variable(make, **params):
-    """``make(**params)``, or, when a parameter is callable in t, a callable
-    ``t -> make(**params at t)`` as cpoisson_source accepts it (sdepy's
-    _make_rv_params_variable, infrastructure.py:1640-1650)."""
-    if any(callable(x) for x in params.values()):
-        return lambda t: make(**{k: (x(t) if callable(x) else x)
-                                 for k, x in params.items()})
-    return make(**params)
-
-
-class _uniform_rv:
-    """Uniform on [a, b] (infrastructure.py:1723-1728), sampled on the host."""
-
-    def __init__(self, a=0., b=1.):
-        self.a, self.b = np.broadcast_arrays(a, b)
-
-    def rvs(self, size, random_state=None):
-        return scipy.stats.uniform(loc=self.a, scale=self.b - self.a).rvs(
-            size=size, random_state=random_state)
-
-    def exp_mean(self):
-        a, b = self.a, self.b
-        return (np.exp(b) - np.exp(a)) / (b - a) + 0
-
-
-def _uniform_default():
-    return _uniform_rv(0., 1.)
-
-
-def uniform_rv(a=0., b=1.):
-    """Uniform jump sizes on [a, b], possibly time-dependent
-    (sdepy/infrastructure.py:1667-1677); sampled on the host through
-    ``rvs``."""
-    return _rv_params_variable(_uniform_rv, a=a, b=b)
-
-
-class _exp_rv:
-    """Exponential with scale |a|, on the side of the sign of a
-    (infrastructure.py:1731-1754)."""
-
-    def __init__(self, a=1.):
-        a = self._a = np.asarray(a)
-        if (a == 0).any():
-            raise ValueError('domain error in arguments')
-        self._s = np.sign(a)
-        self._rv = scipy.stats.expon(scale=np.abs(a))
-
-    def rvs(self, size, random_state=None):
-        return self._s * self._rv.rvs(size=size, random_state=random_state)
-
-    def mean(self):
-        return self._s * self._rv.mean()
-
-    def var(self):
-        return self._rv.var()
-
-    def std(self):
-        return np.sqrt(self._rv.var())
-
-    def exp_mean(self):
-        a = self._a
-        return np.where(a < 1, 1 / (1 - a), np.inf) + 0
-
-
-def exp_rv(a=1.):
-    """Exponential jump sizes with scale a, possibly time-dependent
-    (infrastructure.py:1680-1693); sampled on the host."""
-    return _rv_params_variable(_exp_rv, a=a)
-
-
-class _double_exp_rv:
-    """Scale a with probability pa, -b with probability 1 - pa
-    (infrastructure.py:1757-1796)."""
-
-    def __init__(self, a=1., b=1., pa=.5):
-        a, b, pa, pb = self._a, self._b, self._pa, self._pb = \
-            np.broadcast_arrays(a, b, pa, 1 - np.asarray(pa))
-        if (a <= 0).any() or (b <= 0).any() or (pa > 1).any() or (pa < 0).any():
-            raise ValueError('domain error in arguments')
-        self._rvxa = scipy.stats.expon(scale=a)
-        self._rvxb = scipy.stats.expon(scale=b)
-        self._rvu = scipy.stats.uniform(scale=1.)
-
-    def rvs(self, size, random_state=None):
-        plus = self._rvxa.rvs(size=size, random_state=random_state)
-        minus = self._rvxb.rvs(size=size, random_state=random_state)
-        u = self._rvu.rvs(size=size, random_state=random_state)
-        return np.where(u <= self._pa, plus, -minus) + 0
-
-    def mean(self):
-        return (self._pa * self._a - self._pb * self._b) + 0
-
-    def var(self):
-        a, b, pa, pb = self._a, self._b, self._pa, self._pb
-        return pa * pb * (a + b) ** 2 + (pa * a ** 2 + pb * b ** 2) + 0
-
-    def std(self):
-        return np.sqrt(self.var())
-
-    def exp_mean(self):
-        a, b, pa, pb = self._a, self._b, self._pa, self._pb
-        return (np.where(a < 1, pa / (1 - np.where(a < 1, a, 0)), np.inf)
-                + pb / (1 + b) + 0)
-
-
-def double_exp_rv(a=1., b=1., pa=.5):
-    """Double exponential jump sizes (Kou), possibly time-dependent
-    (infrastructure.py:1696-1712); sampled on the host."""
-    return _rv_params_variable(_double_exp_rv, a=a, b=b, pa=pa)
-
-
-def rvmap(f, y):
-    """Map f over the variates of y, possibly time-dependent
-    (infrastructure.py:1799-1864): ``f(y)``, or ``f(t, y)`` when f's first
-    parameter is named t or s."""
-    import inspect
-    first = next(iter(inspect.signature(f).parameters), None)
-    time_dependent_f = first in ('t', 's')
-    if callable(y) or time_dependent_f:
-        def new_y(t):
-            yt = y(t) if callable(y) else y
-
-            class _mapped:
-                def rvs(self, size, random_state=None):
-                    v = yt.rvs(size=size, random_state=random_state)
-                    return f(t, v) if time_dependent_f else f(v)
-            return _mapped()
-        return new_y
-
-    class _mapped:
-        def rvs(self, size, random_state=None):
-            return f(y.rvs(size=size, random_state=random_state))
-    return _mapped()
+    half_class = cpoisson_source
 
 
 # --------------------------------------------------------------------------
 # process
 # --------------------------------------------------------------------------
+
+def _process_key(key):
+    """(mode, index tuple) of a process key led by a string ('v', 't', 'p'
+    and their index), or (None, key) for a plain ndarray key.  A single
+    tuple after the mode is the index itself: p['v', (i, j)] is
+    p['v', i, j]."""
+    if isinstance(key, str):
+        return key, ()
+    if isinstance(key, tuple) and key and isinstance(key[0], str):
+        rest = key[1:]
+        if len(rest) == 1 and isinstance(rest[0], tuple):
+            rest = rest[0]
+        return key[0], rest
+    return None, key
+
+
+def _constant_in_time(x):
+    """f(s) = x[0] for every time s, shaped s.shape + x.shape[1:]."""
+    def f(s):
+        s = np.asarray(s)
+        return np.broadcast_to(x[0], s.shape + x.shape[1:]).copy()
+    return f
+
+
+def _timeline_interpolant(t, x, kind):
+    """scipy's interp1d along axis 0 of x over the timeline t, constant
+    beyond its ends, giving values of x's dtype."""
+    g = scipy.interpolate.interp1d(t, x, axis=0, kind=kind,
+                                   assume_sorted=True, copy=False,
+                                   bounds_error=False,
+                                   fill_value=(x[0], x[-1]))
+    return lambda s: g(s).astype(x.dtype, copy=False)
+
 
 class process(np.ndarray):
     """process(t=0., *, x=None, v=None, c=None, dtype=None)
@@ -704,32 +826,34 @@ class process(np.ndarray):
     interp_kind = 'linear'
 
     def __new__(cls, t=0., *, x=None, v=None, c=None, dtype=None):
-        t = np.asarray(t)
-        if t.ndim > 1 or t.size == 0:
+        tl = np.asarray(t)
+        if tl.ndim > 1 or tl.size == 0:
             raise ValueError('the shape of a process timeline should be () '
-                             'or (n,), not {}'.format(t.shape))
-        if t.ndim == 0:
-            t = t.reshape(1)
-        if sum(z is not None for z in (x, v, c)) != 1:
+                             'or (n,), not {}'.format(tl.shape))
+        if tl.ndim == 0:
+            tl = tl.reshape(1)      # a constant process: one time point
+        given = [(kind, z) for kind, z in (('x', x), ('v', v), ('c', c))
+                 if z is not None]
+        if len(given) != 1:
             raise ValueError('when creating a process instance, one and only '
                              'one of x or v or c should be provided')
-        if x is not None:
-            x = np.asarray(x, dtype=dtype)
-        elif v is not None:
-            x = np.asarray(v, dtype=dtype)[..., np.newaxis]
-        else:
-            c = np.asarray(c, dtype=dtype)
-            if t.size == 1:
-                x = c[np.newaxis, ..., np.newaxis]
-            else:
-                x = np.empty(shape=t.shape + c.shape + (1,), dtype=c.dtype)
-                x[...] = c[np.newaxis, ..., np.newaxis]
-        if t.shape != x.shape[:1]:
+        kind, body = given[0]
+        body = np.asarray(body, dtype=dtype)
+        if kind == 'v':             # values of one path: add the paths axis
+            body = body[..., None]
+        elif kind == 'c':           # one value for all times, one path
+            body = body[None, ..., None]
+            if tl.size > 1:
+                # repeated along the timeline in a new array of the
+                # requested dtype (float when none is given)
+                body = np.broadcast_to(body, tl.shape + body.shape[1:]).astype(
+                    float if dtype is None else dtype)
+        if body.shape[:1] != tl.shape:
             raise ValueError('process could not be created from timeline t '
                              'shaped {} and body shaped {}'
-                             .format(t.shape, x.shape))
-        obj = x.view(cls)
-        obj.t = t
+                             .format(tl.shape, body.shape))
+        obj = body.view(cls)
+        obj.t = tl
         return obj
 
     def __array_finalize__(self, obj):
@@ -800,66 +924,53 @@ class process(np.ndarray):
     # interpolation (infrastructure.py:544-647) ---------------------------
 
     def interp(self, *, kind=None):
-        t, x = self.t, self.view(np.ndarray)
+        """f(s): the values at times s (any shape), interpolated along the
+        timeline with ``kind`` (default ``interp_kind``) and held at the end
+        values outside it; a one-point process is constant in time."""
         kind = self.interp_kind if kind is None else kind
-        if t.size == 1:
-            def f(s):
-                s = np.asarray(s)
-                return np.ones(s.shape + x.shape[1:], dtype=x.dtype) * \
-                    x.reshape(tuple(1 for i in s.shape) + x.shape[1:])
-            return f
-        g = scipy.interpolate.interp1d(t, x, axis=0, kind=kind,
-                                       assume_sorted=True, copy=False,
-                                       bounds_error=False,
-                                       fill_value=(x[0], x[-1]))
-
-        def f(s):
-            return g(s).astype(x.dtype, copy=False)
-        return f
+        x = self.view(np.ndarray)
+        if self.t.size == 1:
+            return _constant_in_time(x)
+        return _timeline_interpolant(self.t, x, kind)
 
     def __call__(self, s, ds=None, *, kind=None):
+        """Values at s, or their increments from s to s + ds."""
+        at = self.interp(kind=kind)
         s = np.asarray(s)
-        f = self.interp(kind=kind)
-        if ds is None:
-            return f(s)
-        ds = np.asarray(ds)
-        return f(s + ds) - f(s)
+        return at(s) if ds is None else at(s + np.asarray(ds)) - at(s)
 
     def rebase(self, t, *, kind=None):
-        t = np.asarray(t)
-        if t.ndim == 0:
-            t = t.reshape(1)
+        """The process interpolated onto timeline t (a scalar t gives a
+        constant process)."""
+        t = np.atleast_1d(np.asarray(t))
         return process(t, x=self(t, kind=kind))
 
     # indexing (infrastructure.py:653-700) -------------------------------
 
     def __getitem__(self, key):
+        """Plain keys index the values as an ndarray would; a key led by a
+        string selects a mode: ``p['v', i...]`` values (a process),
+        ``p['t', i]`` time points, ``p['p', j]`` paths; an integer i or j
+        keeps its axis (one point, one path)."""
         x = self.view(np.ndarray)
-        cls = type(self)
-        colon = (slice(None),)
-        if not isinstance(key, (tuple, str)):
+        mode, rest = _process_key(key)
+        if mode is None:
             return x[key]
-        if isinstance(key, str):
-            key = (key,)
-        elif len(key) == 0:
-            return x[()]
-        elif not isinstance(key[0], str):
-            return x[key]
-        a, key = key[0], key[1:]
-        if len(key) == 1 and isinstance(key[0], tuple):
-            key = key[0]
-        if a not in ('v', 't', 'p'):
+        if mode not in ('v', 't', 'p'):
             raise IndexError('process indexing error - unsupported indexing '
-                             'mode ' + repr(a))
-        if a == 'v':
-            return cls(self.t, x=x[colon + key + colon])
-        assert len(key) <= 1
-        if len(key) == 1 and isinstance(key[0], numbers.Integral):
-            i = key[0]
-            key = (slice(i, None, None),) if i == -1 else (slice(i, i + 1, None),)
-        if a == 't':
-            return cls(self.t[key], x=x[key])
-        return cls(self.t, x=x[(Ellipsis,) + key])
+                             'mode ' + repr(mode))
+        cls = type(self)
+        if mode == 'v':
+            return cls(self.t, x=x[(slice(None),) + rest + (slice(None),)])
+        if len(rest) > 1:
+            raise IndexError('process indexing error - mode {!r} takes one '
+                             'index, not {}'.format(mode, len(rest)))
+        sel = rest[0] if rest else slice(None)
+        if isinstance(sel, numbers.Integral):
+            sel = slice(sel, sel + 1 or None)
+        if mode == 't':
+            return cls(self.t[sel], x=x[sel])
+        return cls(self.t, x=x[..., sel])
 
     # properties (infrastructure.py:706-767) ------------------------------
 
@@ -890,30 +1001,40 @@ class process(np.ndarray):
         return dt.reshape(dt.shape + (1,) * (len(self.shape) - 1))
 
     def shapeas(self, vshape_or_process):
-        vshape = (vshape_or_process.vshape
-                  if isinstance(vshape_or_process, process)
-                  else vshape_or_process)
-        cls = type(self)
-        k = self.ndim - 2
-        h = len(vshape)
-        if h > k:
-            newshape = self.shape[:1] + (1,) * (h - k) + self.shape[1:]
+        """The same values with unit value axes added or removed after the
+        timeline axis, so that they broadcast against values of the given
+        shape (or against the given process); ValueError when an axis to
+        remove is not of size 1."""
+        target = vshape_or_process.vshape \
+            if isinstance(vshape_or_process, process) else vshape_or_process
+        body = self.view(np.ndarray)
+        extra = (self.ndim - 2) - len(target)
+        if extra <= 0:
+            shape = body.shape[:1] + (1,) * -extra + body.shape[1:]
         else:
-            if h < k and set(self.shape[1:k - h + 1]) != {1}:
+            if any(n != 1 for n in body.shape[1:1 + extra]):
                 raise ValueError('could not reshape {} process values as {}'
-                                 .format(self.vshape, vshape))
-            newshape = self.shape[:1] + self.shape[k - h + 1:]
-        return cls(self.t, x=self.view(np.ndarray).reshape(newshape))
+                                 .format(self.vshape, target))
+            shape = body.shape[:1] + body.shape[1 + extra:]
+        return type(self)(self.t, x=body.reshape(shape))
+
+    def _clone(self, new_t, new_x, args):
+        t = self.t.copy(**args) if new_t else self.t
+        x = self.view(np.ndarray)
+        return type(self)(t=t, x=x.copy(**args) if new_x else x)
 
     def pcopy(self, **args):
-        return type(self)(t=self.t.copy(**args),
-                          x=self.view(np.ndarray).copy(**args))
+        """Copy of the timeline and of the values (``args`` go to
+        ``ndarray.copy``)."""
+        return self._clone(True, True, args)
 
     def xcopy(self, **args):
-        return type(self)(t=self.t, x=self.view(np.ndarray).copy(**args))
+        """Copy of the values on the same timeline."""
+        return self._clone(False, True, args)
 
     def tcopy(self, **args):
-        return type(self)(t=self.t.copy(**args), x=self.view(np.ndarray))
+        """Copy of the timeline over the same values."""
+        return self._clone(True, False, args)
 
     # summaries across paths (infrastructure.py:833-889) ------------------
 
@@ -1164,32 +1285,28 @@ def piecewise(t=0., *, x=None, v=None, dtype=None, mode='mid'):
     """A process that interpolates to a piecewise constant function of time
     (sdepy piecewise, infrastructure.py:1216-1281), for use as a time
     dependent parameter: value x[i] (or v[i]) on the segment that has t[i] as
-    its midpoint ('mid'), start ('forward') or end ('backward').  The forward
-    and backward forms repeat every time point, so that 'nearest'
-    interpolation jumps there (at the point itself the earlier segment's
-    value is returned); the returned timeline and values are an
-    implementation detail, as in sdepy."""
-    p = process(t, x=x, v=v, dtype=dtype)
-    t, xv = p.t, p.view(np.ndarray)
-    if mode == 'mid':
-        s, y = t, xv
-    elif mode in ('forward', 'backward'):
-        s = np.repeat(t, 2)
-        y = np.empty((2 * xv.shape[0],) + xv.shape[1:], dtype=xv.dtype)
-        if mode == 'forward':
-            # [t_i, t_{i+1}) -> x_i: each time point ends the previous
-            # segment and starts its own
-            y[0::2] = np.concatenate((xv[:1], xv[:-1]))
-            y[1::2] = xv
-        else:
-            # (t_{i-1}, t_i] -> x_i
-            y[0::2] = xv
-            y[1::2] = np.concatenate((xv[1:], xv[-1:]))
-    else:
+    its midpoint ('mid'), start ('forward') or end ('backward').  The returned
+    timeline and values are an implementation detail, as in sdepy: the
+    forward and backward forms hold every time point twice, the value of the
+    segment that ends there and then the value of the one that starts
+    there, and interpolate 'nearest', so that at a point itself the earlier
+    segment's value is returned (scipy's tie rule)."""
+    base = process(t, x=x, v=v, dtype=dtype)
+    if mode not in ('mid', 'forward', 'backward'):
         raise ValueError(
             "mode should be one of 'mid', 'forward', 'backward', "
             'but {} was given'.format(mode))
-    out = process(s, x=y, dtype=dtype)
+    tt, vals = base.t, base.view(np.ndarray)
+    if mode == 'mid':
+        out = process(tt, x=vals, dtype=dtype)
+    else:
+        if mode == 'forward':       # [t_i, t_i+1) -> x_i
+            ending, starting = np.concatenate((vals[:1], vals[:-1])), vals
+        else:                       # (t_i-1, t_i] -> x_i
+            ending, starting = vals, np.concatenate((vals[1:], vals[-1:]))
+        body = np.stack((ending, starting), axis=1).reshape(
+            (2 * tt.size,) + vals.shape[1:])
+        out = process(np.repeat(tt, 2), x=body, dtype=dtype)
     out.interp_kind = 'nearest'
     return out
 

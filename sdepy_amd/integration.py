@@ -15,12 +15,14 @@ integration.py:1081-1148), timeline validation and step consolidation
 start (SDE.A, :1216-1225) and at mid-step for correlations and intensities
 (infrastructure.py:1540, :1629).  Device work: everything per path and step.
 
-Sources: the default ``wiener_source`` / ``cpoisson_source(y=norm_rv)``
-draw inside the kernel.  Any other object complying with sdepy's source
-protocol (a ``process``, a user callable with ``paths`` and ``vshape``, a
-compound Poisson source with another jump law) is honoured by calling it on
-the host once per step, in sorted source-id order (SDE.dZ, :1227-1234), and
-replaying the recorded increments in the kernel.
+Sources: the default ``wiener_source`` and ``cpoisson_source`` draw inside
+the kernel, the latter with any of sdepy's jump laws (``norm_rv``,
+``uniform_rv``, ``exp_rv``, ``double_exp_rv``, constant or callable in t).
+Any other object complying with sdepy's source protocol (a ``process``, a
+user callable with ``paths`` and ``vshape``, a compound Poisson source with
+another ``y``, such as an ``rvmap``) is honoured by calling it on the host
+once per step, in sorted source-id order (SDE.dZ, :1227-1234), and replaying
+the recorded increments in the kernel.
 """
 import ctypes
 import inspect
@@ -361,21 +363,23 @@ class _gpu_sde:
         params = self._param_rows(t0s, ds)
         dj = self.sources.get('dj')
         self._anti_dj = False
-        pois = ab = None
+        pois = laws = None
         if dj is not None:
-            gdj = dj
-            if isinstance(dj, even_cpoisson_source) and \
-                    isinstance(dj.half, cpoisson_source) and dj.half.gpu_jumps:
-                # antithetic halves: the trailing paths repeat the jumps
-                gdj = dj.half
-                prob.anti_j = gdj.paths
-                self._anti_dj = True
-            if isinstance(gdj, cpoisson_source) and gdj.gpu_jumps:
+            # a compound Poisson source whose law the kernel draws (the
+            # leading half of an antithetic one: the trailing paths repeat
+            # its jumps)
+            gdj = dj.half if isinstance(dj, even_cpoisson_source) else dj
+            if isinstance(gdj, cpoisson_source):
+                laws = gdj.kernel_laws(t0s, ds)
+            if laws is not None:
+                if gdj is not dj:
+                    prob.anti_j = gdj.paths
+                    self._anti_dj = True
                 prob.key_j[0], prob.key_j[1] = gdj.key
                 prob.epoch_j = gdj.next_epoch()
                 gdj.dn.epoch = gdj.epoch
                 pois = gdj.dn.lam_table(t0s, ds)
-                ab = gdj.ab_table(t0s, ds)
+                prob.jump_law = laws[0]
             else:
                 host_sources['dj'] = dj
         self._host_dj = 'dj' in host_sources
@@ -443,9 +447,14 @@ class _gpu_sde:
             wtab = self._fold(wtab, gpar, has_corr)
             tab = np.concatenate((wtab, gpar), axis=1)
             if dj is not None:
-                extra = np.zeros((nsteps, 2 * mg)) if ab is None else \
-                    np.concatenate((ab[:, g0:g0 + mg, 0], ab[:, g0:g0 + mg, 1]),
-                                   axis=1)
+                # the jump law's parameter blocks (p0, p1, and p2 but for the
+                # normal law); zeros when the increments are replayed
+                if laws is None:
+                    extra = np.zeros((nsteps, 2 * mg))
+                else:
+                    nb = 2 if laws[0] == _lib.JUMP_NORMAL else 3
+                    extra = np.concatenate([laws[1][:, g0:g0 + mg, q]
+                                            for q in range(nb)], axis=1)
                 tab = np.concatenate((tab, extra), axis=1)
             gp.ncoef = tab.shape[1]
             if nsteps:
@@ -1068,9 +1077,8 @@ class merton_jumpdiff_process(jumpdiff_process):
 
 class kou_jumpdiff_process(jumpdiff_process):
     """kou_jumpdiff_process: jumpdiff_process with double exponential jump
-    sizes (sdepy kou_jumpdiff_SDE, integration.py:2734-2770); counts are drawn
-    on the GPU, sizes on the host through ``rvs``, and the increments are
-    replayed in the kernel."""
+    sizes (sdepy kou_jumpdiff_SDE, integration.py:2734-2770); counts and
+    sizes are drawn inside the Euler kernel."""
     dj_defaults = {'dj': None, 'dn': None, 'ptype': int, 'lam': 1.,
                    'a': .5, 'b': .5, 'pa': .5}
 

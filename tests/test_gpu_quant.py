@@ -12,8 +12,8 @@ the reference's parameter sets (test_quant.py:198-216) and step counts.
 Where Euler's own moments are known exactly the comparison is with them, so
 the discretisation bias is not mistaken for an error:
   * wiener, lognorm (log-Euler), Merton and Kou (log-Euler with compound
-    Poisson increments; Kou's double-exponential sizes drawn on the host and
-    replayed) are exact in distribution for constant parameters;
+    Poisson increments; Kou's double-exponential sizes drawn in the kernel)
+    are exact in distribution for constant parameters;
   * OU and Hull-White are linear recursions: oracle.closed_forms.
     ou_euler_moments;
   * CIR and Heston's variance follow y' = y + k (theta - y+) ds + xi sqrt(y+) dw,
@@ -169,14 +169,16 @@ def test_merton_exact_in_distribution(gpu):
 
 
 def test_kou_exact_in_distribution(gpu):
-    """Kou's double-exponential jumps are drawn on the host (through the
-    source protocol) and replayed in the kernel; the log-Euler scheme with
-    compound Poisson increments is exact in distribution.  2e5 paths (the
-    host draws each step's jumps)."""
-    Pk = 200_000
+    """Kou's double-exponential jumps are drawn inside the Euler kernel (no
+    recorded increments); the log-Euler scheme with compound Poisson
+    increments is exact in distribution.  1M paths."""
+    Pk = P
     prm = dict(x0=6., mu=.1, sigma=.3, lam=2., pa=.8, a=.2, b=.3)
-    x = sa.kou_jumpdiff_process(paths=Pk, dtype=np.float32, steps=200,
-                                rng=np.random.default_rng(108), **prm)(S)
+    proc = sa.kou_jumpdiff_process(paths=Pk, dtype=np.float32, steps=200,
+                                   rng=np.random.default_rng(108), **prm)
+    run = proc.prepare(S)
+    assert not run.passes[0].probs[0].replay_dj
+    x = run.run()
     pm = x.pmean()
     for i in (1, 2):
         xs = np.asarray(x[i], np.float64)

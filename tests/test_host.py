@@ -256,7 +256,24 @@ def test_preset_argument_handling_and_tables():
     he = sa.heston_process(paths=4)
     assert he.xshape == () and he._output_map() == (1, _lib.OUT_FIRST, 1, 1)
     m = sa.merton_jumpdiff_process(paths=4, lam=2., a=-.1, b=.2)
-    assert m.sources['dj'].gpu_jumps
+    law, tab = m.sources['dj'].kernel_laws(np.zeros(3), np.full(3, .1))
+    assert law == _lib.JUMP_NORMAL and np.allclose(tab, [[-.1, .2, 0.]])
+    # Kou, the default uniform y, an exponential and a time-dependent law
+    # are drawn in the kernel too; an rvmap'd law is not
+    k = sa.kou_jumpdiff_process(paths=4, a=.3, b=.2, pa=.6)
+    law, tab = k.sources['dj'].kernel_laws(np.zeros(2), np.full(2, .1))
+    assert law == _lib.JUMP_DOUBLE_EXP and np.allclose(tab, [[.3, .2, .6]])
+    j = sa.jumpdiff_process(paths=4)
+    law, tab = j.sources['dj'].kernel_laws(np.zeros(2), np.full(2, .1))
+    assert law == _lib.JUMP_UNIFORM and np.allclose(tab, [[0., 1., 0.]])
+    e = sa.jumpdiff_process(paths=4, y=sa.exp_rv(-.25))
+    law, tab = e.sources['dj'].kernel_laws(np.zeros(2), np.full(2, .1))
+    assert law == _lib.JUMP_DOUBLE_EXP and np.allclose(tab, [[-.25, 1., 1.]])
+    td = sa.jumpdiff_process(paths=4, y=sa.norm_rv(a=lambda t: t, b=.1))
+    law, tab = td.sources['dj'].kernel_laws(np.array([0., 1.]), np.array([1., 2.]))
+    assert law == _lib.JUMP_NORMAL and np.allclose(tab[:, 0, 0], [.5, 2.])
+    mapped = sa.jumpdiff_process(paths=4, y=sa.rvmap(np.exp, sa.norm_rv()))
+    assert mapped.sources['dj'].kernel_laws(np.zeros(2), np.full(2, .1)) is None
     x0 = sa.ornstein_uhlenbeck_process(paths=3, x0=np.array([1., 2., 3.]))
     assert x0._x0_state(np.float64).shape == (1, 3)     # per-path initial values
 
@@ -315,7 +332,7 @@ def test_antithetic_sources_host_side():
     assert isinstance(dw.half, sa.wiener_source)
     assert not isinstance(dw, sa.wiener_source)
     dj = sa.even_cpoisson_source(paths=6, lam=2., y=sa.norm_rv(0., 1.))
-    assert dj.half.gpu_jumps and dj.paths == 6
+    assert dj.half.kernel_laws(np.zeros(1), np.ones(1)) is not None and dj.paths == 6
     assert sa.odd_dw is sa.odd_wiener_source and sa.even_dj is sa.even_cpoisson_source
     # the oracle's restatement of the wrapper
     from oracle import euler as ora
@@ -406,3 +423,58 @@ def test_batched_takes_float32_processes_only():
                              dtype=np.float32, rng=np.random.default_rng(1))
     d, tab = batched._instance(p32, (0., 1.))
     assert d.nsteps == 4 and tab.shape[0] == 4
+
+
+@pytest.mark.parametrize('make,density,support', [
+    (lambda: sa.norm_rv(-.1, .15),
+     lambda y: np.exp(-(y + .1) ** 2 / (2 * .15 ** 2)) / (.15 * np.sqrt(2 * np.pi)),
+     (-3., 3.)),
+    (lambda: sa.uniform_rv(-.2, .1), lambda y: np.full_like(y, 1 / .3), (-.2, .1)),
+    (lambda: sa.exp_rv(.25), lambda y: np.exp(-y / .25) / .25, (0., 20.)),
+    (lambda: sa.exp_rv(-.25), lambda y: np.exp(y / .25) / .25, (-20., 0.)),
+    (lambda: sa.double_exp_rv(.2, .3, .6),
+     lambda y: np.where(y >= 0, .6 * np.exp(-np.abs(y) / .2) / .2,
+                        .4 * np.exp(-np.abs(y) / .3) / .3), (-20., 20.)),
+])
+def test_jump_law_closed_forms_and_host_draws(make, density, support):
+    """mean, var, std and exp_mean of each jump law against quadrature of
+    its density, and host draws (rvs) within 4 SE of the mean and var."""
+    import scipy.integrate
+    law = make()
+    lo, hi = support
+    pts = [0.] if lo < 0 < hi else None
+
+    def q(f):
+        return scipy.integrate.quad(lambda y: f(y) * density(np.array(y)), lo, hi,
+                                    points=pts, limit=200)[0]
+    m = q(lambda y: y)
+    v = q(lambda y: (y - m) ** 2)
+    assert abs(q(lambda y: 1.) - 1) < 1e-7
+    assert abs(law.mean() - m) < 1e-7 and abs(law.var() - v) < 1e-7
+    assert abs(law.std() - np.sqrt(v)) < 1e-7
+    assert abs(law.exp_mean() - q(np.exp)) < 1e-6 * q(np.exp)
+    y = law.rvs(size=200_000, random_state=np.random.default_rng(3))
+    y2 = law.rvs(size=(400, 500), random_state=np.random.RandomState(4))
+    assert y2.shape == (400, 500)
+    for s in (y, y2.reshape(-1)):
+        assert abs(s.mean() - m) < 4 * np.sqrt(v / s.size)
+        d2 = (s - m) ** 2
+        assert abs(d2.mean() - v) < 4 * d2.std() / np.sqrt(s.size)
+
+
+def test_jump_law_arguments_and_time_dependence():
+    """Domain errors as sdepy's; callable parameters give a law of t; rvmap
+    maps the variates (with t first when f takes it)."""
+    with pytest.raises(ValueError):
+        sa.exp_rv(0.)
+    for bad in (dict(a=-1.), dict(b=0.), dict(pa=1.5)):
+        with pytest.raises(ValueError):
+            sa.double_exp_rv(**bad)
+    y = sa.double_exp_rv(a=lambda t: .1 + t, b=.3, pa=.5)
+    assert callable(y) and abs(y(1.).mean() - (.5 * 1.1 - .5 * .3)) < 1e-15
+    assert sa.exp_rv(2.).exp_mean() == np.inf
+    g = np.random.default_rng(0)
+    m = sa.rvmap(np.exp, sa.norm_rv(0., 1e-12))
+    assert np.allclose(m.rvs(5, random_state=g), 1.)
+    mt = sa.rvmap(lambda t, v: t + 0 * v, sa.uniform_rv())
+    assert np.allclose(mt(2.5).rvs(4, random_state=g), 2.5)

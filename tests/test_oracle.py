@@ -772,6 +772,73 @@ def _mutated_run_pass(old, new):
     return ns['_run_pass']
 
 
+def _dexp_cdf(a, b, pa):
+    def cdf(y):
+        y = np.asarray(y, float)
+        return np.where(y < 0, (1 - pa) * np.exp(np.minimum(y, 0.) / b),
+                        1 - pa * np.exp(-np.maximum(y, 0.) / a))
+    return cdf
+
+
+# law: (kernel code, (p0, p1, p2), CDF of one size, mean, variance)
+_JUMP_LAWS = {
+    'normal': (philox.JUMP_NORMAL, (-.1, .15, 0.), scipy.stats.norm(-.1, .15).cdf,
+               -.1, .15 ** 2),
+    'uniform': (philox.JUMP_UNIFORM, (-.2, .3, 0.), scipy.stats.uniform(-.2, .3).cdf,
+                -.05, .3 ** 2 / 12),
+    # exp_rv(-0.25): -0.25 E
+    'exp_negative': (philox.JUMP_DOUBLE_EXP, (-.25, 1., 1.),
+                     lambda y: np.minimum(np.exp(np.asarray(y, float) / .25), 1.),
+                     -.25, .25 ** 2),
+    # double_exp_rv(0.2, 0.3, 0.6)
+    'double_exp': (philox.JUMP_DOUBLE_EXP, (.2, .3, .6), _dexp_cdf(.2, .3, .6),
+                   .6 * .2 - .4 * .3, 2 * (.6 * .04 + .4 * .09) - (.6 * .2 - .4 * .3) ** 2),
+}
+
+
+@pytest.mark.parametrize('law', sorted(_JUMP_LAWS))
+def test_jump_total_laws_in_distribution(law):
+    """The oracle's jump sizes of each law (philox.jump_total): single sizes
+    follow the law's CDF (Kolmogorov-Smirnov), and sums of 5 sizes, which
+    reach into a second Philox block (normal, uniform) or a third (double
+    exponential), have 5 times the law's mean and variance (4 SE), as
+    independent sizes do."""
+    code, prm, cdf, mean, var = _JUMP_LAWS[law]
+    key = (0x51, 0x52)
+    ys = np.array([philox.jump_total(key, 0, 0, s, p, 1, code, *prm)
+                   for s in range(1500) for p in range(4)])
+    assert scipy.stats.kstest(ys, cdf).pvalue > 1e-3
+    sums = np.array([philox.jump_total(key, 1, 16, s, p, 5, code, *prm)
+                     for s in range(800) for p in range(4)])
+    n = sums.size
+    assert abs(sums.mean() - 5 * mean) < 4 * math.sqrt(5 * var / n)
+    dev2 = (sums - 5 * mean) ** 2
+    assert abs(dev2.mean() - 5 * var) < 4 * dev2.std() / math.sqrt(n)
+
+
+def _mutated_jump_total(old, new):
+    import inspect
+    src = inspect.getsource(philox.jump_total)
+    assert old in src
+    ns = dict(vars(philox))
+    exec(compile(src.replace(old, new), philox.__file__, 'exec'), ns)
+    return ns['jump_total']
+
+
+@pytest.mark.parametrize('mistake,law', [
+    ('u01(w[2 * q + 1]) <= p2', 'double_exp'),     # the side from E's own word
+    ('c2 | ((j >> 2) << 8)', 'uniform'),            # every jump in block 0
+])
+def test_planted_jump_law_mistakes_fail_the_pins(monkeypatch, mistake, law):
+    """A side chosen from the exponential's own word, or sizes past the
+    fourth read again from the first block, fail the distribution pins."""
+    new = {'u01(w[2 * q + 1]) <= p2': 'u01(w[2 * q]) <= p2',
+           'c2 | ((j >> 2) << 8)': 'c2'}[mistake]
+    monkeypatch.setattr(philox, 'jump_total', _mutated_jump_total(mistake, new))
+    with pytest.raises(AssertionError):
+        test_jump_total_laws_in_distribution(law)
+
+
 def _interleaved_unpack(self, X):
     if self.addaxis:
         return tuple(X[..., k, :] for k in range(2))
