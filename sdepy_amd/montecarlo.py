@@ -61,6 +61,43 @@ def _row_batches(rows):
         yield r0, min(rows, r0 + _MAX_ROWS)
 
 
+def _histogram_kde(x, density, bins, bandwidth, cumulative):
+    """Gaussian kernel estimate from a density histogram: bin b carries the
+    probability mass density_b * width_b, spread as a normal centred on the
+    bin's midpoint with standard deviation bandwidth * width_b.  Returns the
+    density (or the distribution function) of that mixture at x."""
+    width = np.diff(bins)
+    mass = density * width
+    centre = (bins[:-1] + bins[1:]) / 2
+    scale = bandwidth * width
+    z = (x[..., None] - centre) / scale
+    if cumulative:
+        k = (1 + scipy.special.erf(z / math.sqrt(2))) / 2
+        return (mass * k).sum(axis=-1)
+    k = np.exp(-z * z / 2) / math.sqrt(2 * math.pi)
+    return (mass * k / scale).sum(axis=-1)
+
+
+def _histogram_interp(x, density, bins, kind, cumulative):
+    """Interpolation of a density histogram: the distribution function
+    through (edge_k, mass below edge_k), 0 before the first edge and 1
+    after the last; or the density through the bin midpoints, with zeros
+    at the outer edges and outside."""
+    if cumulative:
+        nodes = bins
+        values = np.concatenate(([0.], np.cumsum(np.diff(bins) * density)))
+        outside = (0., 1.)
+    else:
+        nodes = np.concatenate((bins[:1], (bins[:-1] + bins[1:]) / 2, bins[-1:]))
+        values = np.concatenate(([0.], density, [0.]))
+        outside = 0.
+    f = scipy.interpolate.interp1d(nodes, values.astype(density.dtype, copy=False),
+                                   kind=kind, assume_sorted=True,
+                                   bounds_error=False, copy=False,
+                                   fill_value=outside)
+    return f(x)
+
+
 class montecarlo:
     """Summary statistics of Monte Carlo simulations (see sdepy.montecarlo):
     mean, standard deviation, standard error, skewness, kurtosis and 1-d
@@ -219,7 +256,6 @@ class montecarlo:
                           ws, nbytes):
         if self._bins is None:
             return
-        N = self.paths
         if isfirst:
             # int and named bins give equally spaced edges (np.histogram's
             # uniform_bins); explicit edges are searched
@@ -228,62 +264,69 @@ class montecarlo:
                                      nbytes)
             self._counts = np.empty(vshape, dtype=object)
             self._paths_outside = np.zeros(vshape, dtype=self.ctype)
-        nbs = {len(self._bins[i]) - 1 for i in np.ndindex(vshape)}
-        use = _USE[self._use]
-        lib = _lib.lib()
+            for i in np.ndindex(vshape):
+                self._counts[i] = np.zeros(len(self._bins[i]) - 1, self.ctype)
+        before = self.paths
+        # one device launch per distinct bin count: the rows (values) that
+        # share it are gathered with their edges
         idx = list(np.ndindex(vshape))
-        for nb in sorted(nbs):
-            sel = [k for k, i in enumerate(idx) if len(self._bins[i]) - 1 == nb]
-            e = np.stack([np.asarray(self._bins[idx[k]], dtype=np.float64)
-                          for k in sel])
-            de = _lib.to_device(e)
-            if len(sel) == rows:
-                sub = dx
-            else:
-                torch = _lib.torch_mod()
-                sub = dx.index_select(0, torch.as_tensor(sel, device=dx.device))
-            counts = _lib.empty((len(sel), nb + 1), np.int64)
-            for r0, r1 in _row_batches(len(sel)):
-                _lib.check(lib.sdegpu_mc_histogram(
-                    _lib.ptr(sub[r0:r1]), code, r1 - r0, n_all, use,
-                    _lib.ptr(de[r0:r1]), nb, int(self._uniform),
-                    _lib.ptr(counts[r0:r1]), _lib.stream()))
-            c = _lib.to_host(counts)
+        by_nbins = {}
+        for k, i in enumerate(idx):
+            by_nbins.setdefault(len(self._bins[i]) - 1, []).append(k)
+        for nb, sel in sorted(by_nbins.items()):
+            counts = self._device_counts(dx, code, n_all, nb, sel, idx, rows)
             for j, k in enumerate(sel):
-                i = idx[k]
-                ck = c[j, :nb].astype(self.ctype)
-                if isfirst:
-                    self._counts[i] = ck
-                    self._paths_outside[i] = N + m - ck.sum()
-                else:
-                    self._counts[i] = self._counts[i] + ck
-                    self._paths_outside[i] += m - ck.sum()
+                inside = counts[j, :nb].astype(self.ctype)
+                self._counts[idx[k]] = self._counts[idx[k]] + inside
+                self._paths_outside[idx[k]] += m - inside.sum()
         for i in idx:
-            if self._counts[i].sum() + self._paths_outside[i] != N + m:
+            if self._counts[i].sum() + self._paths_outside[i] != before + m:
                 raise RuntimeError(
                     'total number of cumulated paths inconsistent with stored '
                     'cumulated counts - may occur if a multiple valued '
                     'simulation is update in only part of its components')
 
+    def _device_counts(self, dx, code, n_all, nb, sel, idx, rows):
+        """Histogram counts [len(sel)][nb + 1] of the rows `sel` of the
+        device sample, each against its own nb-bin edges (the last column
+        counts the values outside them)."""
+        edges = _lib.to_device(np.stack([np.asarray(self._bins[idx[k]], np.float64)
+                                         for k in sel]))
+        if len(sel) == rows:
+            sub = dx
+        else:
+            torch = _lib.torch_mod()
+            sub = dx.index_select(0, torch.as_tensor(sel, device=dx.device))
+        counts = _lib.empty((len(sel), nb + 1), np.int64)
+        use = _USE[self._use]
+        for r0, r1 in _row_batches(len(sel)):
+            _lib.check(_lib.lib().sdegpu_mc_histogram(
+                _lib.ptr(sub[r0:r1]), code, r1 - r0, n_all, use,
+                _lib.ptr(edges[r0:r1]), nb, int(self._uniform),
+                _lib.ptr(counts[r0:r1]), _lib.stream()))
+        return _lib.to_host(counts)
+
     # -- indexing ------------------------------------------------------------
 
+    # per-value statistics, indexed along vshape by __getitem__
+    _PER_VALUE = ('_mean', '_center', '_vsum', '_counts', '_paths_outside')
+
     def __getitem__(self, i):
-        a = montecarlo()
-        a._paths = self._paths
-        a._use = self._use
-        a._uniform = getattr(self, '_uniform', False)
-        if (self._bins is None) or isinstance(self._bins, (int, str)):
-            a._bins = self._bins
-            a._range = self._range
-        else:
-            a._bins = self._bins[i]
-        if self.paths != 0:
-            a._mean = self._mean[i]
-            a._moments = tuple(moment[i] for moment in self._moments)
-            a._counts = self._counts[i] if self._counts is not None else None
-            a._paths_outside = (self._paths_outside[i]
-                                if self._counts is not None else None)
-        return a
+        """The statistics of the value(s) at index i of vshape.  The result
+        shares this object's path count; its statistics are views."""
+        sub = object.__new__(type(self))
+        sub.__dict__.update(self.__dict__)
+        for name in self._PER_VALUE:
+            v = self.__dict__.get(name)
+            if v is not None:
+                setattr(sub, name, v[i])
+        if self.__dict__.get('_moments') is not None:
+            sub._moments = tuple(mk[i] for mk in self._moments)
+        if self.__dict__.get('_sums') is not None:
+            sub._sums = self._sums[(slice(None),) + (i if isinstance(i, tuple) else (i,))]
+        if not (self._bins is None or isinstance(self._bins, (int, np.integer, str))):
+            sub._bins = self._bins[i]           # edges given or fixed per value
+        return sub
 
     # -- statistics (infrastructure.py:3035-3083) ----------------------------
 
@@ -323,10 +366,9 @@ class montecarlo:
         if self.paths == 0:
             return '<empty montecarlo object>'
         mean, err = np.asarray(self.mean()), np.asarray(self.stderr())
-        if mean.size == 1 and err.size == 1:
-            mean = mean.flatten()[0]
-            err = err.flatten()[0]
-        return repr(mean) + ' +/- ' + repr(err)
+        if mean.size == 1 and err.size == 1:    # one value prints as scalars
+            mean, err = mean.reshape(-1)[0], err.reshape(-1)[0]
+        return '{!r} +/- {!r}'.format(mean, err)
 
     # -- distributions (infrastructure.py:3090-3290) ---------------------------
 
@@ -335,16 +377,15 @@ class montecarlo:
         if self.paths == 0 or self._bins is None:
             raise ValueError('no distribution data available')
         counts, bins = self._counts, self._bins
-        if counts.dtype == object and counts.size > 1:
+        if counts.dtype != object:
+            return counts, bins
+        if counts.size != 1:
             raise IndexError(
                 'histograms and distributions must be invoked '
                 'on single-valued montecarlo instances; '
                 'use indexing to select the value to be addressed '
                 '(es. ``a[i].histogram()``)')
-        if counts.dtype == object and counts.size == 1:
-            counts = counts.flatten()[0]
-            bins = bins.flatten()[0]
-        return counts, bins
+        return counts.reshape(-1)[0], bins.reshape(-1)[0]
 
     def density_histogram(self):
         """Counts normalised so that sum(counts * diff(bins)) == 1."""
@@ -354,7 +395,7 @@ class montecarlo:
     @property
     def outpaths(self):
         """Data points fallen outside of the bins' boundaries."""
-        self.histogram()
+        self.histogram()            # one value only
         return self._paths_outside
 
     def outerr(self):
@@ -362,34 +403,12 @@ class montecarlo:
         return self.outpaths / self.paths
 
     def _pdf_or_cdf(self, x, *, method, bandwidth, kind, cdf_flag):
-        ncounts, bins = self.density_histogram()
+        density, bins = self.density_histogram()
         x = np.asarray(x)
         if method == 'gaussian_kde':
-            newaxes = (1,) * x.ndim
-            widths = np.diff(bins).reshape(newaxes + ncounts.shape) * bandwidth
-            midpoints = ((bins[:-1] + bins[1:]) / 2).reshape(newaxes + ncounts.shape)
-            weights = ncounts.reshape(newaxes + ncounts.shape)
-            u = (x[..., np.newaxis] - midpoints) / widths
-            if cdf_flag:
-                kernels = widths / bandwidth * (scipy.special.erf(u / math.sqrt(2)) + 1) / 2
-            else:
-                kernels = np.exp(-u * u / 2) / math.sqrt(2 * math.pi) / bandwidth
-            return (kernels * weights).sum(axis=-1)
+            return _histogram_kde(x, density, bins, bandwidth, cdf_flag)
         if method == 'interp':
-            if cdf_flag:
-                xx = bins
-                pp = np.empty((bins.size,), dtype=ncounts.dtype)
-                pp[0], pp[1:] = 0, (np.diff(bins) * ncounts).cumsum()
-                fill_value = (0., 1.)
-            else:
-                xx = np.empty((bins.size + 1,), dtype=bins.dtype)
-                xx[0], xx[1:-1], xx[-1] = bins[0], (bins[:-1] + bins[1:]) / 2, bins[-1]
-                pp = np.empty((bins.size + 1,), dtype=ncounts.dtype)
-                pp[0], pp[1:-1], pp[-1] = 0, ncounts, 0
-                fill_value = 0.
-            return scipy.interpolate.interp1d(
-                xx, pp, kind=kind, assume_sorted=True, bounds_error=False,
-                copy=False, fill_value=fill_value)(x)
+            return _histogram_interp(x, density, bins, kind, cdf_flag)
         raise ValueError("pdf or cdf method should be 'gaussian_kde' or "
                          "'interp', not {}".format(method))
 

@@ -478,3 +478,30 @@ def test_jump_law_arguments_and_time_dependence():
     assert np.allclose(m.rvs(5, random_state=g), 1.)
     mt = sa.rvmap(lambda t, v: t + 0 * v, sa.uniform_rv())
     assert np.allclose(mt(2.5).rvs(4, random_state=g), 2.5)
+
+
+def test_montecarlo_pdf_and_cdf_from_a_histogram():
+    """The kernel estimate spreads each bin's mass as a normal around its
+    midpoint: the pdf integrates to 1 and the cdf is its integral; the
+    interpolated cdf passes through the cumulated masses at the edges and
+    the interpolated pdf through the densities at the midpoints."""
+    import scipy.integrate
+    import importlib
+    M = importlib.import_module('sdepy_amd.montecarlo')
+    bins = np.array([-1., -.5, 0., .25, 1., 2.])
+    counts = np.array([3., 10., 7., 12., 4.])
+    density = counts / counts.sum() / np.diff(bins)
+    for bw in (.5, 1., 2.):
+        pdf = lambda x: M._histogram_kde(np.asarray(x), density, bins, bw, False)
+        cdf = lambda x: M._histogram_kde(np.asarray(x), density, bins, bw, True)
+        total = scipy.integrate.quad(pdf, -30, 30, points=list(bins), limit=200)[0]
+        assert abs(total - 1) < 1e-8
+        for x in (-.7, 0.1, 1.5):
+            assert abs(scipy.integrate.quad(pdf, -30, x, limit=200)[0] - cdf(x)) < 1e-8
+        assert abs(cdf(1e3) - 1) < 1e-12 and abs(cdf(-1e3)) < 1e-12
+    c = M._histogram_interp(bins, density, bins, 'linear', True)
+    assert np.allclose(c, np.concatenate(([0.], np.cumsum(counts) / counts.sum())))
+    assert M._histogram_interp(np.array([-5., 5.]), density, bins, 'linear', True).tolist() == [0., 1.]
+    mid = (bins[:-1] + bins[1:]) / 2
+    assert np.allclose(M._histogram_interp(mid, density, bins, 'linear', False), density)
+    assert M._histogram_interp(np.array([-5., 5.]), density, bins, 'linear', False).tolist() == [0., 0.]
