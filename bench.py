@@ -19,7 +19,10 @@ Printed as one JSON line:
               MFMA); HIP-event timing on the launch stream
   ou_fullpath the OU-4 correlated full-path kernel (1M x 1000, FP32), the
               HBM-bound case: stored bytes / kernel time vs 8 TB/s, and the
-              largest deviation from the FP64 run of the same paths
+              largest deviation from the FP64 run of the same paths;
+              fused_moments: the same run with the FP64 moments of every row
+              reduced by the kernel as it stores (prepare(moments=True)),
+              against the kernel plus a separate moments pass
   merton      Merton jump-diffusion 2M x 252 FP32 adjoint (price + d/d x0,
               mu, sigma, a, b), with the FP64 run of the same paths in SE units
               and its VALU issue rate;
@@ -31,7 +34,9 @@ Printed as one JSON line:
               timeline and vmean across the 4 values, on the device (HBM
               read passes)
   gbm_c1      GBM 10k x 252 FP64 full path + pmean/pvar (latency-bound):
-              kernel time and the end-to-end preset call
+              kernel time, the end-to-end preset call with the host process's
+              pmean/pvar, and the same through prepare(moments=True), whose
+              kernel reduces the rows as it stores them
   batched     1024 heterogeneous GBM / HW2F instances in one persistent
               launch; per-instance FP32 moments vs FP64 runs of the same paths;
               VALU issue rate vs the nominal ceiling (SQ_INSTS_VALU per launch
@@ -496,8 +501,21 @@ def main():
                          'frac': gbs / HBM_PEAK_GBS,
                          'traffic': OU_TRAFFIC_BYTES,
                          'bytes_per_path_step': 16}}
+        # the same run with the row moments reduced by the kernel as it
+        # stores (block partials) and merged after it: pmean/pvar of every
+        # row without another pass over the 16 GB
+        ouf = p.prepare(np.linspace(0., 1., 1001), moments=True)
+        f_ms, _ = timed(torch, lambda: ouf.launch(fresh=True), 10, 5)
+        out['ou_fullpath']['fused_moments'] = {
+            'ms': f_ms, 'over_plain_kernel': f_ms / ou_ms,
+            'partials_bytes': int(ouf._row_mom.numel() * 8),
+            'separate_pass_ms': None}
+        del ouf
+        torch.cuda.empty_cache()
         if 'summaries' in legs:
             out['summaries'] = _device_summaries(torch, ou)
+            out['ou_fullpath']['fused_moments']['separate_pass_ms'] = \
+                ou_ms + out['summaries']['moments']['ms']
         if 'ou64' in legs:
             # the same paths (key and last epoch) stepped in FP64
             p64 = sa.ornstein_uhlenbeck_process(
@@ -577,13 +595,26 @@ def main():
             x = c1_proc()(tt1)
             m, v = x.pmean()[-1], x.pvar()[-1]
         e2e = (time.perf_counter() - t0) / reps
+        # the same through a prepared run whose kernel reduces the rows as
+        # it stores them: the path array still comes back to the host, the
+        # moments without another upload or pass
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run = c1_proc().prepare(tt1, moments=True)
+            x2 = run.run()
+            m2, v2 = run.pmean()[-1], run.pvar()[-1]
+        e2e_fused = (time.perf_counter() - t0) / reps
         out['gbm_c1'] = {
             'workload': 'gbm_fullpath_10kx252_fp64',
             'kernel_ms': k_ms,
             'kernel_path_steps_per_s': 10_000 * 252 / (k_ms * 1e-3),
             'end_to_end_ms': e2e * 1e3,
+            'end_to_end_fused_moments_ms': e2e_fused * 1e3,
             'terminal_mean': float(np.asarray(m).ravel()[0]),
-            'terminal_var': float(np.asarray(v).ravel()[0])}
+            'terminal_var': float(np.asarray(v).ravel()[0]),
+            'fused_minus_host_moments': [
+                float(np.asarray(m2).ravel()[0] - np.asarray(m).ravel()[0]),
+                float(np.asarray(v2).ravel()[0] - np.asarray(v).ravel()[0])]}
         del prep, x
 
     # ---- 1024 heterogeneous GBM / 2-factor Hull-White instances ----

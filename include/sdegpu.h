@@ -170,6 +170,15 @@ typedef struct sdegpu_euler_problem {
                                  spread over S addresses; the caller sums the slots */
   int32_t jump_law;           /* JUMPDIFF: SDEGPU_JUMP_* law of the jump sizes drawn in the
                                  kernel (0, the normal law, for merton_jumpdiff_process) */
+  double* row_moments;        /* optional device [ceil(paths/256)][row_moments_rows *
+                                 out_stride][2]: with it the kernel also reduces every stored
+                                 value of each block of 256 paths to (mean, sum of squared
+                                 deviations) in FP64, written to
+                                 [block][row * out_stride + out0 + value];
+                                 sdegpu_row_moments_merge turns them into the rows'
+                                 moments (process.pmean/pvar of the result without another
+                                 pass over it, infrastructure.py:851-889) */
+  int32_t row_moments_rows;   /* output rows (nrows) when row_moments is given */
 } sdegpu_euler_problem;
 
 /* Library and device queries. */
@@ -204,6 +213,17 @@ int sdegpu_axis_reduce(const void* x, int32_t dtype, int64_t outer, int64_t n, i
 
 /* Euler pass, see sdegpu_euler_problem. */
 int sdegpu_euler(const sdegpu_euler_problem* prob, void* stream);
+
+/*
+ * Moments of rows stored by sdegpu_euler passes with row_moments set: the
+ * per-block partials [ceil(paths/256)][rows][2] merged block by block in
+ * order (Chan, Golub and LeVeque's pairwise update), giving mean[r] and
+ * m2[r] = sum over paths of (x - mean)^2 in FP64, as sdegpu_path_moments
+ * gives them for the stored array (process.pmean/pvar, infrastructure.py:
+ * 861-879).  Every row must have been stored by one of the passes.
+ */
+int sdegpu_row_moments_merge(const double* partials, int64_t rows, int64_t paths, double* mean,
+                             double* m2, void* stream);
 
 /*
  * Cross-path moments of each row of a [rows][paths] array:

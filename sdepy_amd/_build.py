@@ -60,7 +60,7 @@ def build(verbose=False, jobs=None):
     """Compile every HIP source and link libsdegpu.so; returns its path."""
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1), 8)
+    jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1), 16)
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + '.tmp'

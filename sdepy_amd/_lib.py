@@ -55,6 +55,7 @@ class EulerProblem(ctypes.Structure):
         ('comp0', c_int32), ('nvars_total', c_int32),
         ('out0', c_int32), ('out_stride', c_int32), ('jump_defer', c_int32),
         ('info_row_slots', c_int32), ('jump_law', c_int32),
+        ('row_moments', c_void_p), ('row_moments_rows', c_int32),
     ]
 
 
@@ -66,6 +67,8 @@ SIGNATURES = [
     ('sdegpu_last_error', ctypes.c_int, [ctypes.c_char_p, c_size_t]),
     ('sdegpu_device_count', ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     ('sdegpu_euler', ctypes.c_int, [ctypes.POINTER(EulerProblem), c_void_p]),
+    ('sdegpu_row_moments_merge', ctypes.c_int,
+     [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     ('sdegpu_moments_workspace', c_size_t, [c_int64, c_int64]),
     ('sdegpu_path_moments', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int64, c_void_p, c_void_p, c_void_p,

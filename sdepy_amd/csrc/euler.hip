@@ -4,10 +4,23 @@
 
 namespace sdegpu {
 
-// The instances without recorded increments and antithetic halves
-// (euler_plain.hip).
-EulerKernel<float> euler_plain_f32(int model, int m, int d, bool defer);
-EulerKernel<double> euler_plain_f64(int model, int m, int d, bool defer);
+// The kernel for a launch: the plain instance when no recorded increments,
+// antithetic halves or component groups are involved, with or without fused
+// row moments.
+template <typename T>
+static EulerKernel<T> choose(bool plain, bool mom, int model, int m, int d, bool defer);
+template <>
+EulerKernel<float> choose<float>(bool plain, bool mom, int model, int m, int d, bool defer) {
+  if (plain) return mom ? euler_set_plain_mom_f32(model, m, d, defer)
+                        : euler_set_plain_f32(model, m, d, defer);
+  return mom ? euler_set_gen_mom_f32(model, m, d, defer) : euler_set_gen_f32(model, m, d, defer);
+}
+template <>
+EulerKernel<double> choose<double>(bool plain, bool mom, int model, int m, int d, bool defer) {
+  if (plain) return mom ? euler_set_plain_mom_f64(model, m, d, defer)
+                        : euler_set_plain_f64(model, m, d, defer);
+  return mom ? euler_set_gen_mom_f64(model, m, d, defer) : euler_set_gen_f64(model, m, d, defer);
+}
 
 // exp_mask the model implies for nout stored values (see OutRule and
 // store_state): every value for lognorm/jumpdiff, the log-x values for
@@ -95,6 +108,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   if (pr->comp0 < 0 || m_total < m || pr->comp0 > m_total - m)
     return fail(SDEGPU_EINVAL, "component group %d..%d outside a state of %d components",
                 pr->comp0, pr->comp0 + m - 1, m_total);
+  if (pr->row_moments && pr->row_moments_rows < 1)
+    return fail(SDEGPU_EINVAL, "row_moments without row_moments_rows");
   if (pr->info_row_slots < 0 || pr->info_row_slots > 1024)
     return fail(SDEGPU_EINVAL, "info_row_slots must be in [0, 1024], not %d", pr->info_row_slots);
   if (pr->out0 < 0 || out_stride < pr->nout || pr->out0 > out_stride - pr->nout)
@@ -127,6 +142,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.info_rows = pr->info_rows;
   a.info_slots = pr->info_row_slots > 1 ? pr->info_row_slots : 1;
   a.jump_law = pr->model == SDEGPU_MODEL_JUMPDIFF ? pr->jump_law : SDEGPU_JUMP_NORMAL;
+  a.row_mom = pr->row_moments;
+  a.mom_rows = (int64_t)pr->row_moments_rows * out_stride;
   for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
   a.paths = pr->paths;
   a.anti_w = pr->anti_w;
@@ -158,17 +175,16 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   const bool plain =
       !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 && !grouped;
   const bool defer = jump && pr->jump_defer != 0;
+  const bool mom = pr->row_moments != nullptr;
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {
-    EulerKernel<float> k =
-        plain ? euler_plain_f32(pr->model, m, hd, defer) : pick_kernel<true, float>(pr->model, m, hd, defer);
+    EulerKernel<float> k = choose<float>(plain, mom, pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
                        pr->pois, (float*)pr->out);
   } else {
-    EulerKernel<double> k =
-        plain ? euler_plain_f64(pr->model, m, hd, defer) : pick_kernel<true, double>(pr->model, m, hd, defer);
+    EulerKernel<double> k = choose<double>(plain, mom, pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
                        pr->pois, (double*)pr->out);

@@ -53,7 +53,45 @@ struct EulerArgs {
   int32_t comp0, m_total, out0, out_stride;
   int32_t info_slots;  // counters per row of info_rows (>= 1)
   int32_t jump_law;    // SDEGPU_JUMP_*: the law of the jump sizes
+  double* row_mom;     // fused row moments: [gridDim.x][mom_rows][2] or null
+  int64_t mom_rows;    // output rows * out_stride
 };
+
+// Fused row moments (the MOM instances of k_euler).  Each wave buffers the
+// values it stores for up to kMomSlots (row, value) pairs in LDS columns of
+// its own, one per pair, 64 lanes each.  When they are full (and after the
+// loop) the wave reduces every column to the FP64 mean and sum of squared
+// deviations of its paths, shifted by the column's first value; then, after
+// one block barrier, wave 0 combines the 4 waves' sums and writes one
+// (mean, m2) per pair to
+// [block][row * out_stride + out0 + value].  sdegpu_row_moments_merge merges
+// the blocks of every row.  The waves' results are double-buffered, so that
+// one barrier per flush suffices.
+constexpr int kMomSlots = 16;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMomStride = 64 + 4;  // column stride: the 4-lane groups of 8
+                                    // columns read at once cover the 32 banks
+
+template <typename T>
+__device__ __forceinline__ T* moment_columns() {
+  __shared__ T cols[kWaves * kMomSlots * kMomStride];
+  return cols;
+}
+
+// [2][kWaves][kMomSlots][3]: each wave's (shift, sum of d, sum of d^2) of
+// every column, d the values minus the shift (the column's first value).
+// With the FP32 columns a block takes 20.5 KB of LDS, so 7 blocks share a
+// CU: measured faster than 8 (20 KB), with the fused kernel VALU-bound.
+__device__ __forceinline__ double* moment_wave_results() {
+  __shared__ double res[2 * kWaves * kMomSlots * 3];
+  return res;
+}
+
+// the rows of the buffered columns (written and read by wave 0 only)
+__device__ __forceinline__ int* moment_rows() {
+  __shared__ int rows[kMomSlots];
+  return rows;
+}
 
 // Steps per chunk so that a chunk consumes a whole number of Philox blocks:
 // the normal with index g = s*M + v of a path lives in block g/4, lane g%4.
@@ -101,14 +139,19 @@ struct OutRule {
 
 // Stores one output row.  Heston stores either the whole state (nout = M,
 // full_heston) or the N log-x components in vshape order (nout = M/2, heston).
-// Non-temporal stores: the output is not read again by this kernel.
-template <int MODEL, typename T, int M, int D, bool GRP>
+// Non-temporal stores: the output is not read again by this kernel.  Lanes
+// without a path (live false, MOM instances only) store nothing; with MOM
+// every stored value also goes to its LDS column (mcol + o * kMomStride).
+template <int MODEL, typename T, int M, int D, bool GRP, bool MOM>
 __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs& a, int row,
-                                            const T (&x)[M], int64_t gid) {
+                                            const T (&x)[M], int64_t gid, bool live, T* mcol) {
   const int64_t P = a.paths;
   T* o_row = GRP ? out + ((int64_t)row * a.out_stride + a.out0) * P + gid
                  : out + (int64_t)row * a.nout * P + gid;
-  auto put = [&](T v, int o) { __builtin_nontemporal_store(v, o_row + o * P); };
+  auto put = [&](T v, int o) {
+    if (!MOM || live) __builtin_nontemporal_store(v, o_row + o * P);
+    if constexpr (MOM) mcol[o * kMomStride] = v;
+  };
   if constexpr (MODEL == SDEGPU_MODEL_HESTON) {
     using L = HestonLayout<D>;
     if (a.nout == M) {
@@ -145,8 +188,98 @@ __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs
   }
 }
 
-template <int MODEL, typename T, int M, int D, bool GEN, bool DEFER = false>
-__global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __restrict__ coef,
+// Paths of wave w of this block (0 for waves past the last path).
+__device__ __forceinline__ int wave_paths(const EulerArgs& a, int w) {
+  const int64_t left = a.paths - ((int64_t)blockIdx.x * kBlock + (int64_t)w * 64);
+  return left <= 0 ? 0 : (left >= 64 ? 64 : (int)left);
+}
+
+// Reduces the nslots buffered columns (see kMomSlots); every thread of the
+// block must call it with the same arguments.  Within a wave, 4 lanes share
+// a column, lane j summing its values j, j + 4, ... minus the column's first
+// value (a conflict-free LDS walk),
+// then a fixed xor-shuffle tree combines the 4.  Wave 0 then moves every
+// wave's sums onto its own shift, d' = d + delta: sum d' = s1 + n delta,
+// sum d'^2 = s2 + delta (2 s1 + n delta), adds them and divides once.
+template <typename T>
+__device__ __forceinline__ void flush_row_moments(const EulerArgs& a, T* cols, double* wres,
+                                                  const int* rows, int nslots, int phase) {
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q = lane >> 2, j = lane & 3;
+  const int nw = wave_paths(a, w);
+  // the wave's own column writes come first (LDS operations of a wave run in order)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  double s1 = 0.0, s2 = 0.0, shift = 0.0;
+  if (q < nslots) {
+    // Every lane of the wave wrote its entry (the lanes without a path a
+    // copy of the last path), so the 16 loads are issued unconditionally.
+    // The column offset goes through an empty asm statement, so that the
+    // compiler cannot hoist the 16 addresses out of the step loop (they
+    // would hold 16 registers through it); the loads take immediate offsets
+    // from one address.
+    int off = (w * kMomSlots + q) * kMomStride;
+    asm volatile("" : "+v"(off));
+    const T* col = cols + off;
+    const T v0 = col[0];
+    const T* mine = col + j;
+    shift = (double)v0;
+    T v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = mine[i * 4];
+    if (nw < 64) {
+      // entries beyond the wave's paths count as the shift itself (d = 0);
+      // compared as constant < nw - j, so that no per-entry index is held
+      const int lim = nw - j;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = i * 4 < lim ? v[i] : v0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double d = (double)v[i] - shift;
+      s1 += d;
+      s2 = fma(d, d, s2);
+    }
+  }
+  s1 += __shfl_xor(s1, 2, 4);
+  s2 += __shfl_xor(s2, 2, 4);
+  s1 += __shfl_xor(s1, 1, 4);
+  s2 += __shfl_xor(s2, 1, 4);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (q < nslots && j == 0) {
+    double* res = wres + ((phase * kWaves + w) * kMomSlots + q) * 3;
+    res[0] = shift;
+    res[1] = s1;
+    res[2] = s2;
+  }
+  __syncthreads();
+  if (w == 0 && lane < nslots) {
+    const double* r0 = wres + (phase * kWaves * kMomSlots + lane) * 3;
+    const double c = r0[0];
+    double n = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const double* rv = r0 + v * kMomSlots * 3;
+      const double nv = (double)wave_paths(a, v);
+      const double delta = rv[0] - c;
+      t1 += fma(nv, delta, rv[1]);
+      t2 += fma(delta, fma(nv, delta, 2.0 * rv[1]), rv[2]);
+      n += nv;
+    }
+    const double mean_d = t1 / n;
+    const double m2 = t2 - t1 * mean_d;
+    const int k = lane / a.nout, o = lane - k * a.nout;
+    const int64_t r = (int64_t)rows[k] * a.out_stride + a.out0 + o;
+    double* dst = a.row_mom + ((int64_t)blockIdx.x * a.mom_rows + r) * 2;
+    dst[0] = c + mean_d;
+    dst[1] = m2 > 0.0 ? m2 : 0.0;
+  }
+}
+
+template <int MODEL, typename T, int M, int D, bool GEN, bool DEFER = false, bool MOM = false>
+__global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a, const T* __restrict__ coef,
                                                   const int32_t* __restrict__ store_row,
                                                   const double* __restrict__ pois,
                                                   T* __restrict__ out) {
@@ -157,8 +290,13 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
   constexpr bool kHeston = MODEL == SDEGPU_MODEL_HESTON;
   constexpr int N = kHeston ? M / 2 : M;
 
-  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (gid >= a.paths) return;
+  // With fused row moments every thread of the block takes part in the LDS
+  // reductions, so the lanes beyond the last path step a copy of the last
+  // path and store nothing; otherwise they leave at once.
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = tid < a.paths;
+  if (!MOM && !live) return;
+  const int64_t gid = live ? tid : a.paths - 1;
   const int64_t P = a.paths;
   const uint32_t path = (uint32_t)gid;
   // antithetic halves: the trailing paths replay the draws of the leading
@@ -182,7 +320,31 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 #pragma unroll
     for (int v = 0; v < M; ++v) x[v] = (T)a.x0[v];
   }
-  if (a.row0 >= 0) store_state<MODEL, T, M, D, GEN>(out, a, a.row0, x, gid);
+  // buffered (row, value) columns of the fused row moments
+  T* mcols = nullptr;
+  double* mres = nullptr;
+  int* mrows = nullptr;
+  int mk = 0, mkmax = 1, mphase = 0;
+  if constexpr (MOM) {
+    mcols = moment_columns<T>() + (threadIdx.x >> 6) * kMomSlots * kMomStride;
+    mres = moment_wave_results();
+    mrows = moment_rows();
+    mkmax = kMomSlots / a.nout;
+  }
+  auto store = [&](int row) {
+    store_state<MODEL, T, M, D, GEN, MOM>(
+        out, a, row, x, gid, live,
+        MOM ? mcols + mk * a.nout * kMomStride + (threadIdx.x & 63) : nullptr);
+    if constexpr (MOM) {
+      if (threadIdx.x == 0) mrows[mk] = row;
+      if (++mk == mkmax) {
+        flush_row_moments<T>(a, moment_columns<T>(), mres, mrows, mk * a.nout, mphase);
+        mk = 0;
+        mphase ^= 1;
+      }
+    }
+  };
+  if (a.row0 >= 0) store(a.row0);
 
   int cnt[M];
 #pragma unroll
@@ -408,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
       const int row = store_row[s];
       if (row >= 0) {
         if constexpr (kDefer) flush_jumps();
-        store_state<MODEL, T, M, D, GEN>(out, a, row, x, gid);
+        store(row);
         if constexpr (kJump) {
           // the interval's jump total: one atomic per wave, into one of the
           // row's info_slots counters (every wave reaches a row at about the
@@ -417,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
           // lanes are all here; the lanes with jumps are summed by a scalar
           // walk of their ballot.
           if (a.info_rows) {
-            uint64_t todo = __ballot(interval_jumps != 0);
+            uint64_t todo = __ballot(live && interval_jumps != 0);
             if (todo) {
               long long sum = 0;
               while (todo) {
@@ -437,7 +599,10 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
     }
   }
 
-  if (a.info_paths && (kHeston || kJump)) {
+  if constexpr (MOM) {
+    if (mk > 0) flush_row_moments<T>(a, moment_columns<T>(), mres, mrows, mk * a.nout, mphase);
+  }
+  if (a.info_paths && (kHeston || kJump) && live) {
     // Heston counts per log-x value (the group's first is comp0 / 2), jump
     // counts per component
     const int64_t i0 = kHeston ? c0 / 2 : c0;
@@ -449,52 +614,65 @@ __global__ __launch_bounds__(kBlock) void k_euler(const EulerArgs a, const T* __
 template <typename T>
 using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, const double*, T*);
 
-template <bool GEN, int MODEL, typename T, bool DEFER = false>
+template <bool GEN, int MODEL, typename T, bool DEFER, bool MOM>
 EulerKernel<T> pick_m(int m) {
   switch (m) {
-    case 1: return k_euler<MODEL, T, 1, 1, GEN, DEFER>;
-    case 2: return k_euler<MODEL, T, 2, 1, GEN, DEFER>;
-    case 3: return k_euler<MODEL, T, 3, 1, GEN, DEFER>;
-    case 4: return k_euler<MODEL, T, 4, 1, GEN, DEFER>;
-    case 5: return k_euler<MODEL, T, 5, 1, GEN, DEFER>;
-    case 6: return k_euler<MODEL, T, 6, 1, GEN, DEFER>;
-    case 7: return k_euler<MODEL, T, 7, 1, GEN, DEFER>;
-    case 8: return k_euler<MODEL, T, 8, 1, GEN, DEFER>;
+    case 1: return k_euler<MODEL, T, 1, 1, GEN, DEFER, MOM>;
+    case 2: return k_euler<MODEL, T, 2, 1, GEN, DEFER, MOM>;
+    case 3: return k_euler<MODEL, T, 3, 1, GEN, DEFER, MOM>;
+    case 4: return k_euler<MODEL, T, 4, 1, GEN, DEFER, MOM>;
+    case 5: return k_euler<MODEL, T, 5, 1, GEN, DEFER, MOM>;
+    case 6: return k_euler<MODEL, T, 6, 1, GEN, DEFER, MOM>;
+    case 7: return k_euler<MODEL, T, 7, 1, GEN, DEFER, MOM>;
+    case 8: return k_euler<MODEL, T, 8, 1, GEN, DEFER, MOM>;
     default: return nullptr;
   }
 }
 
 // Heston kernels by (M, D): D log-x components per group of 2D.
-template <bool GEN, typename T>
+template <bool GEN, typename T, bool MOM>
 EulerKernel<T> pick_heston(int m, int d) {
   constexpr int H = SDEGPU_MODEL_HESTON;
   switch (m * 16 + d) {
-    case 2 * 16 + 1: return k_euler<H, T, 2, 1, GEN>;
-    case 4 * 16 + 1: return k_euler<H, T, 4, 1, GEN>;
-    case 4 * 16 + 2: return k_euler<H, T, 4, 2, GEN>;
-    case 6 * 16 + 1: return k_euler<H, T, 6, 1, GEN>;
-    case 6 * 16 + 3: return k_euler<H, T, 6, 3, GEN>;
-    case 8 * 16 + 1: return k_euler<H, T, 8, 1, GEN>;
-    case 8 * 16 + 2: return k_euler<H, T, 8, 2, GEN>;
-    case 8 * 16 + 4: return k_euler<H, T, 8, 4, GEN>;
+    case 2 * 16 + 1: return k_euler<H, T, 2, 1, GEN, false, MOM>;
+    case 4 * 16 + 1: return k_euler<H, T, 4, 1, GEN, false, MOM>;
+    case 4 * 16 + 2: return k_euler<H, T, 4, 2, GEN, false, MOM>;
+    case 6 * 16 + 1: return k_euler<H, T, 6, 1, GEN, false, MOM>;
+    case 6 * 16 + 3: return k_euler<H, T, 6, 3, GEN, false, MOM>;
+    case 8 * 16 + 1: return k_euler<H, T, 8, 1, GEN, false, MOM>;
+    case 8 * 16 + 2: return k_euler<H, T, 8, 2, GEN, false, MOM>;
+    case 8 * 16 + 4: return k_euler<H, T, 8, 4, GEN, false, MOM>;
     default: return nullptr;
   }
 }
 
 // defer: jumpdiff draws jump sizes at the next stored step (see k_euler)
-template <bool GEN, typename T>
-EulerKernel<T> pick_kernel(int model, int m, int d, bool defer) {
+template <bool GEN, typename T, bool MOM>
+EulerKernel<T> pick_kernel_mom(int model, int m, int d, bool defer) {
   switch (model) {
-    case SDEGPU_MODEL_WIENER: return pick_m<GEN, SDEGPU_MODEL_WIENER, T>(m);
-    case SDEGPU_MODEL_LOGNORM: return pick_m<GEN, SDEGPU_MODEL_LOGNORM, T>(m);
-    case SDEGPU_MODEL_OU: return pick_m<GEN, SDEGPU_MODEL_OU, T>(m);
-    case SDEGPU_MODEL_CIR: return pick_m<GEN, SDEGPU_MODEL_CIR, T>(m);
-    case SDEGPU_MODEL_HESTON: return pick_heston<GEN, T>(m, d);
+    case SDEGPU_MODEL_WIENER: return pick_m<GEN, SDEGPU_MODEL_WIENER, T, false, MOM>(m);
+    case SDEGPU_MODEL_LOGNORM: return pick_m<GEN, SDEGPU_MODEL_LOGNORM, T, false, MOM>(m);
+    case SDEGPU_MODEL_OU: return pick_m<GEN, SDEGPU_MODEL_OU, T, false, MOM>(m);
+    case SDEGPU_MODEL_CIR: return pick_m<GEN, SDEGPU_MODEL_CIR, T, false, MOM>(m);
+    case SDEGPU_MODEL_HESTON: return pick_heston<GEN, T, MOM>(m, d);
     case SDEGPU_MODEL_JUMPDIFF:
-      return defer ? pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, true>(m)
-                   : pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, false>(m);
+      return defer ? pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, true, MOM>(m)
+                   : pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, false, MOM>(m);
     default: return nullptr;
   }
 }
+
+// The instances are compiled in eight translation units of their own (so
+// that they build in parallel), one per (GEN, T, MOM): euler_inst_*.hip.
+#define SDEGPU_EULER_SET(NAME, GEN, T, MOM) \
+  EulerKernel<T> NAME(int model, int m, int d, bool defer)
+SDEGPU_EULER_SET(euler_set_gen_f32, true, float, false);
+SDEGPU_EULER_SET(euler_set_gen_f64, true, double, false);
+SDEGPU_EULER_SET(euler_set_plain_f32, false, float, false);
+SDEGPU_EULER_SET(euler_set_plain_f64, false, double, false);
+SDEGPU_EULER_SET(euler_set_gen_mom_f32, true, float, true);
+SDEGPU_EULER_SET(euler_set_gen_mom_f64, true, double, true);
+SDEGPU_EULER_SET(euler_set_plain_mom_f32, false, float, true);
+SDEGPU_EULER_SET(euler_set_plain_mom_f64, false, double, true);
 
 }  // namespace sdegpu

@@ -247,6 +247,86 @@ extern "C" int sdegpu_path_moments_noalloc(const void* x, int32_t dtype, int64_t
   return launch_moments(x, dtype, rows, paths, mean, m2, part, (hipStream_t)stream);
 }
 
+// ---- merge of the Euler kernels' fused row moments ------------------------
+
+namespace sdegpu {
+
+struct Moments {
+  double n, mean, m2;
+};
+
+// Pairwise update of Chan, Golub and LeVeque: the moments of the union of
+// two disjoint sets from those of each.
+__device__ __forceinline__ Moments merge_moments(Moments a, Moments b) {
+  if (a.n == 0.0) return b;
+  if (b.n == 0.0) return a;
+  const double n = a.n + b.n;
+  const double d = b.mean - a.mean;
+  Moments r;
+  r.n = n;
+  r.mean = a.mean + d * (b.n / n);
+  r.m2 = a.m2 + b.m2 + d * d * (a.n * (b.n / n));
+  return r;
+}
+
+// Rows in tiles of 8, one block per tile: thread (r, c) = (t % 8, t / 8)
+// merges the blocks c, c + 32, ... of row 8 * tile + r in order (the 8
+// threads of one c read the 8 rows of a block: 128 contiguous bytes), then a
+// fixed LDS tree merges each row's 32 results.
+__global__ __launch_bounds__(kBlock) void k_row_moments_merge(const double* __restrict__ part,
+                                                              int64_t nblocks, int64_t rows,
+                                                              int64_t paths,
+                                                              double* __restrict__ mean,
+                                                              double* __restrict__ m2) {
+  __shared__ double sn[kBlock], smean[kBlock], sm2[kBlock];
+  const int r = threadIdx.x & 7, c = threadIdx.x >> 3;
+  const int64_t row = (int64_t)blockIdx.x * 8 + r;
+  Moments acc = {0.0, 0.0, 0.0};
+  if (row < rows) {
+    for (int64_t b = c; b < nblocks; b += kBlock / 8) {
+      const int64_t first = b * kBlock;
+      const double n = (double)(paths - first < kBlock ? paths - first : kBlock);
+      const double* p = part + (b * rows + row) * 2;
+      acc = merge_moments(acc, Moments{n, p[0], p[1]});
+    }
+  }
+  sn[threadIdx.x] = acc.n;
+  smean[threadIdx.x] = acc.mean;
+  sm2[threadIdx.x] = acc.m2;
+  __syncthreads();
+  for (int w = kBlock / 16; w > 0; w >>= 1) {
+    if (c < w) {
+      const int t = threadIdx.x, u = threadIdx.x + 8 * w;
+      const Moments m = merge_moments(Moments{sn[t], smean[t], sm2[t]},
+                                      Moments{sn[u], smean[u], sm2[u]});
+      sn[t] = m.n;
+      smean[t] = m.mean;
+      sm2[t] = m.m2;
+    }
+    __syncthreads();
+  }
+  if (c == 0 && row < rows) {
+    mean[row] = smean[threadIdx.x];
+    m2[row] = sm2[threadIdx.x];
+  }
+}
+
+}  // namespace sdegpu
+
+extern "C" int sdegpu_row_moments_merge(const double* part, int64_t rows, int64_t paths,
+                                        double* mean, double* m2, void* stream) {
+  if (rows < 0 || paths < 1 || paths > kMaxPaths)
+    return fail(SDEGPU_EINVAL, "row_moments_merge: rows=%lld paths=%lld", (long long)rows,
+                (long long)paths);
+  if (rows == 0) return SDEGPU_OK;
+  if (rows > 0x7FFFFFFFll) return fail(SDEGPU_EINVAL, "row_moments_merge: too many rows");
+  if (!part || !mean || !m2) return fail(SDEGPU_EINVAL, "row_moments_merge: null pointer");
+  const int64_t nblocks = (paths + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_row_moments_merge, dim3((unsigned)((rows + 7) / 8)), dim3(kBlock), 0,
+                     (hipStream_t)stream, part, nblocks, rows, paths, mean, m2);
+  return launch_status("k_row_moments_merge");
+}
+
 extern "C" size_t sdegpu_payoff_workspace(int64_t paths) {
   if (paths < 1) return 0;
   return (size_t)(((paths + kChunk - 1) / kChunk) * 2 * sizeof(double));

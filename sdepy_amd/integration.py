@@ -290,11 +290,15 @@ class _gpu_sde:
     def __call__(self, timeline):
         return self.prepare(timeline).run()
 
-    def prepare(self, timeline):
+    def prepare(self, timeline, moments=False):
         """Validate the timeline, build the step tables, upload them and
         allocate the device output; returns a ``PreparedRun`` whose
         ``launch()`` issues the kernels and ``result()`` returns the
-        process(es).  ``__call__`` is ``prepare(timeline).run()``."""
+        process(es).  ``__call__`` is ``prepare(timeline).run()``.
+        With ``moments=True`` the kernels also reduce every stored row to
+        its FP64 moments as they store it, so that the run's ``pmean``,
+        ``pvar``, ``psum`` and ``pstd`` need no further pass over the
+        output."""
         tt, tt_asis, steps_tt = consolidate_timeline(timeline, self.steps,
                                                      self.i0)
         i0 = self.i0
@@ -318,7 +322,10 @@ class _gpu_sde:
                                        code, cdt, xx_dev,
                                        (nout, out_mode, out_group, exp_mask))
                     for k, (sp, tp, row_of, reverse) in enumerate(passes)]
-        return PreparedRun(self, prepared, xx_dev, tt, tt_asis, odt)
+        run = PreparedRun(self, prepared, xx_dev, tt, tt_asis, odt)
+        if moments:
+            run._fuse_moments()
+        return run
 
     def _prepare_pass(self, steps_pass, tt_pass, row_of, reverse, first, x0,
                       code, cdt, xx_dev, outmap):
@@ -571,12 +578,33 @@ class PreparedRun:
     def __init__(self, sde, passes, xx_dev, tt, tt_asis, odt):
         self.sde, self.passes, self.xx_dev = sde, passes, xx_dev
         self.tt, self.tt_asis, self.odt = tt, tt_asis, odt
+        self._row_mom = None
+
+    def _fuse_moments(self):
+        """Have every pass reduce its stored rows as it stores them: block
+        partials [ceil(paths/256)][rows][2], merged after each launch into
+        the FP64 (mean, m2) of every row."""
+        n_t, nout, P = self.xx_dev.shape
+        rows = n_t * nout
+        nblocks = (P + 255) // 256
+        self._row_mom = _lib.empty((nblocks, rows, 2), np.float64)
+        self._mom_mean = _lib.empty((rows,), np.float64)
+        self._mom_m2 = _lib.empty((rows,), np.float64)
+        for p in self.passes:
+            for prob in p.probs:
+                prob.row_moments = self._row_mom.data_ptr()
+                prob.row_moments_rows = n_t
 
     def launch(self, fresh=False):
         for p in self.passes:
             if fresh:
                 p.advance_epochs()
             p.launch()
+        if self._row_mom is not None:
+            n_t, nout, P = self.xx_dev.shape
+            _lib.check(_lib.lib().sdegpu_row_moments_merge(
+                _lib.ptr(self._row_mom), n_t * nout, P, _lib.ptr(self._mom_mean),
+                _lib.ptr(self._mom_m2), _lib.stream()))
 
     def result(self):
         for p in self.passes:
@@ -612,6 +640,12 @@ class PreparedRun:
         return sde._result(self.tt_asis, xx)
 
     def _moments(self):
+        if self._row_mom is not None:
+            # reduced by the kernels as they stored the rows
+            shape = self.xx_dev.shape[:2]
+            return (float(self.xx_dev.shape[2]),
+                    _lib.to_host(self._mom_mean).reshape(shape),
+                    _lib.to_host(self._mom_m2).reshape(shape))
         from .infrastructure import path_moments
         rows = self._rows()
         f32 = rows.dtype == _lib.torch_mod().float32

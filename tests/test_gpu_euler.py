@@ -604,3 +604,76 @@ def test_c_abi_caller(gpu):
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stderr
     assert res.stdout.strip().endswith('ok')
+
+
+def _np_moments(x):
+    """FP64 mean and sum of squared deviations over the paths axis of every
+    stored (row, value), NumPy's two-pass reductions."""
+    xs = np.asarray(x, np.float64)
+    xs = xs.reshape(xs.shape[0], -1, xs.shape[-1])
+    mean = xs.mean(axis=-1)
+    return mean, ((xs - mean[..., None]) ** 2).sum(axis=-1)
+
+
+@pytest.mark.parametrize('case', ['ou4_fp64', 'ou4_fp32', 'one_path', 'ragged_257',
+                                  'heston_backward', 'lognorm_groups', 'kou_deferred'])
+def test_fused_row_moments_match_numpy(gpu, case):
+    """prepare(moments=True): the kernels reduce every stored row as they
+    store it (FP64 block partials, merged in order); pmean/pvar of the run
+    equal NumPy's FP64 moments of the returned array to 1e-12 and the
+    separate reduction pass (moments=False) to 1e-13, and the stored paths
+    are those of the run without moments (1e-12 relative in FP64)."""
+    print('fused row moments', case, flush=True)
+    tt = np.linspace(0., 1., 21)
+    rng = np.random.default_rng(77)
+    C = np.array([[1., .3, -.2, .1], [.3, 1., .25, 0.], [-.2, .25, 1., .4], [.1, 0., .4, 1.]])
+    if case.startswith('ou4'):
+        make = lambda: sa.ornstein_uhlenbeck_process(
+            paths=3000, vshape=4, dtype=np.float64 if case == 'ou4_fp64' else np.float32,
+            x0=.1, theta=.2, k=1.5, sigma=.3, corr=C, steps=61, rng=np.random.default_rng(5))
+    elif case == 'one_path':
+        make = lambda: sa.lognorm_process(paths=1, dtype=np.float64, x0=1., mu=.1, sigma=.3,
+                                          rng=np.random.default_rng(6))
+    elif case == 'ragged_257':
+        make = lambda: sa.wiener_process(paths=257, vshape=3, dtype=np.float64,
+                                         rng=np.random.default_rng(7))
+    elif case == 'heston_backward':
+        tt = np.linspace(0., 1., 9)
+        make = lambda: sa.full_heston_process(paths=1000, vshape=2, dtype=np.float64, i0=3,
+                                              rho=[-.5, -.3], steps=33,
+                                              rng=np.random.default_rng(8))
+    elif case == 'lognorm_groups':
+        make = lambda: sa.lognorm_process(paths=700, vshape=(3, 4), dtype=np.float64,
+                                          x0=1., mu=.05, sigma=.2, rng=np.random.default_rng(9))
+    else:
+        tt = (0., .5, 1.)
+        make = lambda: sa.kou_jumpdiff_process(paths=2000, dtype=np.float64, lam=5., steps=101,
+                                               rng=np.random.default_rng(10))
+    fused = make().prepare(tt, moments=True)
+    if case == 'kou_deferred':
+        assert fused.passes[0].probs[0].jump_defer == 1
+    if case == 'lognorm_groups':
+        assert len(fused.passes[0].probs) == 2
+    x = fused.run()
+    m, v = fused.pmean(dtype=np.float64), fused.pvar(dtype=np.float64)
+    plain = make().prepare(tt)
+    y = plain.run()
+    pm, pv = plain.pmean(dtype=np.float64), plain.pvar(dtype=np.float64)
+    xs, ms, vs, ys, pms, pvs = (x, m, v, y, pm, pv) if isinstance(x, tuple) else \
+        ((x,), (m,), (v,), (y,), (pm,), (pv,))
+    for xi, mi, vi, yi, pmi, pvi in zip(xs, ms, vs, ys, pms, pvs):
+        # the instances with and without the moments are compiled apart, so
+        # the compiler may contract another product into an FMA
+        rel = _rel(xi, yi)
+        assert rel < (1e-12 if xi.dtype == np.float64 else 1e-5), rel
+        mean, m2 = _np_moments(xi)
+        scale = max(1., float(np.abs(mean).max()))
+        n = xi.shape[-1]
+        got_m = np.asarray(mi, np.float64).reshape(mean.shape)
+        got_v = np.asarray(vi, np.float64).reshape(mean.shape)
+        assert np.abs(got_m - mean).max() < 1e-12 * scale
+        assert np.abs(got_v - m2 / n).max() < 1e-12 * max(1., float((m2 / n).max()))
+        if xi.dtype == np.float64:
+            assert np.abs(got_m - np.asarray(pmi, np.float64).reshape(mean.shape)).max() < 1e-13 * scale
+            assert np.allclose(got_v, np.asarray(pvi, np.float64).reshape(mean.shape),
+                               rtol=1e-12, atol=1e-15)
