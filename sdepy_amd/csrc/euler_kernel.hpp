@@ -17,11 +17,12 @@
 // The per-step table is the same for every path, so its loads are scalar
 // (SMEM) loads served by the scalar cache, not per-lane traffic.
 //
-// The kernel template is instantiated twice per (model, dtype, M, D): with
-// GEN = true for every case (recorded increments, antithetic halves) in
-// euler.hip, and with GEN = false, where those paths are compiled out, in
-// euler_plain.hip.  The plain form needs fewer scalar registers (the
-// correlated OU-4 kernel: 92 instead of 106), which lets 8 waves share a
+// The kernel template is instantiated per (model, dtype, M, D) with
+// GEN = true for every case (recorded increments, antithetic halves,
+// component groups) and with GEN = false, where those paths are compiled
+// out, each with and without the fused row moments (MOM), in the
+// euler_inst_*.hip units.  The plain form needs fewer scalar registers (the
+// correlated OU-4 kernel: 93 instead of 106), which lets 8 waves share a
 // SIMD instead of 7.
 #pragma once
 #include "common.hpp"
