@@ -14,6 +14,10 @@ Printed as one JSON line:
               central finite differences on common random numbers, its
               throughput and the speedup of the adjoint over it
   accuracy    |FP32 adjoint - FP64 FD| / SE for the price and every Greek
+  epochs      the same over 8 independent epochs: each estimator's mean and
+              the SE from the spread across epochs (the pathwise v0/k/theta/
+              xi/rho estimators have heavy tails here, so one run's per-path
+              SE is unreliable), and the FP32-minus-FP64 adjoint differences
   roofline    the adjoint kernel: algorithmic FP32 flops / kernel time vs the
               MI355X FP32 vector peak (the kernel is VALU-bound, not HBM or
               MFMA); HIP-event timing on the launch stream
@@ -285,6 +289,51 @@ def _device_summaries(torch, run):
     return res
 
 
+def _epoch_spread(G, fd_ms, n_epochs=8):
+    """The C3 Greeks over n_epochs independent Philox epochs: per estimator
+    (FP32 adjoint, FP64 adjoint, FP64 FD) the mean over epochs and its
+    standard error from their spread; the FP32 and FP64 FD means apart in
+    units of that SE; and the FP32-minus-FP64 adjoint difference on the same
+    paths, in units of the FP64 run's own per-path SE and of the epoch
+    spread."""
+    names = ('price',) + tuple(G.HESTON_GREEKS)
+    kw = dict(paths=PATHS, steps=NSTEPS, strike=STRIKE, maturity=MATURITY,
+              key=(0x5eed, 0xc0ffee), **HPRM)
+    rows = {'fp32_adjoint': [], 'fp64_adjoint': [], 'fp64_fd': []}
+    diff_se64 = []
+    for e in range(n_epochs):
+        r32 = G.heston_greeks(dtype=np.float32, epoch=e, **kw)
+        r64 = G.heston_greeks(dtype=np.float64, epoch=e, **kw)
+        fd = G.HestonFD(dtype=np.float64, epoch=e, h=1e-4, **kw)
+        fd.launch()
+        p64, _, fdg = fd.result()
+        del fd
+        rows['fp32_adjoint'].append([r32.price] + [r32.greeks[k] for k in G.HESTON_GREEKS])
+        rows['fp64_adjoint'].append([r64.price] + [r64.greeks[k] for k in G.HESTON_GREEKS])
+        rows['fp64_fd'].append([p64] + [fdg[k] for k in G.HESTON_GREEKS])
+        diff_se64.append([(r32.greeks[k] - r64.greeks[k]) / r64.greeks_se[k]
+                          for k in G.HESTON_GREEKS])
+    res = {'epochs': n_epochs, 'paths_per_epoch': PATHS}
+    stats = {}
+    for est, v in rows.items():
+        v = np.asarray(v)
+        m, sd = v.mean(axis=0), v.std(axis=0, ddof=1) / math.sqrt(n_epochs)
+        stats[est] = (m, sd)
+        res[est] = {'mean': dict(zip(names, m)), 'se_from_spread': dict(zip(names, sd))}
+    m32, s32 = stats['fp32_adjoint']
+    mfd, sfd = stats['fp64_fd']
+    res['fp32_adjoint_minus_fd_over_spread_se'] = dict(zip(
+        names, (m32 - mfd) / np.sqrt(s32 ** 2 + sfd ** 2)))
+    d = np.asarray(diff_se64)
+    v32 = np.asarray(rows['fp32_adjoint'])[:, 1:]
+    v64 = np.asarray(rows['fp64_adjoint'])[:, 1:]
+    res['fp32_minus_fp64_adjoint'] = {
+        'max_abs_over_se64_per_epoch': dict(zip(G.HESTON_GREEKS, np.abs(d).max(axis=0))),
+        'mean_over_spread_se': dict(zip(G.HESTON_GREEKS, (v32 - v64).mean(axis=0) /
+                                        stats['fp64_adjoint'][1][1:]))}
+    return res
+
+
 def _cpu_name():
     try:
         with open('/proc/cpuinfo') as f:
@@ -473,6 +522,11 @@ def main():
             'fp64_adjoint_minus_fd_over_se64': {
                 k: (r64.greeks[k] - fdg[k]) / r64.greeks_se[k]
                 for k in G.HESTON_GREEKS}}
+        # 8 independent epochs (fresh paths) of the FP32 adjoint, the FP64
+        # adjoint and the FP64 FD set: the spread of each Greek across them
+        # estimates the standard error of one run where the pathwise
+        # estimator's variance is dominated by a few paths
+        out['epochs'] = _epoch_spread(G, fd_ms)
         del fd
         torch.cuda.empty_cache()
 
@@ -501,6 +555,23 @@ def main():
                          'frac': gbs / HBM_PEAK_GBS,
                          'traffic': OU_TRAFFIC_BYTES,
                          'bytes_per_path_step': 16}}
+        # the store ceiling of this output pattern measured in the same run:
+        # the same rows written with nothing computed (one thread per path,
+        # rows in order, non-temporal), and torch's fill of the buffer
+        lib = _lib.lib()
+        rows = int(ou.xx_dev.shape[0] * ou.xx_dev.shape[1])
+
+        def fill_rows():
+            _lib.check(lib.sdegpu_fill_rows(_lib.F32, float('nan'), _lib.ptr(ou.xx_dev),
+                                            rows, 10 ** 6, _lib.stream()))
+        fr_ms, _ = timed(torch, fill_rows, 10, 3)
+        tf_ms, _ = timed(torch, lambda: ou.xx_dev.fill_(float('nan')), 10, 3)
+        store_peak = nbytes / (min(fr_ms, tf_ms) * 1e-3) / 1e9
+        out['ou_fullpath']['roofline']['store_ceiling'] = {
+            'fill_rows_gb_per_s': nbytes / (fr_ms * 1e-3) / 1e9,
+            'torch_fill_gb_per_s': nbytes / (tf_ms * 1e-3) / 1e9,
+            'frac_of_measured_store_peak': gbs / store_peak}
+        ou.launch(fresh=True)
         # the same run with the row moments reduced by the kernel as it
         # stores (block partials) and merged after it: pmean/pvar of every
         # row without another pass over the 16 GB

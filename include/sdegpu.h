@@ -181,6 +181,17 @@ typedef struct sdegpu_euler_problem {
   int32_t row_moments_rows;   /* output rows (nrows) when row_moments is given */
 } sdegpu_euler_problem;
 
+/*
+ * Fills out [rows][paths] (dtype) with one value, one thread per path
+ * storing the rows in order with non-temporal stores: the pattern in which
+ * sdegpu_euler writes its output.  Replaces the NaN pre-fill of sdepy's
+ * output array (np.full(..., np.nan), integration.py:550-555) for callers
+ * that reuse an output buffer for runs that do not write every row; also the
+ * store-only ceiling of sdegpu_euler's output pattern.
+ */
+int sdegpu_fill_rows(int32_t dtype, double value, void* out, int64_t rows, int64_t paths,
+                     void* stream);
+
 /* Library and device queries. */
 int sdegpu_version(void);
 int sdegpu_last_error(char* buf, size_t len);

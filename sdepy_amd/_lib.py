@@ -75,6 +75,8 @@ SIGNATURES = [
       c_size_t, c_void_p]),
     ('sdegpu_path_moments_noalloc', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    ('sdegpu_fill_rows', ctypes.c_int,
+     [c_int32, c_double, c_void_p, c_int64, c_int64, c_void_p]),
     ('sdegpu_payoff_workspace', c_size_t, [c_int64]),
     ('sdegpu_payoff_moments', ctypes.c_int,
      [c_void_p, c_int32, c_int64, c_int32, c_double, c_double, c_void_p,

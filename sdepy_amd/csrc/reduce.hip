@@ -327,6 +327,43 @@ extern "C" int sdegpu_row_moments_merge(const double* part, int64_t rows, int64_
   return launch_status("k_row_moments_merge");
 }
 
+// ---- rows filled with one value, with the Euler kernel's store pattern ----
+
+namespace sdegpu {
+
+// One thread per path walking the rows in order, one non-temporal store per
+// row: the pattern in which k_euler writes its output, without the work
+// between the stores.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_rows(T* __restrict__ out, int64_t rows,
+                                                      int64_t paths, T value) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= paths) return;
+  T* o = out + p;
+  for (int64_t r = 0; r < rows; ++r) __builtin_nontemporal_store(value, o + r * paths);
+}
+
+}  // namespace sdegpu
+
+extern "C" int sdegpu_fill_rows(int32_t dtype, double value, void* out, int64_t rows,
+                                int64_t paths, void* stream) {
+  if (rows < 0 || paths < 1 || paths > kMaxPaths)
+    return fail(SDEGPU_EINVAL, "fill_rows: rows=%lld paths=%lld", (long long)rows,
+                (long long)paths);
+  if (rows == 0) return SDEGPU_OK;
+  if (!out) return fail(SDEGPU_EINVAL, "fill_rows: null output");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == SDEGPU_F32)
+    hipLaunchKernelGGL(k_fill_rows<float>, dim3(blocks_for(paths)), dim3(kBlock), 0, st,
+                       (float*)out, rows, paths, (float)value);
+  else if (dtype == SDEGPU_F64)
+    hipLaunchKernelGGL(k_fill_rows<double>, dim3(blocks_for(paths)), dim3(kBlock), 0, st,
+                       (double*)out, rows, paths, value);
+  else
+    return fail(SDEGPU_EUNSUPPORTED, "fill_rows: dtype code %d", dtype);
+  return launch_status("k_fill_rows");
+}
+
 extern "C" size_t sdegpu_payoff_workspace(int64_t paths) {
   if (paths < 1) return 0;
   return (size_t)(((paths + kChunk - 1) / kChunk) * 2 * sizeof(double));

@@ -677,3 +677,19 @@ def test_fused_row_moments_match_numpy(gpu, case):
             assert np.abs(got_m - np.asarray(pmi, np.float64).reshape(mean.shape)).max() < 1e-13 * scale
             assert np.allclose(got_v, np.asarray(pvi, np.float64).reshape(mean.shape),
                                rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_fill_rows(gpu, dtype):
+    """sdegpu_fill_rows writes every element of [rows][paths] (ragged last
+    block of paths included) and nothing beyond."""
+    from sdepy_amd import _lib
+    code = _lib.F32 if dtype == np.float32 else _lib.F64
+    rows, paths = 7, 1000
+    buf = _lib.to_device(np.zeros(rows * paths + 64, dtype=dtype))
+    _lib.check(_lib.lib().sdegpu_fill_rows(code, float('nan'), _lib.ptr(buf), rows, paths,
+                                           _lib.stream()))
+    h = _lib.to_host(buf)
+    assert np.isnan(h[:rows * paths]).all() and (h[rows * paths:] == 0).all()
+    _lib.check(_lib.lib().sdegpu_fill_rows(code, 2.5, _lib.ptr(buf), rows, paths, _lib.stream()))
+    assert (_lib.to_host(buf)[:rows * paths] == 2.5).all()
