@@ -8,15 +8,19 @@ namespace sdegpu {
 // antithetic halves or component groups are involved, with or without fused
 // row moments.
 template <typename T>
-static EulerKernel<T> choose(bool plain, bool mom, int model, int m, int d, bool defer);
+static EulerKernel<T> choose(bool plain, bool mom, bool pp, int model, int m, int d, bool defer);
 template <>
-EulerKernel<float> choose<float>(bool plain, bool mom, int model, int m, int d, bool defer) {
+EulerKernel<float> choose<float>(bool plain, bool mom, bool pp, int model, int m, int d,
+                                 bool defer) {
+  if (pp) return euler_set_pp_f32(model, m, d, defer);
   if (plain) return mom ? euler_set_plain_mom_f32(model, m, d, defer)
                         : euler_set_plain_f32(model, m, d, defer);
   return mom ? euler_set_gen_mom_f32(model, m, d, defer) : euler_set_gen_f32(model, m, d, defer);
 }
 template <>
-EulerKernel<double> choose<double>(bool plain, bool mom, int model, int m, int d, bool defer) {
+EulerKernel<double> choose<double>(bool plain, bool mom, bool pp, int model, int m, int d,
+                                   bool defer) {
+  if (pp) return euler_set_pp_f64(model, m, d, defer);
   if (plain) return mom ? euler_set_plain_mom_f64(model, m, d, defer)
                         : euler_set_plain_f64(model, m, d, defer);
   return mom ? euler_set_gen_mom_f64(model, m, d, defer) : euler_set_gen_f64(model, m, d, defer);
@@ -143,6 +147,7 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.info_slots = pr->info_row_slots > 1 ? pr->info_row_slots : 1;
   a.jump_law = pr->model == SDEGPU_MODEL_JUMPDIFF ? pr->jump_law : SDEGPU_JUMP_NORMAL;
   a.row_mom = pr->row_moments;
+  a.ppar = pr->param_paths;
   a.mom_rows = (int64_t)pr->row_moments_rows * out_stride;
   for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
   a.paths = pr->paths;
@@ -176,15 +181,19 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
       !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 && !grouped;
   const bool defer = jump && pr->jump_defer != 0;
   const bool mom = pr->row_moments != nullptr;
+  const bool pp = pr->param_paths != nullptr;
+  if (pp && mom)
+    return fail(SDEGPU_EUNSUPPORTED,
+                "fused row moments with per-path parameters: reduce the output separately");
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {
-    EulerKernel<float> k = choose<float>(plain, mom, pr->model, m, hd, defer);
+    EulerKernel<float> k = choose<float>(plain, mom, pp, pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
                        pr->pois, (float*)pr->out);
   } else {
-    EulerKernel<double> k = choose<double>(plain, mom, pr->model, m, hd, defer);
+    EulerKernel<double> k = choose<double>(plain, mom, pp, pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
                        pr->pois, (double*)pr->out);

@@ -56,6 +56,23 @@ struct EulerArgs {
   int32_t jump_law;    // SDEGPU_JUMP_*: the law of the jump sizes
   double* row_mom;     // fused row moments: [gridDim.x][mom_rows][2] or null
   int64_t mom_rows;    // output rows * out_stride
+  const void* ppar;    // per-path parameters (PP instances): [PathParams<MODEL>::NB][..][paths]
+};
+
+// Per-path parameters (the PP instances of k_euler): the model's parameter
+// blocks, constant in time, one value per (component, path), read once into
+// registers.  The blocks, each of M values (N = M/2 log-x values for Heston):
+//   WIENER   mu, sigma                LOGNORM, JUMPDIFF  mu - sigma^2/2, sigma
+//   OU       k, k theta, sigma        CIR                theta, k, xi
+//   HESTON   mu, sigma, theta, k, xi
+// OU forms its step's 1 - k ds and k theta ds from them in the kernel; the
+// others are the blocks of the step table themselves.  The jump law's
+// parameters stay in the step table.
+template <int MODEL>
+struct PathParams {
+  static constexpr int NB = MODEL == SDEGPU_MODEL_HESTON ? 5
+                            : (MODEL == SDEGPU_MODEL_OU || MODEL == SDEGPU_MODEL_CIR) ? 3
+                                                                                       : 2;
 };
 
 // Fused row moments (the MOM instances of k_euler).  Each wave buffers the
@@ -279,7 +296,8 @@ __device__ __forceinline__ void flush_row_moments(const EulerArgs& a, T* cols, d
   }
 }
 
-template <int MODEL, typename T, int M, int D, bool GEN, bool DEFER = false, bool MOM = false>
+template <int MODEL, typename T, int M, int D, bool GEN, bool DEFER = false, bool MOM = false,
+          bool PP = false>
 __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a, const T* __restrict__ coef,
                                                   const int32_t* __restrict__ store_row,
                                                   const double* __restrict__ pois,
@@ -312,6 +330,19 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
   const int64_t mt = GEN ? a.m_total : M;
   const int64_t c0 = GEN ? a.comp0 : 0;
   const bool grouped = GEN && (a.m_total != M || a.comp0 != 0);
+
+  // per-path parameters, read once (PP instances)
+  constexpr int NP = kHeston ? N : M;
+  constexpr int NBP = PathParams<MODEL>::NB;
+  T pp[PP ? NBP * NP : 1];
+  if constexpr (PP) {
+    const int64_t mtb = kHeston ? mt / 2 : mt, c0b = kHeston ? c0 / 2 : c0;
+#pragma unroll
+    for (int b = 0; b < NBP; ++b)
+#pragma unroll
+      for (int v = 0; v < NP; ++v)
+        pp[b * NP + v] = ((const T*)a.ppar)[((int64_t)b * mtb + c0b + v) * P + gid];
+  }
 
   T x[M];
   if (a.x0_paths) {
@@ -446,7 +477,7 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
       // OU with correlations: the host folds sigma_i * sqrt|ds| into the
       // Cholesky rows, and the update below applies them to z directly.
       constexpr bool kOU = MODEL == SDEGPU_MODEL_OU;
-      const bool ou_folded = kOU && a.has_corr && !replay;
+      const bool ou_folded = kOU && a.has_corr && !replay && !PP;
       T dw[M];
       if (replay) {
         const T* rp = (const T*)a.replay_dw + ((int64_t)s * mt + c0) * P + gid;
@@ -454,7 +485,7 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
         for (int v = 0; v < M; ++v) dw[v] = rp[v * P];
       } else {
         const T sq = c[1];
-        if (kOU && a.has_corr) {
+        if (ou_folded) {
           // folded into the OU update
         } else if (a.has_corr) {
           const T* L = c + 2;
@@ -473,6 +504,20 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
       }
 
       const T* p = c + a.poff;
+      T prow[PP ? NBP * NP : 1];
+      if constexpr (PP) {
+        // the step's parameter blocks from the path's own values
+#pragma unroll
+        for (int i = 0; i < NBP * NP; ++i) prow[i] = pp[i];
+        if constexpr (kOU) {
+#pragma unroll
+          for (int v = 0; v < M; ++v) {
+            prow[v] = fma(-pp[v], ds, (T)1);
+            prow[M + v] = pp[M + v] * ds;
+          }
+        }
+        p = prow;
+      }
       if constexpr (MODEL == SDEGPU_MODEL_WIENER || MODEL == SDEGPU_MODEL_LOGNORM) {
         // wiener_SDE: {'dt': mu, 'dw': sigma}; lognorm_SDE: {'dt': mu - sigma^2/2, 'dw': sigma}
 #pragma unroll
@@ -555,7 +600,7 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
                 for (int q = 0; q < kJumpEvents; ++q) jev[q] = njev == q ? e : jev[q];
                 ++njev;
               } else {
-                acc = jump_sum(s, v, n, p);
+                acc = jump_sum(s, v, n, c + a.poff);
               }
             }
             dj[v] = neg ? -acc : acc;
@@ -615,58 +660,61 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
 template <typename T>
 using EulerKernel = void (*)(const EulerArgs, const T*, const int32_t*, const double*, T*);
 
-template <bool GEN, int MODEL, typename T, bool DEFER, bool MOM>
+template <bool GEN, int MODEL, typename T, bool DEFER, bool MOM, bool PP = false>
 EulerKernel<T> pick_m(int m) {
   switch (m) {
-    case 1: return k_euler<MODEL, T, 1, 1, GEN, DEFER, MOM>;
-    case 2: return k_euler<MODEL, T, 2, 1, GEN, DEFER, MOM>;
-    case 3: return k_euler<MODEL, T, 3, 1, GEN, DEFER, MOM>;
-    case 4: return k_euler<MODEL, T, 4, 1, GEN, DEFER, MOM>;
-    case 5: return k_euler<MODEL, T, 5, 1, GEN, DEFER, MOM>;
-    case 6: return k_euler<MODEL, T, 6, 1, GEN, DEFER, MOM>;
-    case 7: return k_euler<MODEL, T, 7, 1, GEN, DEFER, MOM>;
-    case 8: return k_euler<MODEL, T, 8, 1, GEN, DEFER, MOM>;
+    case 1: return k_euler<MODEL, T, 1, 1, GEN, DEFER, MOM, PP>;
+    case 2: return k_euler<MODEL, T, 2, 1, GEN, DEFER, MOM, PP>;
+    case 3: return k_euler<MODEL, T, 3, 1, GEN, DEFER, MOM, PP>;
+    case 4: return k_euler<MODEL, T, 4, 1, GEN, DEFER, MOM, PP>;
+    case 5: return k_euler<MODEL, T, 5, 1, GEN, DEFER, MOM, PP>;
+    case 6: return k_euler<MODEL, T, 6, 1, GEN, DEFER, MOM, PP>;
+    case 7: return k_euler<MODEL, T, 7, 1, GEN, DEFER, MOM, PP>;
+    case 8: return k_euler<MODEL, T, 8, 1, GEN, DEFER, MOM, PP>;
     default: return nullptr;
   }
 }
 
 // Heston kernels by (M, D): D log-x components per group of 2D.
-template <bool GEN, typename T, bool MOM>
+template <bool GEN, typename T, bool MOM, bool PP = false>
 EulerKernel<T> pick_heston(int m, int d) {
   constexpr int H = SDEGPU_MODEL_HESTON;
   switch (m * 16 + d) {
-    case 2 * 16 + 1: return k_euler<H, T, 2, 1, GEN, false, MOM>;
-    case 4 * 16 + 1: return k_euler<H, T, 4, 1, GEN, false, MOM>;
-    case 4 * 16 + 2: return k_euler<H, T, 4, 2, GEN, false, MOM>;
-    case 6 * 16 + 1: return k_euler<H, T, 6, 1, GEN, false, MOM>;
-    case 6 * 16 + 3: return k_euler<H, T, 6, 3, GEN, false, MOM>;
-    case 8 * 16 + 1: return k_euler<H, T, 8, 1, GEN, false, MOM>;
-    case 8 * 16 + 2: return k_euler<H, T, 8, 2, GEN, false, MOM>;
-    case 8 * 16 + 4: return k_euler<H, T, 8, 4, GEN, false, MOM>;
+    case 2 * 16 + 1: return k_euler<H, T, 2, 1, GEN, false, MOM, PP>;
+    case 4 * 16 + 1: return k_euler<H, T, 4, 1, GEN, false, MOM, PP>;
+    case 4 * 16 + 2: return k_euler<H, T, 4, 2, GEN, false, MOM, PP>;
+    case 6 * 16 + 1: return k_euler<H, T, 6, 1, GEN, false, MOM, PP>;
+    case 6 * 16 + 3: return k_euler<H, T, 6, 3, GEN, false, MOM, PP>;
+    case 8 * 16 + 1: return k_euler<H, T, 8, 1, GEN, false, MOM, PP>;
+    case 8 * 16 + 2: return k_euler<H, T, 8, 2, GEN, false, MOM, PP>;
+    case 8 * 16 + 4: return k_euler<H, T, 8, 4, GEN, false, MOM, PP>;
     default: return nullptr;
   }
 }
 
 // defer: jumpdiff draws jump sizes at the next stored step (see k_euler)
-template <bool GEN, typename T, bool MOM>
+template <bool GEN, typename T, bool MOM, bool PP = false>
 EulerKernel<T> pick_kernel_mom(int model, int m, int d, bool defer) {
   switch (model) {
-    case SDEGPU_MODEL_WIENER: return pick_m<GEN, SDEGPU_MODEL_WIENER, T, false, MOM>(m);
-    case SDEGPU_MODEL_LOGNORM: return pick_m<GEN, SDEGPU_MODEL_LOGNORM, T, false, MOM>(m);
-    case SDEGPU_MODEL_OU: return pick_m<GEN, SDEGPU_MODEL_OU, T, false, MOM>(m);
-    case SDEGPU_MODEL_CIR: return pick_m<GEN, SDEGPU_MODEL_CIR, T, false, MOM>(m);
-    case SDEGPU_MODEL_HESTON: return pick_heston<GEN, T, MOM>(m, d);
+    case SDEGPU_MODEL_WIENER: return pick_m<GEN, SDEGPU_MODEL_WIENER, T, false, MOM, PP>(m);
+    case SDEGPU_MODEL_LOGNORM: return pick_m<GEN, SDEGPU_MODEL_LOGNORM, T, false, MOM, PP>(m);
+    case SDEGPU_MODEL_OU: return pick_m<GEN, SDEGPU_MODEL_OU, T, false, MOM, PP>(m);
+    case SDEGPU_MODEL_CIR: return pick_m<GEN, SDEGPU_MODEL_CIR, T, false, MOM, PP>(m);
+    case SDEGPU_MODEL_HESTON: return pick_heston<GEN, T, MOM, PP>(m, d);
     case SDEGPU_MODEL_JUMPDIFF:
-      return defer ? pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, true, MOM>(m)
-                   : pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, false, MOM>(m);
+      return defer ? pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, true, MOM, PP>(m)
+                   : pick_m<GEN, SDEGPU_MODEL_JUMPDIFF, T, false, MOM, PP>(m);
     default: return nullptr;
   }
 }
 
-// The instances are compiled in eight translation units of their own (so
-// that they build in parallel), one per (GEN, T, MOM): euler_inst_*.hip.
+// The instances are compiled in translation units of their own (so that
+// they build in parallel), one per (GEN, T, MOM) and the two with per-path
+// parameters (general, no fused moments): euler_inst_*.hip.
 #define SDEGPU_EULER_SET(NAME, GEN, T, MOM) \
   EulerKernel<T> NAME(int model, int m, int d, bool defer)
+SDEGPU_EULER_SET(euler_set_pp_f32, true, float, false);
+SDEGPU_EULER_SET(euler_set_pp_f64, true, double, false);
 SDEGPU_EULER_SET(euler_set_gen_f32, true, float, false);
 SDEGPU_EULER_SET(euler_set_gen_f64, true, double, false);
 SDEGPU_EULER_SET(euler_set_plain_f32, false, float, false);

@@ -273,6 +273,56 @@ class _gpu_sde:
         """Stored components of each process the preset returns."""
         return [list(range(self._output_map()[0]))]
 
+    # parameters that vary across paths ------------------------------------
+
+    # parameter blocks of the step table (of the working components; of the
+    # N log-x values for Heston)
+    n_param_blocks = 2
+
+    def _param_shape(self):
+        """Shape (without the paths axis) the SDE parameters broadcast to."""
+        return self.wshape
+
+    def _path_blocks(self, prm):
+        """The per-path parameter blocks (sdegpu_euler_problem.param_paths)
+        from the parameters broadcast to the parameter shape + (paths,)."""
+        return [prm['mu'], prm['sigma']]
+
+    def _path_params(self):
+        """None when every SDE parameter is the same on all paths (possibly
+        callable in t); else the FP64 per-path blocks [NB][M][paths] of the
+        kernels' per-path parameter instances.  Parameters may vary across
+        paths or in time, not both."""
+        shape, P = self._param_shape(), self.paths
+        per_path, timed, prm = False, False, {}
+        for name in self.sde_defaults:
+            z = self._args[name]
+            if callable(z):
+                timed = True
+                continue
+            a = np.asarray(z, dtype=float)
+            try:
+                np.broadcast_to(a, shape + (1,))
+            except ValueError:
+                try:
+                    np.broadcast_to(a, shape + (P,))
+                except ValueError:
+                    raise ValueError(
+                        'cannot broadcast parameter {} shaped {} to the '
+                        'process shape vshape + (paths,) = {}'.format(
+                            name, a.shape, shape + (P,)))
+                per_path = True
+            prm[name] = np.broadcast_to(a, shape + (P,))
+        if not per_path:
+            return None
+        if timed:
+            raise NotImplementedError(
+                'the GPU integrator takes parameters that vary across paths '
+                'or in time, not both')
+        n = int(np.prod(shape, dtype=int))
+        return np.stack([np.ascontiguousarray(b, dtype=float).reshape(n, P)
+                         for b in self._path_blocks(prm)])
+
     # per-parameter evaluation -------------------------------------------
 
     def _eval(self, name, t0s, shape):
@@ -367,7 +417,17 @@ class _gpu_sde:
             host_sources['dw'] = dw
         has_corr = wsrc is not None and wsrc.corr is not None
         groups = self._groups(has_corr, out_mode, out_group)
-        params = self._param_rows(t0s, ds)
+        path_params = self._path_params()
+        if path_params is None:
+            params = self._param_rows(t0s, ds)
+        else:
+            # the kernel takes the model's parameters from the per-path
+            # array; the table keeps the width of its parameter blocks
+            nb_width = self.n_param_blocks * int(np.prod(self._param_shape(), dtype=int))
+            params = np.zeros((nsteps, nb_width))
+            d_pp = _lib.to_device(path_params.astype(cdt))
+            keep.append(d_pp)
+            prob.param_paths = d_pp.data_ptr()
         dj = self.sources.get('dj')
         self._anti_dj = False
         pois = laws = None
@@ -451,7 +511,8 @@ class _gpu_sde:
                 wtab = np.stack((ds, np.sqrt(np.abs(ds))), axis=1) if nsteps \
                     else np.zeros((0, 2))
             gpar = self._group_params(params, g0, mg)
-            wtab = self._fold(wtab, gpar, has_corr)
+            if path_params is None:
+                wtab = self._fold(wtab, gpar, has_corr)
             tab = np.concatenate((wtab, gpar), axis=1)
             if dj is not None:
                 # the jump law's parameter blocks (p0, p1, and p2 but for the
@@ -584,6 +645,8 @@ class PreparedRun:
         """Have every pass reduce its stored rows as it stores them: block
         partials [ceil(paths/256)][rows][2], merged after each launch into
         the FP64 (mean, m2) of every row."""
+        if any(prob.param_paths for p in self.passes for prob in p.probs):
+            return      # per-path parameters: the separate reduction serves
         n_t, nout, P = self.xx_dev.shape
         rows = n_t * nout
         nblocks = (P + 255) // 256
@@ -857,6 +920,9 @@ class lognorm_process(_gpu_sde):
     model = _lib.MODEL_LOGNORM
     log = True
 
+    def _path_blocks(self, prm):
+        return [prm['mu'] - prm['sigma'] ** 2 / 2, prm['sigma']]
+
     def _param_rows(self, t0s, ds):
         mu = self._eval('mu', t0s, self.wshape)
         sigma = self._eval('sigma', t0s, self.wshape)
@@ -873,6 +939,10 @@ class ornstein_uhlenbeck_process(_gpu_sde):
     model = _lib.MODEL_OU
     init_defaults = {'x0': 0.}
     sde_defaults = {'theta': 0., 'k': 1., 'sigma': 1.}
+    n_param_blocks = 3
+
+    def _path_blocks(self, prm):
+        return [prm['k'], prm['k'] * prm['theta'], prm['sigma']]
 
     def _param_rows(self, t0s, ds):
         # rows (1 - k ds, k theta ds, sigma): x + k (theta - x) ds + sigma dw
@@ -932,6 +1002,10 @@ class cox_ingersoll_ross_process(_gpu_sde):
     model = _lib.MODEL_CIR
     init_defaults = {'x0': 1.}
     sde_defaults = {'theta': 1., 'k': 1., 'xi': 1.}
+    n_param_blocks = 3
+
+    def _path_blocks(self, prm):
+        return [prm['theta'], prm['k'], prm['xi']]
 
     def _param_rows(self, t0s, ds):
         w = self.wshape
@@ -955,6 +1029,13 @@ class full_heston_process(_gpu_sde):
     q = 2
     init_defaults = {'x0': 1., 'y0': 1.}
     sde_defaults = {'mu': 0., 'sigma': 1., 'theta': 1., 'k': 1., 'xi': 1.}
+    n_param_blocks = 5
+
+    def _param_shape(self):
+        return self.vshape
+
+    def _path_blocks(self, prm):
+        return [prm[n] for n in ('mu', 'sigma', 'theta', 'k', 'xi')]
 
     def shapes(self, vshape):
         self.addaxis = vshape == ()
@@ -1046,6 +1127,9 @@ class jumpdiff_process(_gpu_sde):
     log = True
     sources = ('dt', 'dw', 'dj')
     dj_defaults = {'dj': None, 'dn': None, 'ptype': int, 'lam': 1., 'y': None}
+
+    def _path_blocks(self, prm):
+        return [prm['mu'] - prm['sigma'] ** 2 / 2, prm['sigma']]
 
     def _dj_setup(self):
         a = self._args

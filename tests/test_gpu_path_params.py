@@ -1,0 +1,225 @@
+"""Parameters that vary across paths (GPU tests).
+
+sdepy's SDE parameters broadcast against vshape + (paths,)
+(integration.py:1688-1792, the shapes the SDE methods see), so a parameter
+array with a trailing paths axis gives every path its own value.  The GPU
+presets hand such parameters to the per-path instances of the Euler kernel,
+which read each path's parameter blocks once into registers; the oracle
+integrates the same arrays by broadcasting.  FP64 runs must equal the oracle
+to 1e-12 relative (absolute for the mean-reverting presets), FP32 runs to
+the stated tolerance, for lognorm (per-path mu and sigma), OU (per-path k and
+theta, uncorrelated and correlated with per-path sigma), CIR, Hull-White,
+Heston (per-path xi and k), Wiener and Merton's jump-diffusion; and a
+backward pass (i0 != 0) and moments over paths take them too.
+"""
+import numpy as np
+import pytest
+
+import sdepy_amd as sa
+from oracle import euler as ora
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-12
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def _corr(n, seed):
+    g = np.random.default_rng(seed)
+    A = g.standard_normal((n, n))
+    S = A @ A.T + n * np.eye(n)
+    d = 1 / np.sqrt(np.diag(S))
+    return S * d[:, None] * d[None, :]
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_lognorm_per_path_mu_sigma(gpu, dtype):
+    P, tt = 3000, np.linspace(0., 1., 21)
+    g = np.random.default_rng(1)
+    mu, sigma = g.uniform(-.1, .2, P), g.uniform(.05, .6, P)
+    p = sa.lognorm_process(paths=P, dtype=dtype, steps=81, x0=1.5, mu=mu, sigma=sigma,
+                           rng=np.random.default_rng(2))
+    run = p.prepare(tt)
+    assert run.passes[0].probs[0].param_paths
+    x = run.run()
+    ref = ora.integrate('lognorm', tt, paths=P, steps=81,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, ())},
+                        x0=1.5, mu=mu, sigma=sigma)
+    assert _rel(x, ref) < (RTOL64 if dtype == np.float64 else 3e-5)
+    # the per-path values matter: the same run with the mean parameters differs
+    q = sa.lognorm_process(paths=P, dtype=dtype, steps=81, x0=1.5, mu=mu.mean(),
+                           sigma=sigma.mean(), rng=np.random.default_rng(2))
+    assert _rel(q(tt), x) > 1e-2
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_ou_per_path_uncorrelated_and_correlated(gpu, dtype):
+    P, tt = 2048, np.linspace(0., 1., 26)
+    g = np.random.default_rng(3)
+    k, theta = g.uniform(.5, 3., (3, P)), g.uniform(-1., 1., (3, P))
+    sigma = g.uniform(.1, .5, (3, 1))
+    C = _corr(3, 4)
+    tol = 1e-12 if dtype == np.float64 else 2e-5
+    for corr in (None, C):
+        sig = sigma if corr is None else g.uniform(.1, .5, (3, P))
+        p = sa.ornstein_uhlenbeck_process(paths=P, vshape=3, dtype=dtype, steps=51, x0=.2,
+                                          theta=theta, k=k, sigma=sig, corr=corr,
+                                          rng=np.random.default_rng(5))
+        x = p(tt)
+        ref = ora.integrate('ou', tt, paths=P, vshape=(3,), steps=51,
+                            sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (3,),
+                                                            corr=corr)},
+                            x0=.2, theta=theta, k=k, sigma=sig)
+        assert x.shape == (26, 3, P)
+        assert np.max(np.abs(x - ref)) < tol, corr is None
+
+
+def test_cir_hull_white_wiener_per_path(gpu):
+    P, tt = 1500, np.linspace(0., 1., 11)
+    g = np.random.default_rng(6)
+    # CIR with per-path xi (large enough for negative excursions) and k
+    xi, k = g.uniform(.3, 1., P), g.uniform(.5, 2., P)
+    p = sa.cox_ingersoll_ross_process(paths=P, steps=41, x0=.05, theta=.04, k=k, xi=xi,
+                                      rng=np.random.default_rng(7))
+    x = p(tt)
+    ref = ora.integrate('cir', tt, paths=P, steps=41,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, ())},
+                        x0=.05, theta=.04, k=k, xi=xi)
+    assert (ref < 0).any()
+    assert np.max(np.abs(x - ref)) < 1e-12
+    # Hull-White, 2 correlated factors, per-path sigma of each factor
+    sigma = g.uniform(.005, .03, (2, P))
+    args = dict(x0=np.array([[.01], [.02]]), theta=np.array([[.03], [.01]]),
+                k=np.array([[.1], [.3]]), sigma=sigma)
+    p = sa.hull_white_process(paths=P, factors=2, steps=41, rho=.4,
+                              rng=np.random.default_rng(8), **args)
+    x = p(tt)
+    ref = ora.integrate('hull_white', tt, paths=P, factors=2, steps=41,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (2,),
+                                                        rho=.4)}, **args)
+    assert x.shape == (11, P)
+    assert np.max(np.abs(x - ref)) < 1e-14
+    # Wiener, vshape (2,), per-path drift of the first component only
+    mu = np.stack([g.uniform(-1, 1, P), np.full(P, .3)])
+    p = sa.wiener_process(paths=P, vshape=2, steps=41, x0=1., mu=mu, sigma=.5,
+                          rng=np.random.default_rng(9))
+    x = p(tt)
+    ref = ora.integrate('wiener', tt, paths=P, vshape=(2,), steps=41,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (2,))},
+                        x0=1., mu=mu, sigma=.5)
+    assert np.max(np.abs(x - ref)) < 1e-12
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_full_heston_per_path_xi_k(gpu, dtype):
+    P, n = 2000, 60
+    g = np.random.default_rng(10)
+    args = dict(x0=100., mu=.02, sigma=1., y0=.04, theta=.04,
+                k=g.uniform(1., 3., P), xi=g.uniform(.2, .8, P))
+    info = {}
+    p = sa.full_heston_process(paths=P, dtype=dtype, steps=n + 1, rho=-.7, info=info,
+                               rng=np.random.default_rng(11), **args)
+    x, y = p((0., .5, 1.))
+    ref_info = {}
+    rx, ry = ora.integrate('full_heston', (0., .5, 1.), paths=P, steps=n + 1, info=ref_info,
+                           sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (2,),
+                                                           rho=-.7)}, **args)
+    assert (ry < 0).any()
+    if dtype == np.float64:
+        assert _rel(x, rx) < 1e-10
+        assert np.max(np.abs(y - ry)) < 1e-12
+        assert np.array_equal(info['negative_y_count'], ref_info['negative_y_count'])
+    else:
+        # with xi up to 0.8 the variance often sits near zero, where the
+        # truncation max(y, 0) amplifies rounding: the bound is 4 times the
+        # deviation of the oracle itself run in FP32 (about 1e-3 here; the
+        # parameters alone, rounded to FP32, move x by 2e-4)
+        fx, fy = ora.integrate('full_heston', (0., .5, 1.), paths=P, steps=n + 1,
+                               dtype=np.float32,
+                               sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (2,),
+                                                               rho=-.7)}, **args)
+        assert _rel(x, rx) < 4 * max(_rel(fx, rx), 1e-5)
+        assert np.max(np.abs(y - ry)) < 4 * max(np.max(np.abs(fy - ry)), 1e-5)
+
+
+def test_heston_multi_axis_per_path(gpu):
+    """full_heston_process with vshape (2,): two (log x, y) pairs under a
+    full correlation, per-path mu and theta of each pair."""
+    P, n = 800, 40
+    g = np.random.default_rng(12)
+    args = dict(x0=1., mu=g.uniform(-.1, .1, (2, P)), sigma=1., y0=.04,
+                theta=g.uniform(.02, .08, (2, P)), k=2., xi=.5)
+    C = _corr(4, 18)
+    p = sa.full_heston_process(paths=P, vshape=2, steps=n + 1, corr=C,
+                               rng=np.random.default_rng(13), **args)
+    x, y = p((0., 1.))
+    rx, ry = ora.integrate('full_heston', (0., 1.), paths=P, vshape=(2,), steps=n + 1,
+                           sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (4,),
+                                                           corr=C)},
+                           **args)
+    assert x.shape == (2, 2, P)
+    assert _rel(x, rx) < 1e-10
+    assert np.max(np.abs(y - ry)) < 1e-12
+
+
+def test_merton_per_path_sigma_and_backward(gpu):
+    """Merton's jump-diffusion with per-path mu and sigma; and i0 = 2 (a
+    backward and a forward pass, each with the per-path parameters)."""
+    P, n = 600, 40
+    g = np.random.default_rng(14)
+    mu, sigma = g.uniform(0., .1, P), g.uniform(.1, .4, P)
+    law = dict(lam=5., a=-.1, b=.15)
+    tt = np.linspace(0., 1., 5)
+    for i0 in (0, 2):
+        info = {}
+        p = sa.merton_jumpdiff_process(paths=P, steps=n + 1, i0=i0, info=info, x0=1., mu=mu,
+                                       sigma=sigma, rng=np.random.default_rng(15), **law)
+        kw, kj = p.sources['dw'].key, p.sources['dj'].key
+        x = p(tt)
+        ref_info = {}
+
+        class W(ora.PhiloxWiener):
+            passes = 0
+
+            def begin_pass(self, tag):
+                super().begin_pass(tag)
+                self.epoch, W.passes = W.passes, W.passes + 1
+
+        class J(ora.PhiloxCPoisson):
+            passes = 0
+
+            def begin_pass(self, tag):
+                super().begin_pass(tag)
+                self.epoch, J.passes = J.passes, J.passes + 1
+
+        ref = ora.integrate('jumpdiff', tt, paths=P, steps=n + 1, i0=i0, info=ref_info,
+                            sources={'dw': W(kw, 0, P, ()), 'dj': J(kj, 0, P, (), **law)},
+                            x0=1., mu=mu, sigma=sigma)
+        assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+        assert _rel(x, ref) < RTOL64, i0
+
+
+def test_per_path_moments_and_refusals(gpu):
+    """prepare(moments=True) with per-path parameters reduces the stored
+    rows in a pass of its own (the fused form takes constant parameters)
+    with the same result; parameters varying across paths and in time, or
+    not broadcastable to vshape + (paths,), are refused before any launch."""
+    P, tt = 4096, np.linspace(0., 1., 9)
+    sigma = np.random.default_rng(16).uniform(.1, .4, P)
+    p = sa.lognorm_process(paths=P, steps=33, x0=1., mu=.05, sigma=sigma,
+                           rng=np.random.default_rng(17))
+    run = p.prepare(tt, moments=True)
+    assert run._row_mom is None
+    x = run.run()
+    mean = np.asarray(run.pmean(dtype=np.float64)).reshape(x.shape[:-1])
+    var = np.asarray(run.pvar(dtype=np.float64)).reshape(x.shape[:-1])
+    assert np.allclose(mean, x.mean(axis=-1), rtol=1e-12, atol=0)
+    assert np.allclose(var, x.var(axis=-1), rtol=1e-10, atol=1e-300)
+    with pytest.raises(NotImplementedError):
+        sa.lognorm_process(paths=P, mu=lambda t: .1 * t, sigma=sigma)(tt)
+    with pytest.raises(ValueError):
+        sa.lognorm_process(paths=P, sigma=np.ones(P + 1))(tt)
