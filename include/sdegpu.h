@@ -59,7 +59,13 @@ extern "C" {
 #define SDEGPU_OUT_FIRST 2      /* out[o] = x[o], o < nout; HESTON: the log-x components
                                    (heston_SDE.let :2533-2537) */
 
+/* Working components one sdegpu_euler call steps: up to SDEGPU_MAX_VARS in any
+   arrangement; SDEGPU_MAX_VARS+1 .. SDEGPU_MAX_COUPLED when they form one
+   coupled block (a correlated dw over the last axis, the 2D components of a
+   Heston state of width D, the factors of a Hull-White value), which sdepy
+   draws with one multivariate normal (infrastructure.py:1524-1553). */
 #define SDEGPU_MAX_VARS 8
+#define SDEGPU_MAX_COUPLED 32
 
 /* ---- compound Poisson jump-size laws (cpoisson_source's y, infrastructure.py:
    2017-2040, :1653-1796), three parameters (p0, p1, p2) per component ---- */
@@ -101,7 +107,12 @@ extern "C" {
 typedef struct sdegpu_euler_problem {
   int32_t model;      /* SDEGPU_MODEL_* */
   int32_t dtype;      /* SDEGPU_F32 or SDEGPU_F64: state, table and output type */
-  int32_t nvars;      /* M = prod(wshape), 1..SDEGPU_MAX_VARS */
+  int32_t nvars;      /* M = prod(wshape) (or the group's share, see comp0):
+                         1..SDEGPU_MAX_VARS, or up to SDEGPU_MAX_COUPLED for one
+                         coupled block: a dw correlated over all M, a Heston state of
+                         width M/2, or a Hull-White value of M factors (out_group = M).
+                         Above SDEGPU_MAX_VARS, row_moments is not taken and jump_defer
+                         is ignored (the jumps are summed in place). */
   int32_t nout;       /* values stored per path and output row = prod(xshape) */
   int32_t out_mode;   /* SDEGPU_OUT_* */
   int32_t out_group;  /* factor count for SDEGPU_OUT_SUM */
@@ -118,7 +129,7 @@ typedef struct sdegpu_euler_problem {
   const int32_t* store_row; /* device [nsteps]: output row reached after step s, or -1 */
   int32_t row0;       /* output row of the initial state, or -1 */
   int32_t stream_tag; /* 0 for the forward pass, 16 for the backward pass (i0 != 0) */
-  double x0[SDEGPU_MAX_VARS]; /* initial working state (log space for log SDEs) */
+  double x0[SDEGPU_MAX_COUPLED]; /* initial working state (log space for log SDEs) */
   const void* x0_paths;       /* optional device [nvars][paths] initial state, overrides x0 */
   void* out;                  /* device [nrows][nout][paths] */
   const void* replay_dw;      /* optional device [nsteps][nvars][paths]: recorded dw increments */
@@ -338,7 +349,8 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  *   call fills the component group comp0 .. comp0+nvars-1 (uncorrelated with
  *   the others, L its own block) of out [nslots][nvars_total][paths], with the
  *   normals of the flat indices s*nvars_total + comp0 + v, as
- *   sdegpu_euler_problem's component groups draw them.
+ *   sdegpu_euler_problem's component groups draw them.  nvars is at most
+ *   SDEGPU_MAX_COUPLED (a correlated block above SDEGPU_MAX_VARS).
  * sdegpu_poisson_fill: poisson_source.__call__ (infrastructure.py:1617-1633).
  *   pois [nslots][nvars][2] = (lam*|ds|, exp(-lam*|ds|)); sign [nslots] = sign(ds);
  *   out int64 [nslots][nvars][paths].

@@ -24,7 +24,8 @@ MODEL_WIENER, MODEL_LOGNORM, MODEL_OU, MODEL_CIR, MODEL_HESTON, \
     MODEL_JUMPDIFF = range(6)
 OUT_IDENTITY, OUT_SUM, OUT_FIRST = range(3)
 JUMP_NORMAL, JUMP_UNIFORM, JUMP_DOUBLE_EXP = range(3)
-MAX_VARS = 8
+MAX_VARS = 8        # components of one k_euler launch, in any arrangement
+MAX_COUPLED = 32    # components of one coupled block (k_euler_wide)
 STREAM_BACKWARD = 16
 REDUCE_MIN, REDUCE_MAX, REDUCE_SUM, REDUCE_MEAN, REDUCE_VAR, REDUCE_STD = range(6)
 
@@ -45,7 +46,7 @@ class EulerProblem(ctypes.Structure):
         ('paths', c_int64), ('nsteps', c_int32), ('ncoef', c_int32),
         ('coef', c_void_p), ('pois', c_void_p), ('store_row', c_void_p),
         ('row0', c_int32), ('stream_tag', c_int32),
-        ('x0', c_double * MAX_VARS), ('x0_paths', c_void_p),
+        ('x0', c_double * MAX_COUPLED), ('x0_paths', c_void_p),
         ('out', c_void_p), ('replay_dw', c_void_p), ('replay_dj', c_void_p),
         ('info_paths', c_void_p), ('info_rows', c_void_p),
         ('key_w', c_uint32 * 2), ('epoch_w', c_uint32),

@@ -1,6 +1,7 @@
 // Entry point of the Euler path kernels (the kernels are in euler_kernel.hpp):
 // validation of the problem, the kernel choice and the launch.
 #include "euler_kernel.hpp"
+#include "euler_wide.hpp"
 
 namespace sdegpu {
 
@@ -60,10 +61,12 @@ using namespace sdegpu;
 extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   if (!pr) return fail(SDEGPU_EINVAL, "sdegpu_euler: null problem");
   const int m = pr->nvars;
-  if (m < 1 || m > SDEGPU_MAX_VARS)
+  if (m < 1 || m > SDEGPU_MAX_COUPLED)
     return fail(SDEGPU_EUNSUPPORTED,
-                "the GPU integrator supports 1..%d working-state components per path, not %d",
-                SDEGPU_MAX_VARS, m);
+                "the GPU integrator steps 1..%d working-state components per launch, not %d",
+                SDEGPU_MAX_COUPLED, m);
+  // above SDEGPU_MAX_VARS: one coupled block, stepped by k_euler_wide
+  const bool wide = m > SDEGPU_MAX_VARS;
   if (pr->dtype != SDEGPU_F32 && pr->dtype != SDEGPU_F64)
     return fail(SDEGPU_EUNSUPPORTED, "unsupported dtype code %d", pr->dtype);
   if (pr->paths < 1 || pr->paths > kMaxPaths)
@@ -114,6 +117,18 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
                 pr->comp0, pr->comp0 + m - 1, m_total);
   if (pr->row_moments && pr->row_moments_rows < 1)
     return fail(SDEGPU_EINVAL, "row_moments without row_moments_rows");
+  if (wide && pr->row_moments)
+    return fail(SDEGPU_EUNSUPPORTED,
+                "fused row moments with more than %d components per launch: reduce the "
+                "output separately", SDEGPU_MAX_VARS);
+  if (wide && pr->out_mode == SDEGPU_OUT_SUM && group != m)
+    return fail(SDEGPU_EUNSUPPORTED,
+                "a launch of %d components stores one factor sum of all of them, not groups "
+                "of %d", m, group);
+  if (wide && heston && hd != m / 2)
+    return fail(SDEGPU_EUNSUPPORTED,
+                "a launch of %d Heston components steps one group of width %d, not %d", m, m / 2,
+                hd);
   if (pr->info_row_slots < 0 || pr->info_row_slots > 1024)
     return fail(SDEGPU_EINVAL, "info_row_slots must be in [0, 1024], not %d", pr->info_row_slots);
   if (pr->out0 < 0 || out_stride < pr->nout || pr->out0 > out_stride - pr->nout)
@@ -149,7 +164,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.row_mom = pr->row_moments;
   a.ppar = pr->param_paths;
   a.mom_rows = (int64_t)pr->row_moments_rows * out_stride;
-  for (int v = 0; v < SDEGPU_MAX_VARS; ++v) a.x0[v] = pr->x0[v];
+  a.nvars = m;
+  for (int v = 0; v < SDEGPU_MAX_COUPLED; ++v) a.x0[v] = pr->x0[v];
   a.paths = pr->paths;
   a.anti_w = pr->anti_w;
   a.anti_j = pr->anti_j;
@@ -179,7 +195,7 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   // no recorded increments and no antithetic halves: the plain instance
   const bool plain =
       !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 && !grouped;
-  const bool defer = jump && pr->jump_defer != 0;
+  const bool defer = jump && pr->jump_defer != 0 && !wide;
   const bool mom = pr->row_moments != nullptr;
   const bool pp = pr->param_paths != nullptr;
   if (pp && mom)
@@ -188,12 +204,14 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   const dim3 grid(blocks_for(pr->paths)), block(kBlock);
   hipStream_t st = (hipStream_t)stream;
   if (pr->dtype == SDEGPU_F32) {
-    EulerKernel<float> k = choose<float>(plain, mom, pp, pr->model, m, hd, defer);
+    EulerKernel<float> k = wide ? euler_wide_kernel<float>(pr->model, m)
+                                : choose<float>(plain, mom, pp, pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const float*)pr->coef, pr->store_row,
                        pr->pois, (float*)pr->out);
   } else {
-    EulerKernel<double> k = choose<double>(plain, mom, pp, pr->model, m, hd, defer);
+    EulerKernel<double> k = wide ? euler_wide_kernel<double>(pr->model, m)
+                                 : choose<double>(plain, mom, pp, pr->model, m, hd, defer);
     if (!k) return fail(SDEGPU_EUNSUPPORTED, "no kernel for model %d with %d components", pr->model, m);
     hipLaunchKernelGGL(k, grid, block, 0, st, a, (const double*)pr->coef, pr->store_row,
                        pr->pois, (double*)pr->out);

@@ -40,7 +40,7 @@ struct EulerArgs {
   const void* replay_dj;
   int32_t* info_paths;
   long long* info_rows;
-  double x0[SDEGPU_MAX_VARS];
+  double x0[SDEGPU_MAX_COUPLED];
   int64_t paths;
   int64_t anti_w, anti_j;  // antithetic half sizes (0: off)
   int32_t nsteps, ncoef, poff, nout, out_group, has_corr, row0;
@@ -57,6 +57,7 @@ struct EulerArgs {
   double* row_mom;     // fused row moments: [gridDim.x][mom_rows][2] or null
   int64_t mom_rows;    // output rows * out_stride
   const void* ppar;    // per-path parameters (PP instances): [PathParams<MODEL>::NB][..][paths]
+  int32_t nvars;       // components this launch steps (the run-time width of k_euler_wide)
 };
 
 // Per-path parameters (the PP instances of k_euler): the model's parameter
@@ -203,6 +204,29 @@ __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs
       if constexpr (OutRule<MODEL>::exp_all) val = exp(val);
       put(val, o);
     }
+  }
+}
+
+// Adds the jumps of the interval that ends at output row `row` to the row's
+// total: one atomic per wave, into one of the row's info_slots counters
+// (every wave reaches a row at about the same time; on one address their
+// adds queue behind each other).  The step and its row are the same for the
+// whole wave, so its lanes are all here; the lanes with jumps are summed by a
+// scalar walk of their ballot.
+__device__ __forceinline__ void add_interval_jumps(const EulerArgs& a, int row, bool live,
+                                                   int jumps) {
+  uint64_t todo = __ballot(live && jumps != 0);
+  if (todo) {
+    long long sum = 0;
+    while (todo) {
+      const int l = __ffsll((unsigned long long)todo) - 1;
+      sum += __builtin_amdgcn_readlane(jumps, l);
+      todo &= todo - 1;
+    }
+    if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
+      atomicAdd((unsigned long long*)(a.info_rows + (int64_t)row * a.info_slots +
+                                      (int)(blockIdx.x % (unsigned)a.info_slots)),
+                (unsigned long long)sum);
   }
 }
 
@@ -618,27 +642,7 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
         if constexpr (kDefer) flush_jumps();
         store(row);
         if constexpr (kJump) {
-          // the interval's jump total: one atomic per wave, into one of the
-          // row's info_slots counters (every wave reaches a row at about the
-          // same time; on one address their adds queue behind each other).
-          // The step and its row are the same for the whole wave, so its live
-          // lanes are all here; the lanes with jumps are summed by a scalar
-          // walk of their ballot.
-          if (a.info_rows) {
-            uint64_t todo = __ballot(live && interval_jumps != 0);
-            if (todo) {
-              long long sum = 0;
-              while (todo) {
-                const int l = __ffsll((unsigned long long)todo) - 1;
-                sum += __builtin_amdgcn_readlane(interval_jumps, l);
-                todo &= todo - 1;
-              }
-              if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
-                atomicAdd((unsigned long long*)(a.info_rows + (int64_t)row * a.info_slots +
-                                                (int)(blockIdx.x % (unsigned)a.info_slots)),
-                          (unsigned long long)sum);
-            }
-          }
+          if (a.info_rows) add_interval_jumps(a, row, live, interval_jumps);
           interval_jumps = 0;
         }
       }

@@ -1,0 +1,8 @@
+// k_euler_wide and k_wiener_fill_wide instances in FP64 (euler_wide.hpp).
+#include "euler_wide.hpp"
+
+namespace sdegpu {
+
+SDEGPU_EULER_WIDE_SET(double)
+
+}  // namespace sdegpu

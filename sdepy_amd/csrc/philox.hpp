@@ -223,6 +223,87 @@ __device__ __forceinline__ void words_at(uint64_t g, uint32_t path, uint32_t c2,
   }
 }
 
+// The normals with flat indices g .. g+cnt-1 into z[0 .. cnt-1] for a count
+// cnt <= W known only at run time (the same for every lane): the Philox
+// blocks covering them are drawn, those past the span skipped, and the values
+// picked by the offset g % 4.  Entries from cnt on hold other draws or zeros.
+template <typename T, int W>
+__device__ __forceinline__ void normals_span(uint64_t g, int cnt, uint32_t path, uint32_t c2,
+                                             uint32_t epoch, uint32_t k0, uint32_t k1, T (&z)[W]) {
+  constexpr int NB = (W + 6) / 4;
+  T zz[4 * NB];
+  const uint32_t b0 = (uint32_t)(g >> 2);
+  const int off = (int)(g & 3u);
+  const int nb = (off + cnt + 3) >> 2;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    if (q < nb) {
+      normals4<T>(b0 + q, path, c2, epoch, k0, k1, zz + 4 * q);
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) zz[4 * q + l] = (T)0;
+    }
+  }
+  switch (off) {
+    case 0:
+#pragma unroll
+      for (int j = 0; j < W; ++j) z[j] = zz[j];
+      break;
+    case 1:
+#pragma unroll
+      for (int j = 0; j < W; ++j) z[j] = zz[j + 1];
+      break;
+    case 2:
+#pragma unroll
+      for (int j = 0; j < W; ++j) z[j] = zz[j + 2];
+      break;
+    default:
+#pragma unroll
+      for (int j = 0; j < W; ++j) z[j] = zz[j + 3];
+      break;
+  }
+}
+
+// The raw words with flat indices g .. g+cnt-1 (Poisson uniforms), as
+// normals_span.
+template <int W>
+__device__ __forceinline__ void words_span(uint64_t g, int cnt, uint32_t path, uint32_t c2,
+                                           uint32_t epoch, uint32_t k0, uint32_t k1,
+                                           uint32_t (&w)[W]) {
+  constexpr int NB = (W + 6) / 4;
+  uint32_t ww[4 * NB];
+  const uint32_t b0 = (uint32_t)(g >> 2);
+  const int off = (int)(g & 3u);
+  const int nb = (off + cnt + 3) >> 2;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    ctr4 r = {0u, 0u, 0u, 0u};
+    if (q < nb) r = philox4x32_10(ctr4{b0 + q, path, c2, epoch}, k0, k1);
+    ww[4 * q] = r.x;
+    ww[4 * q + 1] = r.y;
+    ww[4 * q + 2] = r.z;
+    ww[4 * q + 3] = r.w;
+  }
+  switch (off) {
+    case 0:
+#pragma unroll
+      for (int j = 0; j < W; ++j) w[j] = ww[j];
+      break;
+    case 1:
+#pragma unroll
+      for (int j = 0; j < W; ++j) w[j] = ww[j + 1];
+      break;
+    case 2:
+#pragma unroll
+      for (int j = 0; j < W; ++j) w[j] = ww[j + 2];
+      break;
+    default:
+#pragma unroll
+      for (int j = 0; j < W; ++j) w[j] = ww[j + 3];
+      break;
+  }
+}
+
 // Poisson variate by sequential inversion of the CDF, in double precision on
 // every target so that counts are bit-identical to the host twin.
 // lam = intensity * |ds|, p0 = exp(-lam) (precomputed on the host in double).

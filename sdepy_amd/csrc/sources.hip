@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include "common.hpp"
+#include "euler_wide.hpp"
 #include "jumps.hpp"
 #include "philox.hpp"
 
@@ -192,9 +193,9 @@ extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t
                                   uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
                                   const void* coef, int32_t ncoef, int32_t has_corr, int32_t comp0,
                                   int32_t nvars_total, void* out, void* stream) {
-  if (!key || nvars < 1 || nvars > SDEGPU_MAX_VARS)
+  if (!key || nvars < 1 || nvars > SDEGPU_MAX_COUPLED)
     return fail(SDEGPU_EUNSUPPORTED, "wiener_fill: %d components per path (1..%d supported)", nvars,
-                SDEGPU_MAX_VARS);
+                SDEGPU_MAX_COUPLED);
   const int m_total = nvars_total != 0 ? nvars_total : nvars;
   if (comp0 < 0 || m_total < nvars || comp0 > m_total - nvars)
     return fail(SDEGPU_EINVAL, "wiener_fill: components %d..%d outside %d", comp0,
@@ -210,6 +211,20 @@ extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t
   const uint32_t c2 = ((uint32_t)stream_tag & 0xF0u) | kStreamNormal;
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(blocks_for(paths)), block(kBlock);
+  if (nvars > SDEGPU_MAX_VARS) {
+    // one correlated block of more than SDEGPU_MAX_VARS components
+    if (dtype == SDEGPU_F32)
+      hipLaunchKernelGGL(wiener_fill_wide_kernel<float>(nvars), grid, block, 0, st, key[0], key[1],
+                         epoch, c2, paths, nslots, (const float*)coef, ncoef, has_corr, nvars,
+                         comp0, m_total, (float*)out);
+    else if (dtype == SDEGPU_F64)
+      hipLaunchKernelGGL(wiener_fill_wide_kernel<double>(nvars), grid, block, 0, st, key[0],
+                         key[1], epoch, c2, paths, nslots, (const double*)coef, ncoef, has_corr,
+                         nvars, comp0, m_total, (double*)out);
+    else
+      return fail(SDEGPU_EUNSUPPORTED, "wiener_fill: dtype code %d", dtype);
+    return launch_status("k_wiener_fill_wide");
+  }
   if (dtype == SDEGPU_F32)
     hipLaunchKernelGGL(pick_wiener<float>(nvars), grid, block, 0, st, key[0], key[1], epoch, c2,
                        paths, nslots, (const float*)coef, ncoef, has_corr, comp0, m_total,

@@ -216,10 +216,12 @@ class wiener_source(_philox_source):
         if M <= _lib.MAX_VARS:
             return [(0, M)]
         unit = self.vshape[-1] if self.corr is not None else 1
-        if unit > _lib.MAX_VARS:
+        if unit > _lib.MAX_COUPLED:
             raise NotImplementedError(
                 'the GPU sampler correlates at most {} components, not {}'
-                .format(_lib.MAX_VARS, unit))
+                .format(_lib.MAX_COUPLED, unit))
+        if unit > _lib.MAX_VARS:
+            return [(g0, unit) for g0 in range(0, M, unit)]
         mg = (_lib.MAX_VARS // unit) * unit
         return [(g0, min(mg, M - g0)) for g0 in range(0, M, mg)]
 

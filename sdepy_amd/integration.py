@@ -562,18 +562,22 @@ class _gpu_sde:
         working state when it has at most MAX_VARS components, else groups
         of whole coupled blocks.  Components are coupled along the last axis
         of wshape by a correlated dw, Heston's (log x, y) pairs and
-        Hull-White's summed factors; otherwise each is on its own."""
+        Hull-White's summed factors; otherwise each is on its own.  A
+        coupled block of more than MAX_VARS (up to MAX_COUPLED) components
+        is a launch of its own (the wide kernel)."""
         M = self._nvars()
         if M <= _lib.MAX_VARS:
             return [(0, M)]
         coupled = has_corr or self.model == _lib.MODEL_HESTON or \
             out_mode == _lib.OUT_SUM
         unit = self.wshape[-1] if coupled else 1
-        if unit > _lib.MAX_VARS:
+        if unit > _lib.MAX_COUPLED:
             raise NotImplementedError(
                 'the GPU integrator steps at most {} coupled working-state '
                 'components per path; {} with vshape {} couples {}'.format(
-                    _lib.MAX_VARS, type(self).__name__, self.vshape, unit))
+                    _lib.MAX_COUPLED, type(self).__name__, self.vshape, unit))
+        if unit > _lib.MAX_VARS:
+            return [(g0, unit) for g0 in range(0, M, unit)]
         mg = (_lib.MAX_VARS // unit) * unit
         return [(g0, min(mg, M - g0)) for g0 in range(0, M, mg)]
 
@@ -645,8 +649,11 @@ class PreparedRun:
         """Have every pass reduce its stored rows as it stores them: block
         partials [ceil(paths/256)][rows][2], merged after each launch into
         the FP64 (mean, m2) of every row."""
-        if any(prob.param_paths for p in self.passes for prob in p.probs):
-            return      # per-path parameters: the separate reduction serves
+        if any(prob.param_paths or prob.nvars > _lib.MAX_VARS
+               for p in self.passes for prob in p.probs):
+            # per-path parameters or a wide coupled block: the separate
+            # reduction serves
+            return
         n_t, nout, P = self.xx_dev.shape
         rows = n_t * nout
         nblocks = (P + 255) // 256

@@ -132,7 +132,7 @@ int main(void) {
   memset(&bad, 0, sizeof bad);
   bad.model = SDEGPU_MODEL_LOGNORM;
   bad.dtype = SDEGPU_F64;
-  bad.nvars = 9;
+  bad.nvars = SDEGPU_MAX_COUPLED + 1;
   rc = sdegpu_euler(&bad, NULL);
   char msg[512];
   sdegpu_last_error(msg, sizeof msg);

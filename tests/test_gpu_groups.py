@@ -33,19 +33,24 @@ def _corr(k, seed):
 
 
 def test_groups_split_as_documented(gpu):
-    """Group sizes: whole blocks of the coupled axis, up to 8 components."""
+    """Group sizes: whole blocks of the coupled axis, up to 8 components;
+    a coupled block of 9..32 on its own."""
     p = sa.ornstein_uhlenbeck_process(paths=4, vshape=(5, 3), corr=_corr(3, 1))
     assert p._groups(True, 0, 1) == [(0, 6), (6, 6), (12, 3)]
     q = sa.lognorm_process(paths=4, vshape=(11,))
     assert q._groups(False, 0, 1) == [(0, 8), (8, 3)]
     h = sa.full_heston_process(paths=4, vshape=(3, 2))
     assert h._groups(False, 0, 1) == [(0, 8), (8, 4)]
+    # coupled blocks above 8 components: one launch each (the wide kernel)
+    w = sa.ornstein_uhlenbeck_process(paths=4, vshape=(2, 9), corr=_corr(9, 2))
+    assert w._groups(True, 0, 1) == [(0, 9), (9, 9)]
+    # above 32 coupled components: refused before any launch
     with pytest.raises(NotImplementedError):
-        sa.ornstein_uhlenbeck_process(paths=4, vshape=9, corr=_corr(9, 2))((0., 1.))
+        sa.ornstein_uhlenbeck_process(paths=4, vshape=33, corr=_corr(33, 2))((0., 1.))
     with pytest.raises(NotImplementedError):
-        sa.full_heston_process(paths=4, vshape=5)((0., 1.))
+        sa.full_heston_process(paths=4, vshape=17)((0., 1.))
     with pytest.raises(NotImplementedError):
-        sa.hull_white_process(paths=4, factors=9)((0., 1.))
+        sa.hull_white_process(paths=4, factors=33)((0., 1.))
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
