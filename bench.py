@@ -299,10 +299,12 @@ def _epoch_spread(G, fd_ms, n_epochs=8):
     names = ('price',) + tuple(G.HESTON_GREEKS)
     kw = dict(paths=PATHS, steps=NSTEPS, strike=STRIKE, maturity=MATURITY,
               key=(0x5eed, 0xc0ffee), **HPRM)
-    rows = {'fp32_adjoint': [], 'fp64_adjoint': [], 'fp64_fd': []}
-    diff_se64 = []
+    rows = {'fp32_adjoint': [], 'fp64_adjoint': [], 'fp64_fd': [],
+            'fp32_adjoint_fp64_variance': []}
+    diff_se64, diff_v64 = [], []
     for e in range(n_epochs):
         r32 = G.heston_greeks(dtype=np.float32, epoch=e, **kw)
+        rv = G.heston_greeks(dtype=np.float32, variance_dtype=np.float64, epoch=e, **kw)
         r64 = G.heston_greeks(dtype=np.float64, epoch=e, **kw)
         fd = G.HestonFD(dtype=np.float64, epoch=e, h=1e-4, **kw)
         fd.launch()
@@ -311,8 +313,12 @@ def _epoch_spread(G, fd_ms, n_epochs=8):
         rows['fp32_adjoint'].append([r32.price] + [r32.greeks[k] for k in G.HESTON_GREEKS])
         rows['fp64_adjoint'].append([r64.price] + [r64.greeks[k] for k in G.HESTON_GREEKS])
         rows['fp64_fd'].append([p64] + [fdg[k] for k in G.HESTON_GREEKS])
+        rows['fp32_adjoint_fp64_variance'].append(
+            [rv.price] + [rv.greeks[k] for k in G.HESTON_GREEKS])
         diff_se64.append([(r32.greeks[k] - r64.greeks[k]) / r64.greeks_se[k]
                           for k in G.HESTON_GREEKS])
+        diff_v64.append([(rv.greeks[k] - r64.greeks[k]) / r64.greeks_se[k]
+                         for k in G.HESTON_GREEKS])
     res = {'epochs': n_epochs, 'paths_per_epoch': PATHS}
     stats = {}
     for est, v in rows.items():
@@ -330,6 +336,14 @@ def _epoch_spread(G, fd_ms, n_epochs=8):
     res['fp32_minus_fp64_adjoint'] = {
         'max_abs_over_se64_per_epoch': dict(zip(G.HESTON_GREEKS, np.abs(d).max(axis=0))),
         'mean_over_spread_se': dict(zip(G.HESTON_GREEKS, (v32 - v64).mean(axis=0) /
+                                        stats['fp64_adjoint'][1][1:]))}
+    # the FP32 kernel with an FP64 variance state (FP32 normals), on the same
+    # paths: how much of the FP32-FP64 difference the variance's rounding makes
+    dv = np.asarray(diff_v64)
+    vv = np.asarray(rows['fp32_adjoint_fp64_variance'])[:, 1:]
+    res['fp32_fp64_variance_minus_fp64_adjoint'] = {
+        'max_abs_over_se64_per_epoch': dict(zip(G.HESTON_GREEKS, np.abs(dv).max(axis=0))),
+        'mean_over_spread_se': dict(zip(G.HESTON_GREEKS, (vv - v64).mean(axis=0) /
                                         stats['fp64_adjoint'][1][1:]))}
     return res
 
@@ -484,6 +498,19 @@ def main():
                 'frac_of_nominal': TANGENT_VALU_WAVE_INSTS / (t_ms * 1e-3)
                 / VALU_ISSUE_PEAK_PER_S}}
         del trun
+        # the FP64-variance option of both kernels: its cost
+        fv = {}
+        for m in ('adjoint', 'tangent'):
+            vrun = G.HestonGreeks(paths=PATHS, steps=NSTEPS, strike=STRIKE,
+                                  maturity=MATURITY, dtype=np.float32,
+                                  variance_dtype=np.float64, key=(0x5eed, 0xc0ffee),
+                                  epoch=0, method=m, **HPRM)
+            v_ms, _ = timed(torch, vrun.launch, args.steps, args.warmup)
+            fv[m + '_ms'] = v_ms
+            del vrun
+        fv['adjoint_cost_over_fp32'] = fv['adjoint_ms'] / ms
+        fv['tangent_cost_over_fp32'] = fv['tangent_ms'] / t_ms
+        out['fp64_variance'] = fv
 
     # ---- FP64 forward + CRN finite differences, same paths ----
     if 'fd' in legs:

@@ -408,6 +408,12 @@ typedef struct sdegpu_heston_greeks {
                       recomputed segment by segment); 1: forward mode, the
                       derivatives carried along the one forward pass (no
                       checkpoints); the same estimator either way */
+  int32_t variance_f64; /* SDEGPU_F32 only, 1: the variance, its checkpoints and its
+                           adjoint (forward mode: every derivative) in FP64, the log
+                           price, the Philox normals and Box-Muller in FP32; where the
+                           Feller condition fails the pathwise variance Greeks go
+                           through 1/sqrt(v) near v = 0, and FP32 rounding of v moves
+                           single paths' values by more than their standard error */
 } sdegpu_heston_greeks;
 
 size_t sdegpu_heston_greeks_workspace(const sdegpu_heston_greeks* g);

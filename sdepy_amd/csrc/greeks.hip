@@ -92,28 +92,28 @@ __device__ __forceinline__ double trcp<double>(double v) {
 // (integration.py:2425-2433) with the increments summed before the add.
 constexpr int kHestonTab = 12;
 
-template <typename T>
+template <typename T, typename TO = T>
 __global__ __launch_bounds__(kBlock) void k_heston_table(const T* __restrict__ steps, int n,
                                                          double r, double sig, double k, double th,
-                                                         double xi, T* __restrict__ tab,
+                                                         double xi, TO* __restrict__ tab,
                                                          double* __restrict__ sum_ds) {
   __shared__ double sh[kBlock];
   double part = 0.0;
   for (int s = threadIdx.x; s < n; s += kBlock) {
     const double ds = (double)steps[2 * s], sq = (double)steps[2 * s + 1];
-    T* row = tab + (int64_t)s * kHestonTab;
-    row[0] = (T)ds;
-    row[1] = (T)sq;
-    row[2] = (T)(r * ds);
-    row[3] = (T)(0.5 * sig * sig * ds);
-    row[4] = (T)(k * ds);
-    row[5] = (T)(k * th * ds);
-    row[6] = (T)(sig * sq);
-    row[7] = (T)(xi * sq);
-    row[8] = (T)(0.5 * xi * sq);
-    row[9] = (T)(0.5 * sig * sq);
-    row[10] = (T)(1.0 - k * ds);
-    row[11] = (T)0;
+    TO* row = tab + (int64_t)s * kHestonTab;
+    row[0] = (TO)ds;
+    row[1] = (TO)sq;
+    row[2] = (TO)(r * ds);
+    row[3] = (TO)(0.5 * sig * sig * ds);
+    row[4] = (TO)(k * ds);
+    row[5] = (TO)(k * th * ds);
+    row[6] = (TO)(sig * sq);
+    row[7] = (TO)(xi * sq);
+    row[8] = (TO)(0.5 * xi * sq);
+    row[9] = (TO)(0.5 * sig * sq);
+    row[10] = (TO)(1.0 - k * ds);
+    row[11] = (TO)0;
     part += ds;
   }
   sh[threadIdx.x] = part;
@@ -152,17 +152,28 @@ struct HestonRev {
   T xbar, vbar, s1, s2, sx, sr;
 };
 
+// Waves per SIMD the Heston kernels are built for: FP32 8, FP32 with an FP64
+// variance state 4 (its variance registers take twice the room), FP64 1.
+template <typename T, typename TV>
+constexpr int heston_waves() {
+  return sizeof(TV) == 8 ? (sizeof(T) == 4 ? 4 : 1) : 8;
+}
+
 // One checkpoint segment of the reverse sweep: recompute the segment's
 // variances and normals from its checkpoint and the Philox counters into
 // registers, then walk its steps backwards.  FULL segments hold C steps.
-template <typename T, int C, bool FULL>
-__device__ __forceinline__ void heston_segment(const HestonArgs& a, const T* __restrict__ tb,
-                                               const T* __restrict__ ck, int64_t gid,
-                                               uint32_t path, int c, int n, T rho, T rhoc,
-                                               HestonRev<T>& rv) {
+// The normals are drawn in T; the variance, its adjoint and the parameter
+// sums are carried in TV (T, or FP64 with FP32 normals), with the step
+// constants of the table tv of that type.
+template <typename T, typename TV, int C, bool FULL>
+__device__ __forceinline__ void heston_segment(const HestonArgs& a, const TV* __restrict__ tv,
+                                               const TV* __restrict__ ck, int64_t gid,
+                                               uint32_t path, int c, int n, TV rho, TV rhoc,
+                                               HestonRev<TV>& rv) {
   const int s0 = c * C;
-  T svs[C], z0s[C], ws[C];
-  T v = ck[(int64_t)c * a.paths + gid];
+  TV svs[C], ws[C];
+  T z0s[C];
+  TV v = ck[(int64_t)c * a.paths + gid];
 #pragma unroll
   for (int j = 0; j < C; j += 2) {
     if (FULL || s0 + j < n) {
@@ -173,11 +184,11 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const T* __r
       for (int q = 0; q < 2; ++q) {
         const int s = s0 + j + q;
         z0s[j + q] = z[2 * q];
-        ws[j + q] = rho * z[2 * q] + rhoc * z[2 * q + 1];
+        ws[j + q] = rho * (TV)z[2 * q] + rhoc * (TV)z[2 * q + 1];
         if (FULL || s < n) {
-          const T* r = tb + (int64_t)s * kHestonTab;
-          const T vp = pos_part(v);
-          const T sv = tsqrt<T>(vp);
+          const TV* r = tv + (int64_t)s * kHestonTab;
+          const TV vp = pos_part(v);
+          const TV sv = tsqrt<TV>(vp);
           svs[j + q] = sv;
           v = v + fma(sv * r[7], ws[j + q], fma(-r[4], vp, r[5]));
         }
@@ -191,29 +202,30 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const T* __r
       // keep the scalar table loads next to their step: hoisting all of a
       // segment's rows at once overflows the SGPR file into VGPR lanes
       __asm__ volatile("" ::: "memory");
-      const T* r = tb + (int64_t)s * kHestonTab;
-      const T sv = svs[j], z0 = z0s[j], w = ws[j];
-      const T vb = rv.vbar;
-      const T u = vb * r[0];
+      const TV* r = tv + (int64_t)s * kHestonTab;
+      const TV sv = svs[j], z0 = (TV)z0s[j], w = ws[j];
+      const TV vb = rv.vbar;
+      const TV u = vb * r[0];
       rv.s1 = rv.s1 + u;
       rv.s2 = fma(u, sv * sv, rv.s2);
-      const T b = vb * (sv * r[1]);
+      const TV b = vb * (sv * r[1]);
       rv.sx = fma(b, w, rv.sx);
       rv.sr = fma(b, fma(-rho, w, z0), rv.sr);
-      const T h = trcp<T>(sv);
-      const T c1 = fma(w, h * r[8], -r[4]);
-      const T c2 = fma(z0, h * r[9], -r[3]);
-      const T d = fma(vb, c1, rv.xbar * c2);
-      rv.vbar = sv > (T)0 ? vb + d : vb;
+      const TV h = trcp<TV>(sv);
+      const TV c1 = fma(w, h * r[8], -r[4]);
+      const TV c2 = fma(z0, h * r[9], -r[3]);
+      const TV d = fma(vb, c1, rv.xbar * c2);
+      rv.vbar = sv > (TV)0 ? vb + d : vb;
     }
   }
 }
 
-template <typename T, int C>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoint(const HestonArgs a,
-                                                           const T* __restrict__ tb,
-                                                           T* __restrict__ ck,
-                                                           const double* __restrict__ sum_ds) {
+// tb: the step constants in T (the log-price update); tv: the same in TV
+// (the variance, its adjoint and the parameter sums; tv == tb when TV == T).
+template <typename T, typename TV, int C>
+__global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjoint(
+    const HestonArgs a, const T* __restrict__ tb, const TV* __restrict__ tv,
+    TV* __restrict__ ck, const double* __restrict__ sum_ds) {
   static_assert(C % 2 == 0, "one Philox block feeds two steps");
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool live = gid < a.paths;
@@ -221,6 +233,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoi
   const int64_t P = a.paths;
   const int n = a.nsteps;
   const T rho = (T)a.rho, rhoc = (T)a.rhoc;
+  const TV rho_v = (TV)a.rho, rhoc_v = (TV)a.rhoc;
 
   double acc[kHestonNS];
 #pragma unroll
@@ -228,7 +241,8 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoi
 
   if (live) {
     // ---- pass 1: forward, variance checkpoints every C steps ----
-    T X = (T)a.lx0, V = (T)a.y0;
+    T X = (T)a.lx0;
+    TV V = (TV)a.y0;
     for (int s0 = 0; s0 < n; s0 += 2) {
       T z[4];
       normals4<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
@@ -238,12 +252,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoi
         if (s < n) {
           if (s % C == 0) ck[(int64_t)(s / C) * P + gid] = V;
           const T* r = tb + (int64_t)s * kHestonTab;
+          const TV* rv = tv + (int64_t)s * kHestonTab;
           const T z0 = z[2 * j];
-          const T w = rho * z0 + rhoc * z[2 * j + 1];
-          const T vp = pos_part(V);
-          const T sv = tsqrt<T>(vp);
-          X = X + fma(sv * r[6], z0, fma(-r[3], vp, r[2]));
-          V = V + fma(sv * r[7], w, fma(-r[4], vp, r[5]));
+          const TV w = rho_v * (TV)z0 + rhoc_v * (TV)z[2 * j + 1];
+          const TV vp = pos_part(V);
+          const TV sv = tsqrt<TV>(vp);
+          X = X + fma((T)sv * r[6], z0, fma(-r[3], (T)vp, r[2]));
+          V = V + fma(sv * rv[7], w, fma(-rv[4], vp, rv[5]));
         }
       }
     }
@@ -256,17 +271,17 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoi
     const double pay = itm ? a.disc * intr : 0.0;
     // dP/dX_T = disc * (+-S) on the money
     const double xbar = itm ? (call ? a.disc * S : -a.disc * S) : 0.0;
-    const T Xbar = (T)xbar;
+    const TV Xbar = (TV)xbar;
 
     // ---- reverse sweep over the checkpoint segments ----
     // Full segments run without step guards; only the last one can be short.
-    HestonRev<T> rv{Xbar, (T)0, (T)0, (T)0, (T)0, (T)0};
+    HestonRev<TV> rv{Xbar, (TV)0, (TV)0, (TV)0, (TV)0, (TV)0};
     const int nfull = n / C;
     if (nfull * C < n)
-      heston_segment<T, C, false>(a, tb, ck, gid, path, nfull, n, rho, rhoc, rv);
+      heston_segment<T, TV, C, false>(a, tv, ck, gid, path, nfull, n, rho_v, rhoc_v, rv);
     for (int c = nfull - 1; c >= 0; --c)
-      heston_segment<T, C, true>(a, tb, ck, gid, path, c, n, rho, rhoc, rv);
-    const T Vbar = rv.vbar, S1 = rv.s1, S2 = rv.s2, Sx = rv.sx, Sr = rv.sr;
+      heston_segment<T, TV, C, true>(a, tv, ck, gid, path, c, n, rho_v, rhoc_v, rv);
+    const TV Vbar = rv.vbar, S1 = rv.s1, S2 = rv.s2, Sx = rv.sx, Sr = rv.sr;
 
     const double s1 = (double)S1;
     const double g[kHestonNG] = {xbar / a.x0,
@@ -302,23 +317,26 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_adjoi
 //   v: none, dV_0 = 1   v0
 // and each Greek xbar dX_T along its direction: the same numbers as the
 // adjoint's sums (S1 = dX_1, S2 = dX_2, ...), in another order of operations.
-template <typename T>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_tangent(
-    const HestonArgs a, const T* __restrict__ tb, const double* __restrict__ sum_ds) {
+template <typename T, typename TV>
+__global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tangent(
+    const HestonArgs a, const T* __restrict__ tb, const TV* __restrict__ tv,
+    const double* __restrict__ sum_ds) {
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool live = gid < a.paths;
   const uint32_t path = (uint32_t)gid;
   const int n = a.nsteps;
-  const T rho = (T)a.rho, rhoc = (T)a.rhoc;
+  const TV rho = (TV)a.rho, rhoc = (TV)a.rhoc;
 
   double acc[kHestonNS];
 #pragma unroll
   for (int i = 0; i < kHestonNS; ++i) acc[i] = 0.0;
 
   if (live) {
-    T X = (T)a.lx0, V = (T)a.y0;
-    T x1 = (T)0, v1 = (T)0, x2 = (T)0, v2 = (T)0, xx = (T)0, vx = (T)0;
-    T xr = (T)0, vr = (T)0, xv = (T)0, vv = (T)1;
+    // the log price in T, the variance and every derivative in TV
+    T X = (T)a.lx0;
+    TV V = (TV)a.y0;
+    TV x1 = (TV)0, v1 = (TV)0, x2 = (TV)0, v2 = (TV)0, xx = (TV)0, vx = (TV)0;
+    TV xr = (TV)0, vr = (TV)0, xv = (TV)0, vv = (TV)1;
     for (int s0 = 0; s0 < n; s0 += 2) {
       T z[4];
       normals4<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
@@ -326,19 +344,20 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_tange
       for (int j = 0; j < 2; ++j) {
         const int s = s0 + j;
         if (s < n) {
-          const T* r = tb + (int64_t)s * kHestonTab;
-          const T z0 = z[2 * j];
-          const T w = rho * z0 + rhoc * z[2 * j + 1];
-          const T vp = pos_part(V);
-          const T sv = tsqrt<T>(vp);
-          const T h = trcp<T>(sv);
-          const bool pos = sv > (T)0;
-          const T av = pos ? fma(w * r[8], h, r[10]) : (T)1;
-          const T bx = pos ? fma(z0 * r[9], h, -r[3]) : (T)0;
-          const T ssq = sv * r[1];
-          const T c2 = vp * r[0];
-          const T cx = ssq * w;
-          const T cr = ssq * fma(-rho, w, z0);
+          const T* rx = tb + (int64_t)s * kHestonTab;
+          const TV* r = tv + (int64_t)s * kHestonTab;
+          const TV z0 = (TV)z[2 * j];
+          const TV w = rho * z0 + rhoc * (TV)z[2 * j + 1];
+          const TV vp = pos_part(V);
+          const TV sv = tsqrt<TV>(vp);
+          const TV h = trcp<TV>(sv);
+          const bool pos = sv > (TV)0;
+          const TV av = pos ? fma(w * r[8], h, r[10]) : (TV)1;
+          const TV bx = pos ? fma(z0 * r[9], h, -r[3]) : (TV)0;
+          const TV ssq = sv * r[1];
+          const TV c2 = vp * r[0];
+          const TV cx = ssq * w;
+          const TV cr = ssq * fma(-rho, w, z0);
           x1 = fma(v1, bx, x1);
           v1 = fma(v1, av, r[0]);
           x2 = fma(v2, bx, x2);
@@ -349,7 +368,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 1) void k_heston_tange
           vr = fma(vr, av, cr);
           xv = fma(vv, bx, xv);
           vv = vv * av;
-          X = X + fma(sv * r[6], z0, fma(-r[3], vp, r[2]));
+          X = X + fma((T)sv * rx[6], z[2 * j], fma(-rx[3], (T)vp, rx[2]));
           V = V + fma(sv * r[7], w, fma(-r[4], vp, r[5]));
         }
       }
@@ -529,22 +548,27 @@ __global__ __launch_bounds__(kBlock) void k_sum_columns(const double* __restrict
 }
 
 // Workspace of one Heston adjoint launch, in this order (each part 256-byte
-// aligned): the per-step table [n][kHestonTab], the FP64 sum of ds, the
-// variance checkpoints [nseg][paths] and the per-block partial sums.
+// aligned): the per-step table [n][kHestonTab] of the dtype, with an FP64
+// variance state the same table in FP64, the FP64 sum of ds, the variance
+// checkpoints [nseg][paths] (of the variance's type) and the per-block
+// partial sums.
 struct HestonWs {
-  size_t tab, sum, ckpt, part, total;
+  size_t tab, tabv, sum, ckpt, part, total;
 };
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-static HestonWs heston_ws(int dtype, int64_t paths, int nsteps) {
+static HestonWs heston_ws(int dtype, bool v64, int64_t paths, int nsteps) {
   const size_t es = dtype == SDEGPU_F32 ? 4 : 8;
+  const size_t ev = v64 ? 8 : es;
   const int64_t nseg = (nsteps + kHestonSeg - 1) / kHestonSeg;
+  const size_t rows = (size_t)(nsteps > 0 ? nsteps : 1) * kHestonTab;
   HestonWs w;
   w.tab = 0;
-  w.sum = align256((size_t)(nsteps > 0 ? nsteps : 1) * kHestonTab * es);
+  w.tabv = align256(rows * es);
+  w.sum = w.tabv + (ev != es ? align256(rows * ev) : 0);
   w.ckpt = w.sum + 256;
-  w.part = w.ckpt + align256((size_t)(nseg > 0 ? nseg : 1) * (size_t)paths * es);
+  w.part = w.ckpt + align256((size_t)(nseg > 0 ? nseg : 1) * (size_t)paths * ev);
   w.total = w.part + (size_t)blocks_for(paths) * kHestonNS * sizeof(double);
   return w;
 }
@@ -555,7 +579,7 @@ using namespace sdegpu;
 
 extern "C" size_t sdegpu_heston_greeks_workspace(const sdegpu_heston_greeks* g) {
   if (!g || g->paths < 1) return 0;
-  return heston_ws(g->dtype, g->paths, g->nsteps).total;
+  return heston_ws(g->dtype, g->variance_f64 != 0, g->paths, g->nsteps).total;
 }
 
 extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* out, void* ws,
@@ -573,12 +597,16 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
     return fail(SDEGPU_EINVAL, "heston_greeks: payoff kind %d", g->payoff);
   if (g->method != 0 && g->method != 1)
     return fail(SDEGPU_EINVAL, "heston_greeks: method %d", g->method);
+  if (g->variance_f64 != 0 && g->variance_f64 != 1)
+    return fail(SDEGPU_EINVAL, "heston_greeks: variance_f64 %d", g->variance_f64);
+  // an FP64 variance state with FP32 normals (FP64 runs carry it anyway)
+  const bool v64 = g->variance_f64 != 0 && g->dtype == SDEGPU_F32;
   const size_t need = sdegpu_heston_greeks_workspace(g);
   if (!ws || ws_bytes < need)
     return fail(SDEGPU_ENOMEM, "heston_greeks: workspace of %zu bytes, %zu needed", ws_bytes, need);
   HestonArgs a = {};
   a.steps = g->steps;
-  const HestonWs lay = heston_ws(g->dtype, g->paths, g->nsteps);
+  const HestonWs lay = heston_ws(g->dtype, v64, g->paths, g->nsteps);
   char* base = (char*)ws;
   a.ckpt = base + lay.ckpt;
   a.part = (double*)(base + lay.part);
@@ -609,12 +637,28 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
     hipLaunchKernelGGL(k_heston_table<float>, dim3(1), dim3(kBlock), 0, st, (const float*)g->steps,
                        g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi, tab, sum_ds);
     if (int e = launch_status("k_heston_table")) return e;
-    if (g->method == 1)
-      hipLaunchKernelGGL(k_heston_tangent<float>, dim3(nb), dim3(kBlock), 0, st, a,
-                         (const float*)tab, (const double*)sum_ds);
-    else
-      hipLaunchKernelGGL((k_heston_adjoint<float, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st, a,
-                         (const float*)tab, (float*)a.ckpt, (const double*)sum_ds);
+    if (v64) {
+      // the variance's constants from the same FP32 (ds, sqrt ds), in FP64
+      double* tabv = (double*)(base + lay.tabv);
+      hipLaunchKernelGGL((k_heston_table<float, double>), dim3(1), dim3(kBlock), 0, st,
+                         (const float*)g->steps, g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi,
+                         tabv, sum_ds);
+      if (int e = launch_status("k_heston_table")) return e;
+      if (g->method == 1)
+        hipLaunchKernelGGL((k_heston_tangent<float, double>), dim3(nb), dim3(kBlock), 0, st, a,
+                           (const float*)tab, (const double*)tabv, (const double*)sum_ds);
+      else
+        hipLaunchKernelGGL((k_heston_adjoint<float, double, kHestonSeg>), dim3(nb), dim3(kBlock),
+                           0, st, a, (const float*)tab, (const double*)tabv, (double*)a.ckpt,
+                           (const double*)sum_ds);
+    } else if (g->method == 1) {
+      hipLaunchKernelGGL((k_heston_tangent<float, float>), dim3(nb), dim3(kBlock), 0, st, a,
+                         (const float*)tab, (const float*)tab, (const double*)sum_ds);
+    } else {
+      hipLaunchKernelGGL((k_heston_adjoint<float, float, kHestonSeg>), dim3(nb), dim3(kBlock), 0,
+                         st, a, (const float*)tab, (const float*)tab, (float*)a.ckpt,
+                         (const double*)sum_ds);
+    }
   } else {
     double* tab = (double*)(base + lay.tab);
     hipLaunchKernelGGL(k_heston_table<double>, dim3(1), dim3(kBlock), 0, st,
@@ -622,11 +666,12 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
                        tab, sum_ds);
     if (int e = launch_status("k_heston_table")) return e;
     if (g->method == 1)
-      hipLaunchKernelGGL(k_heston_tangent<double>, dim3(nb), dim3(kBlock), 0, st, a,
-                         (const double*)tab, (const double*)sum_ds);
+      hipLaunchKernelGGL((k_heston_tangent<double, double>), dim3(nb), dim3(kBlock), 0, st, a,
+                         (const double*)tab, (const double*)tab, (const double*)sum_ds);
     else
-      hipLaunchKernelGGL((k_heston_adjoint<double, kHestonSeg>), dim3(nb), dim3(kBlock), 0, st,
-                         a, (const double*)tab, (double*)a.ckpt, (const double*)sum_ds);
+      hipLaunchKernelGGL((k_heston_adjoint<double, double, kHestonSeg>), dim3(nb), dim3(kBlock),
+                         0, st, a, (const double*)tab, (const double*)tab, (double*)a.ckpt,
+                         (const double*)sum_ds);
   }
   if (int e = launch_status(g->method == 1 ? "k_heston_tangent" : "k_heston_adjoint")) return e;
   hipLaunchKernelGGL(k_sum_columns, dim3(kHestonNS), dim3(kBlock), 0, st, (const double*)a.part,

@@ -46,7 +46,7 @@ class HestonGreeksProblem(ctypes.Structure):
                 ('rho', ctypes.c_double), ('y0', ctypes.c_double),
                 ('strike', ctypes.c_double), ('maturity', ctypes.c_double),
                 ('key', ctypes.c_uint32 * 2), ('epoch', ctypes.c_uint32),
-                ('method', ctypes.c_int32)]
+                ('method', ctypes.c_int32), ('variance_f64', ctypes.c_int32)]
 
 
 class LogGreeksProblem(ctypes.Structure):
@@ -122,16 +122,23 @@ class HestonGreeks:
     ``method='adjoint'`` sweeps the reverse-mode adjoint over recomputed
     checkpoint segments; ``method='tangent'`` carries the five derivative
     directions along the forward pass instead (fewer instructions for this
-    few parameters); both give the same pathwise estimator."""
+    few parameters); both give the same pathwise estimator.
+    ``variance_dtype=np.float64`` with an FP32 run carries the variance, its
+    checkpoints and its adjoint (every derivative, in forward mode) in FP64,
+    with the log price and the normals in FP32."""
 
     def __init__(self, *, x0=100., r=0.02, sigma=1., k=2., theta=.04, xi=.5,
                  rho=-.7, y0=.04, strike=100., maturity=1., steps=365,
                  paths=1_000_000, dtype=np.float32, payoff='call', key=None,
-                 epoch=0, rng=None, method='adjoint'):
+                 epoch=0, rng=None, method='adjoint', variance_dtype=None):
         if method not in ('adjoint', 'tangent'):
             raise ValueError("method should be 'adjoint' or 'tangent', not "
                              '{!r}'.format(method))
         code, cdt, _ = _gpu_dtype(dtype)
+        vdt = np.dtype(cdt if variance_dtype is None else variance_dtype)
+        if vdt not in (np.dtype(cdt), np.dtype(np.float64)):
+            raise ValueError('variance_dtype should be the run dtype or float64, '
+                             'not {}'.format(vdt))
         _lib.require_gpu()
         lib = _bind()
         self.paths = int(paths)
@@ -148,6 +155,7 @@ class HestonGreeks:
         pr.key[0], pr.key[1] = key
         pr.epoch = epoch
         pr.method = 0 if method == 'adjoint' else 1
+        pr.variance_f64 = int(vdt != np.dtype(cdt))
         self.key, self.prob = tuple(key), pr
         self.nbytes = lib.sdegpu_heston_greeks_workspace(pr)
         self.ws = _lib.workspace(self.nbytes)

@@ -84,6 +84,35 @@ def test_heston_adjoint_fp32_vs_fp64_oracle(gpu):
         assert abs(res.greeks[name] - ref) < 0.05 * se + 1e-3 * abs(ref), name
 
 
+def test_heston_fp32_with_fp64_variance(gpu):
+    """variance_dtype=np.float64 on an FP32 run: the variance, checkpoints and
+    adjoint (forward mode: every derivative) in FP64 with FP32 normals.  The
+    reverse and forward modes carry the variance terms in FP64, so they agree
+    far more closely than the FP32 pair; against the FP64 oracle on the same
+    paths the Greeks stay within the FP32 bound; an FP64 run ignores it."""
+    P, n = 8192, 100
+    kw = dict(paths=P, steps=n, strike=100., key=KEY, epoch=0, **HPRM)
+    a = G.heston_greeks(dtype=np.float32, variance_dtype=np.float64, **kw)
+    t = G.heston_greeks(dtype=np.float32, variance_dtype=np.float64, method='tangent', **kw)
+    f = G.heston_greeks(dtype=np.float32, **kw)
+    pay, g = og.heston_adjoint(KEY, 0, P, n, 1., strike=100., **HPRM)
+    assert abs(a.price - pay.mean()) < 2e-4 * abs(pay.mean())
+    assert abs(a.price - t.price) < 1e-12 * a.price
+    for i, name in enumerate(og.HESTON_NAMES):
+        ref = g[i].mean()
+        se = g[i].std() / np.sqrt(P)
+        assert abs(a.greeks[name] - ref) < 0.05 * se + 1e-3 * abs(ref), name
+        assert abs(t.greeks[name] - a.greeks[name]) < 1e-9 * max(1., abs(ref)) + 1e-6 * se, name
+        # the variance state is not the FP32 one: the variance Greeks move
+        if name in ('k', 'theta', 'xi', 'rho', 'y0'):
+            assert a.greeks[name] != f.greeks[name], name
+    d64 = G.heston_greeks(dtype=np.float64, **kw)
+    v64 = G.heston_greeks(dtype=np.float64, variance_dtype=np.float64, **kw)
+    assert d64.price == v64.price and d64.greeks == v64.greeks
+    with pytest.raises(ValueError):
+        G.heston_greeks(dtype=np.float64, variance_dtype=np.float32, **kw)
+
+
 def test_heston_price_matches_forward_kernel_and_fd(gpu):
     """Adjoint-kernel price == forward Euler kernel price on the same key, and
     the GPU FP64 forward + CRN finite differences == the oracle's."""
