@@ -92,7 +92,7 @@ ADJ_TRAFFIC_BYTES = 1382824621        # k_heston_adjoint<float, 8>, 4M x 365
 ADJ_VALU_WAVE_INSTS = 2183953125      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
 OU_TRAFFIC_BYTES = 16023621918         # k_euler<OU, float, 4>, 1M x 1000
-BATCH_VALU_WAVE_INSTS = 32935748358    # k_batch_persistent, the 1024-instance set
+BATCH_VALU_WAVE_INSTS = 32673026848    # k_batch_persistent, the 1024-instance set, 8 tiles per claim
 MERTON_VALU_WAVE_INSTS = 362572745     # k_loglin_greeks<float>, 2M x 252
 TANGENT_VALU_WAVE_INSTS = 1523390625   # k_heston_tangent<float>, 4M x 365
 
@@ -744,7 +744,8 @@ def main():
             'instances': bt.n, 'tiles': bt.n_items,
             'path_steps': bt.path_steps, 'ms': bms,
             'path_steps_per_s': bt.path_steps / (bms * 1e-3),
-            'blocks': bt.blocks, 'cu': bt.cu,
+            'blocks': bt.blocks, 'cu': bt.cu, 'tile_waves': bt.tile_waves,
+            'claim_order': bt.order,
             'autotune_s': {str(k): v for k, v in bt.tuning.items()},
             'prepare_s': prep, 'graph': bt.graph is not None,
             'fp32_vs_fp64_same_paths': inst_err,

@@ -482,15 +482,20 @@ typedef struct sdegpu_batch_instance {
 typedef struct sdegpu_batch {
   const sdegpu_batch_instance* inst; /* device [n_inst] */
   int32_t n_inst;
-  int32_t n_items;                   /* number of tiles */
-  const int32_t* work;               /* device [n_items][2] = (instance, tile), claim order */
+  int32_t n_items;                   /* number of claims */
+  const int32_t* work;               /* device [n_items][2] = (instance, claim), claim order;
+                                        claim c of an instance covers its 64-path tiles
+                                        c*tile_waves .. c*tile_waves + tile_waves - 1 */
   const int64_t* slot0;              /* device [n_inst]: first partial slot of each instance */
   const float* coef;                 /* device: concatenated FP32 step tables */
-  double* part;                      /* device [n_items][2] workspace */
+  double* part;                      /* device [sum of ceil(paths/64)][2] workspace: one
+                                        slot per 64-path tile, from slot0[instance] on */
   unsigned int* counter;             /* device, one word */
   double* mean;                      /* device [n_inst] */
   double* m2;                        /* device [n_inst]: sum of squared deviations */
   int32_t blocks;                    /* persistent grid: blocks of 256 threads */
+  int32_t tile_waves;                /* 64-path tiles per claim, stepped in turn by the
+                                        wave that claims them (0 means 1) */
 } sdegpu_batch;
 
 int sdegpu_batch_run(const sdegpu_batch* batch, void* stream);

@@ -15,9 +15,10 @@ A prepared batch is a fixed computation: every ``run()`` replays the same
 Philox epochs and returns the same numbers (build a new batch from the
 processes for fresh draws).  Only float32 processes are accepted.
 
-The persistent grid size is chosen by a small autotuner: the first run of a
-batch shape times each candidate (4..32 wavefronts per CU) and the winner is
-cached on disk (``SDEPY_AMD_TUNE_CACHE``, default
+The launch shape is chosen by a small autotuner: the first run of a batch
+shape searches the persistent grid (8..48 wavefronts per CU), the paths per
+claim (64 to 1024) and the claim order, one axis at a time, and the
+winner is cached on disk (``SDEPY_AMD_TUNE_CACHE``, default
 ``~/.cache/sdepy_amd/batched_tune.json``).  Launches are captured once into a
 HIP graph and replayed.
 """
@@ -55,7 +56,7 @@ class Batch(ctypes.Structure):
                 ('slot0', ctypes.c_void_p), ('coef', ctypes.c_void_p),
                 ('part', ctypes.c_void_p), ('counter', ctypes.c_void_p),
                 ('mean', ctypes.c_void_p), ('m2', ctypes.c_void_p),
-                ('blocks', ctypes.c_int32)]
+                ('blocks', ctypes.c_int32), ('tile_waves', ctypes.c_int32)]
 
 
 def _bind():
@@ -133,9 +134,22 @@ def _instance(proc, timeline):
 
 class BatchedTerminalMoments:
     """Prepared batch: descriptors, step tables, work list and outputs on the
-    device; ``run()`` replays the captured graph and returns (mean, var)."""
+    device; ``run()`` replays the captured graph and returns (mean, var).
 
-    def __init__(self, specs, autotune=True, blocks=None, graph=True):
+    The launch shape is tuned per batch (``autotune=True``, cached by the
+    batch's shape): the persistent grid (``blocks``, a multiple of the CU
+    count), the claim size (``tile_waves`` 64-path tiles stepped in turn by
+    the wave that claims them) and the claim order (``order``: 'cost', the
+    claims by decreasing steps x components; 'instance', instance by
+    instance; 'interleave', the instances' first claims first).  Any point
+    of that space gives the same result."""
+
+    BLOCKS_PER_CU = (2, 4, 6, 7, 8, 12)
+    TILE_WAVES = (1, 2, 4, 8, 16)
+    ORDERS = ('cost', 'instance', 'interleave')
+
+    def __init__(self, specs, autotune=True, blocks=None, graph=True,
+                 tile_waves=None, order=None):
         torch = _lib.require_gpu()
         lib = _bind()
         descs, tabs = [], []
@@ -151,42 +165,42 @@ class BatchedTerminalMoments:
             raise ValueError('empty batch')
         self.paths = np.array([d.paths for d in descs], dtype=np.int64)
         self.nsteps = np.array([d.nsteps for d in descs], dtype=np.int64)
-        cost = self.nsteps * np.array([d.nvars for d in descs])
+        self.nvars = np.array([d.nvars for d in descs], dtype=np.int64)
         ntiles = (self.paths + TILE - 1) // TILE
         slot0 = np.concatenate(([0], np.cumsum(ntiles)[:-1])).astype(np.int64)
-        inst_of = np.repeat(np.arange(self.n), ntiles)
-        tile_of = np.concatenate([np.arange(k) for k in ntiles])
-        order = np.argsort(-cost[inst_of], kind='stable')
-        work = np.stack((inst_of[order], tile_of[order]), axis=1).astype(np.int32)
-        self.n_items = int(ntiles.sum())
         arr = (BatchInstance * self.n)(*descs)
         raw = np.frombuffer(bytes(arr), dtype=np.uint8)
         self.d_inst = _lib.to_device(raw)
-        self.d_work = _lib.to_device(work)
         self.d_slot0 = _lib.to_device(slot0)
         self.d_coef = _lib.to_device(np.concatenate(tabs).astype(np.float32))
-        self.d_part = _lib.empty((self.n_items, 2), np.float64)
+        self.d_part = _lib.empty((int(ntiles.sum()), 2), np.float64)
         self.d_counter = _lib.zeros((1,), np.int32)
         self.d_mean = _lib.empty((self.n,), np.float64)
         self.d_m2 = _lib.empty((self.n,), np.float64)
+        self._works = {}
         self.stream = torch.cuda.Stream()
-        self.sig = hashlib.sha1(work.tobytes() + self.nsteps.tobytes() +
-                                self.paths.tobytes()).hexdigest()[:16]
+        self.sig = hashlib.sha1(b'v2' + self.nsteps.tobytes() + self.paths.tobytes() +
+                                self.nvars.tobytes()).hexdigest()[:16]
         cu = ctypes.c_int(0)
         _lib.check(lib.sdegpu_cu_count(ctypes.byref(cu)))
         self.cu = cu.value
-        self.candidates = [self.cu * k for k in (1, 2, 4, 8)]
+        self.candidates = [self.cu * k for k in self.BLOCKS_PER_CU]
         self.graph = None
         self.use_graph = graph
         self.tuning = {}
-        if blocks is None:
-            blocks = self._cached_blocks()
-            if blocks is None and autotune:
-                blocks = self._autotune()
-            if blocks is None:
-                blocks = self.cu * 4
-        self.blocks = int(blocks)
-        self._desc = self._batch(self.blocks)
+        point = {'blocks': blocks, 'tile_waves': tile_waves, 'order': order}
+        if any(v is None for v in point.values()):
+            cached = self._cached_point()
+            if cached is None and autotune:
+                cached = self._autotune(point)
+            for k, v in (cached or {}).items():
+                if point[k] is None:
+                    point[k] = v
+        self.blocks = int(point['blocks'] or self.cu * 4)
+        self.tile_waves = int(point['tile_waves'] or 1)
+        self.order = point['order'] or 'cost'
+        self._desc = self._batch(self.blocks, self.tile_waves, self.order)
+        self.n_items = self._desc.n_items
         if graph:
             g = ctypes.c_void_p()
             with torch.cuda.stream(self.stream):
@@ -195,40 +209,98 @@ class BatchedTerminalMoments:
                     ctypes.byref(g)))
             self.graph = g
 
-    def _batch(self, blocks):
+    def _work(self, tile_waves, order):
+        """Device claim list [(instance, claim)] for a claim size and order."""
+        key = (tile_waves, order)
+        if key not in self._works:
+            per = TILE * tile_waves
+            nclaims = (self.paths + per - 1) // per
+            inst_of = np.repeat(np.arange(self.n), nclaims)
+            claim_of = np.concatenate([np.arange(k) for k in nclaims])
+            cost = (self.nsteps * self.nvars)[inst_of]
+            if order == 'cost':
+                idx = np.argsort(-cost, kind='stable')
+            elif order == 'instance':
+                idx = np.arange(inst_of.size)
+            elif order == 'interleave':
+                # claim 0 of every instance (costliest first), then claim 1, ...
+                idx = np.lexsort((-cost, claim_of))
+            else:
+                raise ValueError('claim order should be one of {}, not {!r}'
+                                 .format(self.ORDERS, order))
+            work = np.stack((inst_of[idx], claim_of[idx]), axis=1).astype(np.int32)
+            self._works[key] = _lib.to_device(work)
+        return self._works[key]
+
+    def _batch(self, blocks, tile_waves=1, order='cost'):
+        work = self._work(tile_waves, order)
         b = Batch()
-        b.inst, b.n_inst, b.n_items = self.d_inst.data_ptr(), self.n, self.n_items
-        b.work, b.slot0 = self.d_work.data_ptr(), self.d_slot0.data_ptr()
+        b.inst, b.n_inst, b.n_items = self.d_inst.data_ptr(), self.n, work.shape[0]
+        b.work, b.slot0 = work.data_ptr(), self.d_slot0.data_ptr()
         b.coef, b.part = self.d_coef.data_ptr(), self.d_part.data_ptr()
         b.counter = self.d_counter.data_ptr()
         b.mean, b.m2 = self.d_mean.data_ptr(), self.d_m2.data_ptr()
         b.blocks = int(blocks)
+        b.tile_waves = int(tile_waves)
         return b
 
-    def _cached_blocks(self):
+    def _cached_point(self):
         try:
             with open(_cache_path()) as f:
-                return json.load(f).get(self.sig)
+                got = json.load(f).get(self.sig)
         except (OSError, ValueError):
             return None
+        return got if isinstance(got, dict) else None
 
-    def _autotune(self, reps=2):
-        torch = _lib.torch_mod()
+    def _time(self, blocks, tile_waves, order, reps):
         lib = _bind()
-        best, best_t = None, float('inf')
         st = ctypes.c_void_p(self.stream.cuda_stream)
-        for blocks in self.candidates:
-            b = self._batch(blocks)
-            _lib.check(lib.sdegpu_batch_run(b, st))     # warm
-            self.stream.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                _lib.check(lib.sdegpu_batch_run(b, st))
-            self.stream.synchronize()
-            t = (time.perf_counter() - t0) / reps
-            self.tuning[blocks] = t
-            if t < best_t:
-                best, best_t = blocks, t
+        b = self._batch(blocks, tile_waves, order)
+        _lib.check(lib.sdegpu_batch_run(b, st))     # warm
+        self.stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(lib.sdegpu_batch_run(b, st))
+        self.stream.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        self.tuning['{}/{}/{}'.format(blocks, tile_waves, order)] = t
+        return t
+
+    def _autotune(self, fixed, reps=2, sweeps=2):
+        """Coordinate search over (blocks, tile_waves, order), each axis in
+        turn with the others held, from (4 blocks per CU, 1, 'cost'), until
+        a sweep changes nothing; the values given by the caller stay fixed.
+        Of the points within 0.5% of the fastest, the one with the smallest
+        claims is returned, and cached by the batch's shape."""
+        space = {'blocks': self.candidates, 'tile_waves': list(self.TILE_WAVES),
+                 'order': list(self.ORDERS)}
+        cur = {'blocks': self.cu * 4, 'tile_waves': 1, 'order': 'cost'}
+        for k, v in fixed.items():
+            if v is not None:
+                space[k], cur[k] = [v], v
+        times = {}
+
+        def t_of(p):
+            key = (p['blocks'], p['tile_waves'], p['order'])
+            if key not in times:
+                times[key] = self._time(*key, reps)
+            return times[key]
+        best_t = t_of(cur)
+        for _ in range(sweeps):
+            changed = False
+            for axis in ('blocks', 'tile_waves', 'order'):
+                for v in space[axis]:
+                    p = dict(cur, **{axis: v})
+                    t = t_of(p)
+                    if t < best_t:
+                        cur, best_t, changed = p, t, True
+            if not changed:
+                break
+        # within 0.5% of the fastest the differences are noise: take the
+        # smallest claims there (the finest balance for the batch's tail)
+        near = [k for k, t in times.items() if t <= best_t * 1.005]
+        b, tw, o = min(near, key=lambda k: (k[1], times[k]))
+        cur = {'blocks': b, 'tile_waves': tw, 'order': o}
         try:
             path = _cache_path()
             os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -237,13 +309,12 @@ class BatchedTerminalMoments:
                     cache = json.load(f)
             except (OSError, ValueError):
                 cache = {}
-            cache[self.sig] = best
+            cache[self.sig] = cur
             with open(path, 'w') as f:
                 json.dump(cache, f)
         except OSError:
             pass
-        del torch
-        return best
+        return cur
 
     def launch(self):
         """Issue the batch on this object's stream (asynchronous)."""

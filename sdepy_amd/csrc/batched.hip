@@ -8,12 +8,16 @@
 // other, each as a Python loop over steps.
 //
 // Work decomposition: each instance is cut into tiles of 64 paths (one
-// wavefront).  The host sorts the tiles by decreasing cost (steps x
-// components) and the persistent grid's wavefronts claim them through an
-// atomic counter until the list is exhausted, so uneven instances balance
-// across the 256 CUs.  Every tile writes its partial sums to a slot fixed by
-// (instance, tile), and a second kernel adds each instance's slots in order:
-// the result does not depend on which wavefront ran which tile.
+// wavefront).  A claim covers tile_waves consecutive tiles of one instance
+// (64, 128 or 256 paths), stepped one after the other by the wave that
+// claimed it; the host orders the claims (by decreasing cost, steps x
+// components; by instance; or interleaving the instances) and the persistent
+// grid's wavefronts take them through an atomic counter until the list is
+// exhausted, so uneven instances balance across the 256 CUs.  The grid
+// size, the claim size and the order are tuned per batch on the host.
+// Every 64-path tile writes its partial sums to a slot fixed by (instance,
+// tile), and a second kernel adds each instance's slots in order: the result
+// does not depend on which wavefront ran which tile, nor on the tuning.
 //
 // Each instance draws exactly the Philox numbers the stand-alone Euler kernel
 // would draw for it (same key, epoch, counter layout), so a batched instance
@@ -117,31 +121,37 @@ __device__ __forceinline__ double wave_sum64(double v) {
 __global__ __launch_bounds__(kBlock) void k_batch_persistent(
     const sdegpu_batch_instance* __restrict__ insts, const int32_t* __restrict__ work,
     const int64_t* __restrict__ slot0, const float* __restrict__ coef,
-    double* __restrict__ part, unsigned int* __restrict__ counter, int32_t n_items) {
+    double* __restrict__ part, unsigned int* __restrict__ counter, int32_t n_items,
+    int32_t tile_waves) {
   const int lane = threadIdx.x & 63;
   for (;;) {
     const unsigned int old = atomicAdd(counter, lane == 0 ? 1u : 0u);
     const int w = (int)__builtin_amdgcn_readfirstlane(old);
     if (w >= n_items) break;
-    const int inst = work[2 * w], tile = work[2 * w + 1];
+    const int inst = work[2 * w], claim = work[2 * w + 1];
     const sdegpu_batch_instance& d = insts[inst];
-    const int64_t pidx = (int64_t)tile * kTile + lane;
-    const bool live = pidx < d.paths;
-    const uint32_t path = (uint32_t)pidx;
-    float v;
-    switch (d.model * 16 + d.nvars) {
-      case SDEGPU_MODEL_LOGNORM * 16 + 1: v = run_tile<SDEGPU_MODEL_LOGNORM, 1>(d, coef, path, live); break;
-      case SDEGPU_MODEL_WIENER * 16 + 1: v = run_tile<SDEGPU_MODEL_WIENER, 1>(d, coef, path, live); break;
-      case SDEGPU_MODEL_OU * 16 + 1: v = run_tile<SDEGPU_MODEL_OU, 1>(d, coef, path, live); break;
-      case SDEGPU_MODEL_OU * 16 + 2: v = run_tile<SDEGPU_MODEL_OU, 2>(d, coef, path, live); break;
-      case SDEGPU_MODEL_OU * 16 + 3: v = run_tile<SDEGPU_MODEL_OU, 3>(d, coef, path, live); break;
-      default: v = run_tile<SDEGPU_MODEL_OU, 4>(d, coef, path, live); break;
+    // the claim's 64-path tiles, the last one possibly short or absent
+    for (int sub = 0; sub < tile_waves; ++sub) {
+      const int64_t tile = (int64_t)claim * tile_waves + sub;
+      if (tile * kTile >= d.paths) break;
+      const int64_t pidx = tile * kTile + lane;
+      const bool live = pidx < d.paths;
+      const uint32_t path = (uint32_t)pidx;
+      float v;
+      switch (d.model * 16 + d.nvars) {
+        case SDEGPU_MODEL_LOGNORM * 16 + 1: v = run_tile<SDEGPU_MODEL_LOGNORM, 1>(d, coef, path, live); break;
+        case SDEGPU_MODEL_WIENER * 16 + 1: v = run_tile<SDEGPU_MODEL_WIENER, 1>(d, coef, path, live); break;
+        case SDEGPU_MODEL_OU * 16 + 1: v = run_tile<SDEGPU_MODEL_OU, 1>(d, coef, path, live); break;
+        case SDEGPU_MODEL_OU * 16 + 2: v = run_tile<SDEGPU_MODEL_OU, 2>(d, coef, path, live); break;
+        case SDEGPU_MODEL_OU * 16 + 3: v = run_tile<SDEGPU_MODEL_OU, 3>(d, coef, path, live); break;
+        default: v = run_tile<SDEGPU_MODEL_OU, 4>(d, coef, path, live); break;
+      }
+      const double dv = live ? (double)v - d.shift : 0.0;
+      const double s1 = wave_sum64(dv), s2 = wave_sum64(dv * dv);
+      const int64_t slot = slot0[inst] + tile;
+      part[2 * slot] = s1;
+      part[2 * slot + 1] = s2;
     }
-    const double dv = live ? (double)v - d.shift : 0.0;
-    const double s1 = wave_sum64(dv), s2 = wave_sum64(dv * dv);
-    const int64_t slot = slot0[inst] + tile;
-    part[2 * slot] = s1;
-    part[2 * slot + 1] = s2;
   }
 }
 
@@ -192,6 +202,8 @@ static int check_batch(const sdegpu_batch* b) {
     return fail(SDEGPU_EINVAL, "batch: empty batch or missing buffer");
   if (b->blocks < 1 || b->blocks > 65536)
     return fail(SDEGPU_EINVAL, "batch: grid of %d blocks", b->blocks);
+  if (b->tile_waves < 0 || b->tile_waves > 16)
+    return fail(SDEGPU_EINVAL, "batch: %d tiles per claim (0..16)", b->tile_waves);
   return SDEGPU_OK;
 }
 
@@ -199,7 +211,8 @@ static int launch_batch(const sdegpu_batch* b, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->counter);
   if (int e = launch_status("k_batch_reset")) return e;
   hipLaunchKernelGGL(k_batch_persistent, dim3(b->blocks), dim3(kBlock), 0, st, b->inst, b->work,
-                     b->slot0, b->coef, b->part, b->counter, b->n_items);
+                     b->slot0, b->coef, b->part, b->counter, b->n_items,
+                     b->tile_waves > 0 ? b->tile_waves : 1);
   if (int e = launch_status("k_batch_persistent")) return e;
   hipLaunchKernelGGL(k_batch_finish, dim3(b->n_inst), dim3(kBlock), 0, st, b->inst, b->slot0,
                      (const double*)b->part, b->n_inst, b->mean, b->m2);

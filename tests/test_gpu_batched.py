@@ -76,12 +76,42 @@ def test_batch_matches_standalone_and_oracle(gpu):
         assert abs(var[i] - xt.var()) < 2e-4 * xt.var() + 1e-12, (i, kind)
 
 
+def test_batch_launch_shape_does_not_change_results(gpu, tmp_path, monkeypatch):
+    """Every (grid, paths per claim, claim order) gives bit-identical moments:
+    each 64-path tile writes its own slot and the slots are merged in order;
+    the tuner returns a point of the space and caches it by the batch shape."""
+    monkeypatch.setenv('SDEPY_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    ref = None
+    for tw in B.BatchedTerminalMoments.TILE_WAVES:
+        for order in B.BatchedTerminalMoments.ORDERS:
+            for blocks in (64, 1024):
+                # fresh processes: a batch takes the next epoch of each source
+                bt = B.BatchedTerminalMoments(_specs(5)[0], blocks=blocks, tile_waves=tw,
+                                              order=order, graph=False)
+                m, v = bt.run()
+                if ref is None:
+                    ref = (m, v)
+                assert np.array_equal(m, ref[0]) and np.array_equal(v, ref[1]), \
+                    (tw, order, blocks)
+    tuned = B.BatchedTerminalMoments(_specs(5)[0], autotune=True, graph=False)
+    assert tuned.blocks in tuned.candidates
+    assert tuned.tile_waves in tuned.TILE_WAVES and tuned.order in tuned.ORDERS
+    assert len(tuned.tuning) >= len(tuned.candidates) + 3
+    again = B.BatchedTerminalMoments(_specs(5)[0], autotune=True, graph=False)
+    assert not again.tuning     # taken from the cache
+    assert (again.blocks, again.tile_waves, again.order) == \
+        (tuned.blocks, tuned.tile_waves, tuned.order)
+    m, v = tuned.run()
+    assert np.array_equal(m, ref[0]) and np.array_equal(v, ref[1])
+
+
 def test_batch_graph_replays_equal_eager(gpu):
     torch = gpu
     specs, _ = _specs(9)
     bt = B.BatchedTerminalMoments(specs, autotune=True, graph=True)
     assert bt.blocks in bt.candidates
-    eager = B.BatchedTerminalMoments(_specs(9)[0], blocks=bt.blocks, graph=False)
+    eager = B.BatchedTerminalMoments(_specs(9)[0], blocks=bt.blocks,
+                                     tile_waves=bt.tile_waves, order=bt.order, graph=False)
     m0, v0 = eager.run()
     for rep in range(4):
         m, v = bt.run()
