@@ -687,6 +687,10 @@ def main():
                                       rng=np.random.default_rng(1234))
         prep = c1_proc().prepare(tt1)
         k_ms, _ = timed(torch, lambda: prep.launch(fresh=True), 20, 10)
+        # each variant once untimed first (its kernels' first use, the first
+        # pinned result buffer), then 5 timed calls
+        x = c1_proc()(tt1)
+        x.pmean(), x.pvar()
         t0 = time.perf_counter()
         reps = 5
         for _ in range(reps):
@@ -696,6 +700,8 @@ def main():
         # the same through a prepared run whose kernel reduces the rows as
         # it stores them: the path array still comes back to the host, the
         # moments without another upload or pass
+        run = c1_proc().prepare(tt1, moments=True)
+        run.run()
         t0 = time.perf_counter()
         for _ in range(reps):
             run = c1_proc().prepare(tt1, moments=True)

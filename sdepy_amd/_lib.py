@@ -183,14 +183,19 @@ def stream():
     return ctypes.c_void_p(torch_mod().cuda.current_stream().cuda_stream)
 
 
-# Host <-> device copies of large arrays go through two pinned staging
-# buffers allocated once per process: the DMA runs at pinned-memory speed and
-# the copy between the staging buffer and the caller's pageable NumPy array
-# overlaps the DMA of the next chunk.  (A pageable copy is staged by the
-# runtime one buffer at a time, about 4x slower for a 20 MB result.)  Each
-# staging buffer is reused only after the event recorded behind its last DMA.
+# Host -> device copies of large arrays, and device -> host copies above
+# _PINNED_MAX, go through two pinned staging buffers allocated once per
+# process: the DMA runs at pinned-memory speed and the copy between the
+# staging buffer and the caller's pageable NumPy array overlaps the DMA of the
+# next chunk.  (A pageable copy is staged by the runtime one buffer at a time,
+# about 4x slower for a 20 MB result.)  Each staging buffer is reused only
+# after the event recorded behind its last DMA.  Device -> host copies up to
+# _PINNED_MAX land directly in a pinned result array (20 MB: 0.39 ms against
+# 0.89 ms staged; 320 MB: 5.9 ms against 24.7 ms, where the staged copy-out
+# leaves the caches).
 _STAGE_CHUNK = 8 << 20
 _STAGE_MIN = 1 << 20
+_PINNED_MAX = 1 << 30
 _stage = None
 _stage_lock = threading.Lock()
 
@@ -269,12 +274,23 @@ def ptr(t):
 
 
 def to_host(t):
-    """Copy a device tensor into a new NumPy array (synchronous)."""
+    """Copy a device tensor into a new NumPy array (synchronous).
+
+    Up to _PINNED_MAX bytes the array is pinned host memory from torch's
+    caching host allocator, which the DMA fills directly (no second copy on
+    the host; the memory returns to the cache when the array is freed).
+    Larger arrays are pageable, filled through two pinned staging buffers
+    with the DMA of one chunk overlapping the copy-out of the other."""
     npdt = _np_dtype(t.dtype)
     n = t.numel() * t.element_size()
     if n < _STAGE_MIN or npdt is None or not t.is_cuda:
         return t.cpu().numpy()
     torch = torch_mod()
+    if n <= _PINNED_MAX:
+        h = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        return h.numpy()
     src = t.contiguous().reshape(-1).view(torch.uint8)
     out = np.empty(tuple(t.shape), dtype=npdt)
     dst = out.reshape(-1).view(np.uint8)
