@@ -191,10 +191,11 @@ typedef struct sdegpu_euler_problem {
                                  pass over it, infrastructure.py:851-889) */
   int32_t row_moments_rows;   /* output rows (nrows) when row_moments is given */
   const void* param_paths;    /* optional device [NB][nvars_total][paths] (dtype): model
-                                 parameters that vary across paths, constant in time (sdepy
-                                 broadcasts parameters to vshape + (paths,),
-                                 infrastructure.py:64-108, integration.py:1218-1221); they
-                                 replace the table's parameter blocks.  NB blocks of the
+                                 parameters that vary across paths (sdepy broadcasts
+                                 parameters to vshape + (paths,), infrastructure.py:64-108,
+                                 integration.py:1218-1221), the same for every step unless
+                                 param_paths_step is set; they replace the table's
+                                 parameter blocks.  NB blocks of the
                                  working components (of the N log-x values for HESTON):
                                    WIENER mu, sigma;  LOGNORM, JUMPDIFF mu - sigma^2/2, sigma;
                                    OU k, k theta, sigma (sigma not folded into the table's
@@ -202,6 +203,10 @@ typedef struct sdegpu_euler_problem {
                                    HESTON mu, sigma, theta, k, xi.
                                  The jump law's parameters stay in the table.  Not with
                                  row_moments. */
+  int64_t param_paths_step;   /* 0, or the elements between consecutive steps' blocks:
+                                 param_paths is then [nsteps][NB][nvars_total][paths],
+                                 step s's parameters evaluated at its start (parameters
+                                 that vary across paths and in time) */
 } sdegpu_euler_problem;
 
 /*

@@ -57,7 +57,7 @@ class EulerProblem(ctypes.Structure):
         ('out0', c_int32), ('out_stride', c_int32), ('jump_defer', c_int32),
         ('info_row_slots', c_int32), ('jump_law', c_int32),
         ('row_moments', c_void_p), ('row_moments_rows', c_int32),
-        ('param_paths', c_void_p),
+        ('param_paths', c_void_p), ('param_paths_step', c_int64),
     ]
 
 

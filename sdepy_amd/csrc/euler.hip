@@ -129,6 +129,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EUNSUPPORTED,
                 "a launch of %d Heston components steps one group of width %d, not %d", m, m / 2,
                 hd);
+  if (pr->param_paths_step < 0)
+    return fail(SDEGPU_EINVAL, "negative param_paths_step %lld", (long long)pr->param_paths_step);
   if (pr->info_row_slots < 0 || pr->info_row_slots > 1024)
     return fail(SDEGPU_EINVAL, "info_row_slots must be in [0, 1024], not %d", pr->info_row_slots);
   if (pr->out0 < 0 || out_stride < pr->nout || pr->out0 > out_stride - pr->nout)
@@ -163,6 +165,7 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.jump_law = pr->model == SDEGPU_MODEL_JUMPDIFF ? pr->jump_law : SDEGPU_JUMP_NORMAL;
   a.row_mom = pr->row_moments;
   a.ppar = pr->param_paths;
+  a.ppar_step = pr->param_paths ? pr->param_paths_step : 0;
   a.mom_rows = (int64_t)pr->row_moments_rows * out_stride;
   a.nvars = m;
   for (int v = 0; v < SDEGPU_MAX_COUPLED; ++v) a.x0[v] = pr->x0[v];

@@ -58,6 +58,7 @@ struct EulerArgs {
   int64_t mom_rows;    // output rows * out_stride
   const void* ppar;    // per-path parameters (PP instances): [PathParams<MODEL>::NB][..][paths]
   int32_t nvars;       // components this launch steps (the run-time width of k_euler_wide)
+  int64_t ppar_step;   // elements between steps' blocks of ppar (0: the same every step)
 };
 
 // Per-path parameters (the PP instances of k_euler): the model's parameter
@@ -355,18 +356,22 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
   const int64_t c0 = GEN ? a.comp0 : 0;
   const bool grouped = GEN && (a.m_total != M || a.comp0 != 0);
 
-  // per-path parameters, read once (PP instances)
+  // per-path parameters (PP instances): read once, or at every step when
+  // they also change in time (ppar_step != 0)
   constexpr int NP = kHeston ? N : M;
   constexpr int NBP = PathParams<MODEL>::NB;
   T pp[PP ? NBP * NP : 1];
-  if constexpr (PP) {
-    const int64_t mtb = kHeston ? mt / 2 : mt, c0b = kHeston ? c0 / 2 : c0;
+  auto load_pp = [&](int64_t base) {
+    if constexpr (PP) {
+      const int64_t mtb = kHeston ? mt / 2 : mt, c0b = kHeston ? c0 / 2 : c0;
 #pragma unroll
-    for (int b = 0; b < NBP; ++b)
+      for (int b = 0; b < NBP; ++b)
 #pragma unroll
-      for (int v = 0; v < NP; ++v)
-        pp[b * NP + v] = ((const T*)a.ppar)[((int64_t)b * mtb + c0b + v) * P + gid];
-  }
+        for (int v = 0; v < NP; ++v)
+          pp[b * NP + v] = ((const T*)a.ppar)[base + ((int64_t)b * mtb + c0b + v) * P + gid];
+    }
+  };
+  load_pp(0);
 
   T x[M];
   if (a.x0_paths) {
@@ -530,6 +535,7 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
       const T* p = c + a.poff;
       T prow[PP ? NBP * NP : 1];
       if constexpr (PP) {
+        if (a.ppar_step != 0) load_pp((int64_t)s * a.ppar_step);
         // the step's parameter blocks from the path's own values
 #pragma unroll
         for (int i = 0; i < NBP * NP; ++i) prow[i] = pp[i];

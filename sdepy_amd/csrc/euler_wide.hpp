@@ -56,9 +56,11 @@ __global__ __launch_bounds__(kBlock) void k_euler_wide(const EulerArgs a,
   auto x0_of = [&](int v) -> T {
     return x0p ? x0p[(c0 + v) * P + gid] : (T)a.x0[v];
   };
-  // parameter b of component (Heston: pair) v at a step whose block starts at p
+  // parameter b of component (Heston: pair) v at a step whose block starts at
+  // p (per path: at the step's offset pbase into ppar)
+  int64_t pbase = 0;
   auto par = [&](const T* p, int b, int v) -> T {
-    return pp ? ppar[((int64_t)b * mtb + c0b + v) * P + gid] : p[b * n + v];
+    return pp ? ppar[pbase + ((int64_t)b * mtb + c0b + v) * P + gid] : p[b * n + v];
   };
 
   T x[NS];   // log x for Heston
@@ -113,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_euler_wide(const EulerArgs a,
     const T* L = c + 2;
     const T* p = c + a.poff;
     const uint64_t g = (uint64_t)s * mt + c0;  // flat index of the block's first normal
+    pbase = (int64_t)s * a.ppar_step;
     const T* rp = replay ? (const T*)a.replay_dw + (int64_t)g * P + gid : nullptr;
 
     if constexpr (kHeston) {

@@ -288,21 +288,20 @@ class _gpu_sde:
         from the parameters broadcast to the parameter shape + (paths,)."""
         return [prm['mu'], prm['sigma']]
 
-    def _path_params(self):
+    def _path_params(self, t0s):
         """None when every SDE parameter is the same on all paths (possibly
-        callable in t); else the FP64 per-path blocks [NB][M][paths] of the
-        kernels' per-path parameter instances.  Parameters may vary across
-        paths or in time, not both."""
+        callable in t); else the per-path blocks of the kernels' per-path
+        parameter instances, in FP64: [NB][M][paths] when no parameter is a
+        callable, [nsteps][NB][M][paths] (each step's values at its start,
+        t0s) when one is.  A callable counts as per-path when its value at
+        the first step varies across paths."""
         shape, P = self._param_shape(), self.paths
-        per_path, timed, prm = False, False, {}
-        for name in self.sde_defaults:
-            z = self._args[name]
-            if callable(z):
-                timed = True
-                continue
+
+        def to_paths(name, z):
             a = np.asarray(z, dtype=float)
             try:
                 np.broadcast_to(a, shape + (1,))
+                varies = False
             except ValueError:
                 try:
                     np.broadcast_to(a, shape + (P,))
@@ -311,17 +310,35 @@ class _gpu_sde:
                         'cannot broadcast parameter {} shaped {} to the '
                         'process shape vshape + (paths,) = {}'.format(
                             name, a.shape, shape + (P,)))
-                per_path = True
-            prm[name] = np.broadcast_to(a, shape + (P,))
+                varies = True
+            return np.broadcast_to(a, shape + (P,)), varies
+
+        per_path, prm, timed = False, {}, []
+        for name in self.sde_defaults:
+            z = self._args[name]
+            if callable(z):
+                timed.append(name)
+                if len(t0s):
+                    per_path |= to_paths(name, z(t0s[0]))[1]
+                continue
+            prm[name], varies = to_paths(name, z)
+            per_path |= varies
         if not per_path:
             return None
-        if timed:
-            raise NotImplementedError(
-                'the GPU integrator takes parameters that vary across paths '
-                'or in time, not both')
         n = int(np.prod(shape, dtype=int))
-        return np.stack([np.ascontiguousarray(b, dtype=float).reshape(n, P)
-                         for b in self._path_blocks(prm)])
+
+        def blocks(values):
+            return np.stack([np.ascontiguousarray(b, dtype=float).reshape(n, P)
+                             for b in self._path_blocks(values)])
+        if not timed:
+            return blocks(prm)
+        nb = self.n_param_blocks
+        if not len(t0s):
+            return np.zeros((0, nb, n, P))
+        return np.stack([
+            blocks(dict(prm, **{name: to_paths(name, self._args[name](t))[0]
+                                for name in timed}))
+            for t in t0s])
 
     # per-parameter evaluation -------------------------------------------
 
@@ -417,17 +434,21 @@ class _gpu_sde:
             host_sources['dw'] = dw
         has_corr = wsrc is not None and wsrc.corr is not None
         groups = self._groups(has_corr, out_mode, out_group)
-        path_params = self._path_params()
+        path_params = self._path_params(t0s)
         if path_params is None:
             params = self._param_rows(t0s, ds)
         else:
             # the kernel takes the model's parameters from the per-path
-            # array; the table keeps the width of its parameter blocks
+            # array (one set per step when they also change in time); the
+            # table keeps the width of its parameter blocks
             nb_width = self.n_param_blocks * int(np.prod(self._param_shape(), dtype=int))
             params = np.zeros((nsteps, nb_width))
-            d_pp = _lib.to_device(path_params.astype(cdt))
-            keep.append(d_pp)
-            prob.param_paths = d_pp.data_ptr()
+            if path_params.size:
+                d_pp = _lib.to_device(path_params.astype(cdt))
+                keep.append(d_pp)
+                prob.param_paths = d_pp.data_ptr()
+                if path_params.ndim == 4:
+                    prob.param_paths_step = path_params[0].size
         dj = self.sources.get('dj')
         self._anti_dj = False
         pois = laws = None

@@ -206,8 +206,8 @@ def test_merton_per_path_sigma_and_backward(gpu):
 def test_per_path_moments_and_refusals(gpu):
     """prepare(moments=True) with per-path parameters reduces the stored
     rows in a pass of its own (the fused form takes constant parameters)
-    with the same result; parameters varying across paths and in time, or
-    not broadcastable to vshape + (paths,), are refused before any launch."""
+    with the same result; parameters not broadcastable to vshape + (paths,)
+    are refused before any launch."""
     P, tt = 4096, np.linspace(0., 1., 9)
     sigma = np.random.default_rng(16).uniform(.1, .4, P)
     p = sa.lognorm_process(paths=P, steps=33, x0=1., mu=.05, sigma=sigma,
@@ -219,7 +219,70 @@ def test_per_path_moments_and_refusals(gpu):
     var = np.asarray(run.pvar(dtype=np.float64)).reshape(x.shape[:-1])
     assert np.allclose(mean, x.mean(axis=-1), rtol=1e-12, atol=0)
     assert np.allclose(var, x.var(axis=-1), rtol=1e-10, atol=1e-300)
-    with pytest.raises(NotImplementedError):
-        sa.lognorm_process(paths=P, mu=lambda t: .1 * t, sigma=sigma)(tt)
     with pytest.raises(ValueError):
         sa.lognorm_process(paths=P, sigma=np.ones(P + 1))(tt)
+    with pytest.raises(ValueError):
+        sa.lognorm_process(paths=P, sigma=lambda t: np.ones(P + 1))(tt)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_parameters_varying_across_paths_and_in_time(gpu, dtype):
+    """Callables returning per-path values (and per-path constants beside
+    callables): the kernels read each step's per-path blocks, evaluated at
+    the step start as sdepy does (integration.py:1218-1221).  Lognorm with
+    sigma(t) per path and mu(t) shared; OU with per-path k and a shared
+    theta(t), correlated; Heston with a per-path xi(t), forward and with a
+    backward pass (i0 = 1)."""
+    print('per-path and time-dependent', np.dtype(dtype).name, flush=True)
+    tol = RTOL64 if dtype == np.float64 else 5e-5
+    P, tt = 2000, np.linspace(0., 1., 11)
+    g = np.random.default_rng(30)
+    base = g.uniform(.1, .4, P)
+    sig = lambda t: base * (1. + .5 * np.sin(3 * t))  # noqa: E731
+    mu = lambda t: .05 * (1. + t)  # noqa: E731
+    p = sa.lognorm_process(paths=P, dtype=dtype, steps=41, x0=1., mu=mu, sigma=sig,
+                           rng=np.random.default_rng(31))
+    run = p.prepare(tt)
+    assert run.passes[0].probs[0].param_paths_step == 2 * P
+    x = run.run()
+    ref = ora.integrate('lognorm', tt, paths=P, steps=41,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, ())},
+                        x0=1., mu=mu, sigma=sig)
+    assert _rel(x, ref) < tol
+    k = g.uniform(.5, 3., (3, P))
+    theta = lambda t: np.array([[.1], [-.2], [.3]]) * (1. + t)  # noqa: E731
+    C = _corr(3, 32)
+    p = sa.ornstein_uhlenbeck_process(paths=P, vshape=3, dtype=dtype, steps=41, x0=.2, k=k,
+                                      theta=theta, sigma=.3, corr=C,
+                                      rng=np.random.default_rng(33))
+    x = p(tt)
+    ref = ora.integrate('ou', tt, paths=P, vshape=(3,), steps=41,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (3,),
+                                                        corr=C)},
+                        x0=.2, k=k, theta=theta, sigma=.3)
+    assert np.max(np.abs(x - ref)) < (1e-12 if dtype == np.float64 else 3e-5)
+    if dtype == np.float32:
+        return
+    xib = g.uniform(.2, .6, P)
+    xi = lambda t: xib * (1. + t)  # noqa: E731
+
+    class W(ora.PhiloxWiener):
+        passes = 0
+
+        def begin_pass(self, tag):
+            super().begin_pass(tag)
+            self.epoch, W.passes = W.passes, W.passes + 1
+    for i0 in (0, 1):
+        W.passes = 0
+        info = {}
+        h = sa.full_heston_process(paths=P, steps=31, i0=i0, rho=-.6, x0=100., mu=.02,
+                                   y0=.04, theta=.04, k=2., xi=xi, info=info,
+                                   rng=np.random.default_rng(34))
+        hx, hy = h((0., .5, 1.))
+        ref_info = {}
+        rx, ry = ora.integrate('full_heston', (0., .5, 1.), paths=P, steps=31, i0=i0,
+                               info=ref_info,
+                               sources={'dw': W(h.sources['dw'].key, 0, P, (2,), rho=-.6)},
+                               x0=100., mu=.02, y0=.04, theta=.04, k=2., xi=xi)
+        assert _rel(hx, rx) < 1e-10 and np.max(np.abs(hy - ry)) < 1e-12, i0
+        assert np.array_equal(info['negative_y_count'], ref_info['negative_y_count'])
