@@ -207,6 +207,15 @@ typedef struct sdegpu_euler_problem {
                                  param_paths is then [nsteps][NB][nvars_total][paths],
                                  step s's parameters evaluated at its start (parameters
                                  that vary across paths and in time) */
+  const double* lam_paths;    /* optional device [nvars_total][paths] FP64, JUMPDIFF: Poisson
+                                 intensities that vary across paths (poisson_source's lam
+                                 broadcast to vshape + (paths,), infrastructure.py:1617-1633);
+                                 pois then holds (|ds|, 0) per step and component and the
+                                 kernel takes lam |ds| and exp(-lam |ds|) in FP64.  Not with
+                                 antithetic jumps (anti_j). */
+  int64_t lam_paths_step;     /* 0, or the elements between consecutive steps' intensities:
+                                 lam_paths is then [nsteps][nvars_total][paths], lam at each
+                                 step's midpoint */
 } sdegpu_euler_problem;
 
 /*
@@ -364,6 +373,9 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  *   double_exp_rv, :1653-1796).  law_params [nslots][nvars][3] dtype = (p0, p1,
  *   p2); out [nslots][nvars][paths] dtype; dn_out optional int64 like
  *   poisson_fill.
+ * sdegpu_poisson_fill_paths / sdegpu_cpoisson_fill_paths: the same with
+ *   intensities that vary across paths, lam_paths [nslots][nvars][paths] FP64;
+ *   pois then holds (|dt|, 0) per slot and component.
  */
 int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
                        int32_t nvars, int64_t paths, int64_t nslots, const void* coef,
@@ -376,6 +388,17 @@ int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                          uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
                          const double* pois, const int32_t* sign, int32_t jump_law,
                          const void* law_params, void* out, long long* dn_out, void* stream);
+/* poisson_source.__call__ / cpoisson_source.__call__ (infrastructure.py:1617-1633,
+ * 2017-2040) with lam broadcast to vshape + (paths,) varying across paths. */
+int sdegpu_poisson_fill_paths(const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
+                              int32_t nvars, int64_t paths, int64_t nslots, const double* pois,
+                              const double* lam_paths, const int32_t* sign, long long* out,
+                              void* stream);
+int sdegpu_cpoisson_fill_paths(int32_t dtype, const uint32_t key[2], uint32_t epoch,
+                               uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
+                               const double* pois, const double* lam_paths, const int32_t* sign,
+                               int32_t jump_law, const void* law_params, void* out,
+                               long long* dn_out, void* stream);
 
 /*
  * Pathwise Greeks by reverse-mode differentiation of the Euler scheme.

@@ -224,14 +224,20 @@ class PhiloxCPoisson(_PhiloxSource):
         self.slot += 1
         P = self.path_index.size
         lam = self.lam(t + dt / 2) if callable(self.lam) else self.lam
-        lam = np.broadcast_to(np.asarray(lam, float).reshape(-1)
-                              if np.ndim(lam) else lam, (M,))
+        # lam broadcasts against vshape + (paths,) (infrastructure.py:1631):
+        # one value per component, or per component and path
+        lam = np.asarray(lam, float)
+        try:
+            lam = np.broadcast_to(lam, self.vshape + (1,)).reshape(M, 1)
+        except ValueError:
+            lam = np.broadcast_to(lam, self.vshape + (self.paths,)).reshape(
+                M, self.paths)[:, self.path_index]
         lamdt = lam * abs(dt)
         p0 = np.exp(-lamdt)
         w = philox.slot_words(self.key, self.epoch,
                               self.pass_tag | philox.STREAM_POISSON,
                               s, M, self.path_index)              # (M, P)
-        n = philox.poisson_inv(philox.u01(w), lamdt[:, None], p0[:, None])
+        n = philox.poisson_inv(philox.u01(w), lamdt, p0)
         def per_comp(z):
             z = z(t + dt / 2) if callable(z) else z
             return np.broadcast_to(np.asarray(z, float).reshape(-1)

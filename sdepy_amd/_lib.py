@@ -58,6 +58,7 @@ class EulerProblem(ctypes.Structure):
         ('info_row_slots', c_int32), ('jump_law', c_int32),
         ('row_moments', c_void_p), ('row_moments_rows', c_int32),
         ('param_paths', c_void_p), ('param_paths_step', c_int64),
+        ('lam_paths', c_void_p), ('lam_paths_step', c_int64),
     ]
 
 
@@ -109,6 +110,12 @@ SIGNATURES = [
     ('sdegpu_cpoisson_fill', ctypes.c_int,
      [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
       c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ('sdegpu_poisson_fill_paths', ctypes.c_int,
+     [_U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p]),
+    ('sdegpu_cpoisson_fill_paths', ctypes.c_int,
+     [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
+      c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     ('sdegpu_philox_host', ctypes.c_int,
      [ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
       ctypes.POINTER(c_uint32)]),

@@ -129,6 +129,9 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EUNSUPPORTED,
                 "a launch of %d Heston components steps one group of width %d, not %d", m, m / 2,
                 hd);
+  if (pr->lam_paths && (pr->lam_paths_step < 0 || pr->anti_j != 0 || pr->replay_dj))
+    return fail(SDEGPU_EINVAL,
+                "lam_paths: negative step, or with antithetic or recorded jumps");
   if (pr->param_paths_step < 0)
     return fail(SDEGPU_EINVAL, "negative param_paths_step %lld", (long long)pr->param_paths_step);
   if (pr->info_row_slots < 0 || pr->info_row_slots > 1024)
@@ -166,6 +169,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.row_mom = pr->row_moments;
   a.ppar = pr->param_paths;
   a.ppar_step = pr->param_paths ? pr->param_paths_step : 0;
+  a.lamp = jump ? pr->lam_paths : nullptr;
+  a.lamp_step = a.lamp ? pr->lam_paths_step : 0;
   a.mom_rows = (int64_t)pr->row_moments_rows * out_stride;
   a.nvars = m;
   for (int v = 0; v < SDEGPU_MAX_COUPLED; ++v) a.x0[v] = pr->x0[v];
@@ -196,8 +201,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
   a.ej = pr->epoch_j;
 
   // no recorded increments and no antithetic halves: the plain instance
-  const bool plain =
-      !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 && !grouped;
+  const bool plain = !pr->replay_dw && !pr->replay_dj && pr->anti_w == 0 && pr->anti_j == 0 &&
+                     !grouped && !(jump && pr->lam_paths);
   const bool defer = jump && pr->jump_defer != 0 && !wide;
   const bool mom = pr->row_moments != nullptr;
   const bool pp = pr->param_paths != nullptr;

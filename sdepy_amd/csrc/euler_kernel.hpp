@@ -59,6 +59,9 @@ struct EulerArgs {
   const void* ppar;    // per-path parameters (PP instances): [PathParams<MODEL>::NB][..][paths]
   int32_t nvars;       // components this launch steps (the run-time width of k_euler_wide)
   int64_t ppar_step;   // elements between steps' blocks of ppar (0: the same every step)
+  const double* lamp;  // per-path Poisson intensities [..][m_total][paths] (general
+                       // instances), with pois holding |ds|; null otherwise
+  int64_t lamp_step;   // elements between steps' intensities (0: the same every step)
 };
 
 // Per-path parameters (the PP instances of k_euler): the model's parameter
@@ -619,8 +622,13 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
           const bool stored = store_row[s] >= 0;
 #pragma unroll
           for (int v = 0; v < M; ++v) {
-            const int n =
-                poisson_inv(u01_f64(upois[k * M + v]), pq[2 * v], pq[2 * v + 1], kPoissonCap);
+            double lamds = pq[2 * v], p0 = pq[2 * v + 1];
+            if (GEN && a.lamp) {
+              // per-path intensity: pois holds |ds|
+              lamds = a.lamp[(int64_t)s * a.lamp_step + (c0 + v) * P + gid] * lamds;
+              p0 = exp(-lamds);
+            }
+            const int n = poisson_inv(u01_f64(upois[k * M + v]), lamds, p0, kPoissonCap);
             T acc = (T)0;
             if (n > 0) {
               if (kDefer && !stored && njev < kJumpEvents && n < 256 && s < (1 << 20)) {

@@ -280,7 +280,13 @@ __global__ __launch_bounds__(kBlock) void k_euler_wide(const EulerArgs a,
 #pragma unroll
           for (int v = 0; v < W; ++v) {
             if (v < m) {
-              const int nj = poisson_inv(u01_f64(u[v]), pq[2 * v], pq[2 * v + 1], kPoissonCap);
+              double lamds = pq[2 * v], p0 = pq[2 * v + 1];
+              if (a.lamp) {
+                // per-path intensity: pois holds |ds|
+                lamds = a.lamp[(int64_t)s * a.lamp_step + (c0 + v) * P + gid] * lamds;
+                p0 = exp(-lamds);
+              }
+              const int nj = poisson_inv(u01_f64(u[v]), lamds, p0, kPoissonCap);
               T acc = (T)0;
               if (nj > 0) {
                 const T p2 = a.jump_law == SDEGPU_JUMP_NORMAL ? (T)0 : p[4 * m + v];

@@ -91,7 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
                                                          int64_t nslots, const double* __restrict__ pois,
                                                          const int32_t* __restrict__ sign, int law,
                                                          const T* __restrict__ lp, T* __restrict__ out,
-                                                         long long* __restrict__ dn_out) {
+                                                         long long* __restrict__ dn_out,
+                                                         const double* __restrict__ lamp) {
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (gid >= paths) return;
   const uint32_t path = (uint32_t)gid;
@@ -105,7 +106,13 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
       const int64_t g = g0 + l;
       if (g >= total) break;
       const int64_t s = g / nvars;
-      const int n = poisson_inv(u01_f64(bits[l]), pois[2 * g], pois[2 * g + 1], 4096);
+      // with per-path intensities pois holds |dt|: lam |dt| and its exp per path
+      double lamdt = pois[2 * g], p0 = pois[2 * g + 1];
+      if (lamp) {
+        lamdt = lamp[g * paths + gid] * lamdt;
+        p0 = exp(-lamdt);
+      }
+      const int n = poisson_inv(u01_f64(bits[l]), lamdt, p0, 4096);
       const int sg = sign[s];
       if (dn_out) dn_out[g * paths + gid] = (long long)(sg * n);
       if (out) {
@@ -241,7 +248,7 @@ extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t
 static int poisson_common(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t tag,
                           int32_t nvars, int64_t paths, int64_t nslots, const double* pois,
                           const int32_t* sign, int law, const void* lp, void* out,
-                          long long* dn_out, void* stream) {
+                          long long* dn_out, const double* lamp, void* stream) {
   if (!key || nvars < 1 || paths < 1 || paths > kMaxPaths || nslots < 0)
     return fail(SDEGPU_EINVAL, "poisson_fill: nvars=%d paths=%lld slots=%lld", nvars,
                 (long long)paths, (long long)nslots);
@@ -253,10 +260,10 @@ static int poisson_common(int32_t dtype, const uint32_t key[2], uint32_t epoch, 
   const dim3 grid(blocks_for(paths)), block(kBlock);
   if (dtype == SDEGPU_F32)
     hipLaunchKernelGGL(k_poisson_fill<float>, grid, block, 0, st, key[0], key[1], epoch, t, nvars,
-                       paths, nslots, pois, sign, law, (const float*)lp, (float*)out, dn_out);
+                       paths, nslots, pois, sign, law, (const float*)lp, (float*)out, dn_out, lamp);
   else if (dtype == SDEGPU_F64)
     hipLaunchKernelGGL(k_poisson_fill<double>, grid, block, 0, st, key[0], key[1], epoch, t, nvars,
-                       paths, nslots, pois, sign, law, (const double*)lp, (double*)out, dn_out);
+                       paths, nslots, pois, sign, law, (const double*)lp, (double*)out, dn_out, lamp);
   else
     return fail(SDEGPU_EUNSUPPORTED, "poisson_fill: dtype code %d", dtype);
   return launch_status("k_poisson_fill");
@@ -267,7 +274,17 @@ extern "C" int sdegpu_poisson_fill(const uint32_t key[2], uint32_t epoch, uint32
                                    const int32_t* sign, long long* out, void* stream) {
   if (!out) return fail(SDEGPU_EINVAL, "poisson_fill: null output");
   return poisson_common(SDEGPU_F64, key, epoch, stream_tag, nvars, paths, nslots, pois, sign,
-                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, stream);
+                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, nullptr, stream);
+}
+
+extern "C" int sdegpu_poisson_fill_paths(const uint32_t key[2], uint32_t epoch,
+                                         uint32_t stream_tag, int32_t nvars, int64_t paths,
+                                         int64_t nslots, const double* pois,
+                                         const double* lam_paths, const int32_t* sign,
+                                         long long* out, void* stream) {
+  if (!out || !lam_paths) return fail(SDEGPU_EINVAL, "poisson_fill_paths: null pointer");
+  return poisson_common(SDEGPU_F64, key, epoch, stream_tag, nvars, paths, nslots, pois, sign,
+                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, lam_paths, stream);
 }
 
 extern "C" int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
@@ -279,5 +296,19 @@ extern "C" int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32
   if (jump_law < SDEGPU_JUMP_NORMAL || jump_law > SDEGPU_JUMP_DOUBLE_EXP)
     return fail(SDEGPU_EUNSUPPORTED, "cpoisson_fill: unknown jump law %d", jump_law);
   return poisson_common(dtype, key, epoch, stream_tag, nvars, paths, nslots, pois, sign, jump_law,
-                        law_params, out, dn_out, stream);
+                        law_params, out, dn_out, nullptr, stream);
+}
+
+extern "C" int sdegpu_cpoisson_fill_paths(int32_t dtype, const uint32_t key[2], uint32_t epoch,
+                                          uint32_t stream_tag, int32_t nvars, int64_t paths,
+                                          int64_t nslots, const double* pois,
+                                          const double* lam_paths, const int32_t* sign,
+                                          int32_t jump_law, const void* law_params, void* out,
+                                          long long* dn_out, void* stream) {
+  if (!out || !law_params || !lam_paths)
+    return fail(SDEGPU_EINVAL, "cpoisson_fill_paths: null pointer");
+  if (jump_law < SDEGPU_JUMP_NORMAL || jump_law > SDEGPU_JUMP_DOUBLE_EXP)
+    return fail(SDEGPU_EUNSUPPORTED, "cpoisson_fill_paths: unknown jump law %d", jump_law);
+  return poisson_common(dtype, key, epoch, stream_tag, nvars, paths, nslots, pois, sign, jump_law,
+                        law_params, out, dn_out, lam_paths, stream);
 }

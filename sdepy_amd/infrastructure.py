@@ -288,6 +288,54 @@ class poisson_source(_philox_source):
                     'poisson source shape = vshape + (paths,) = {}'
                     .format(np.shape(lam), self.vshape + (paths,)))
 
+    def _lam_at(self, t):
+        """lam at time t broadcast to vshape + (paths,)."""
+        lam = self.lam(t) if callable(self.lam) else self.lam
+        a = np.asarray(lam, dtype=float)
+        try:
+            return np.broadcast_to(a, self.vshape + (self.paths,))
+        except ValueError:
+            raise ValueError(
+                'cannot broadcast lambda parameter shaped {} to requested '
+                'poisson source shape = vshape + (paths,) = {}'
+                .format(a.shape, self.vshape + (self.paths,)))
+
+    def per_path(self, tt, dtt):
+        """Whether lam varies across paths (a callable: at the first slot's
+        midpoint)."""
+        if not len(tt):
+            return False
+        lam = self.lam(tt[0] + dtt[0] / 2) if callable(self.lam) else self.lam
+        try:
+            np.broadcast_to(np.asarray(lam, dtype=float), self.vshape + (1,))
+        except ValueError:
+            return True
+        return False
+
+    def lam_path_tables(self, tt, dtt):
+        """Intensities that vary across paths: the slots' (|dt|, 0) table
+        [slots][M][2] and lam [1][M][paths] (not callable) or
+        [slots][M][paths] (at each slot's midpoint), FP64.  The kernels take
+        lam |dt| and exp(-lam |dt|) per path in FP64."""
+        M = int(np.prod(self.vshape, dtype=int))
+        P = self.paths
+        tab = np.zeros((tt.size, M, 2))
+        tab[:, :, 0] = np.abs(dtt)[:, None]
+        if callable(self.lam):
+            lam = np.stack([self._lam_at(tt[i] + dtt[i] / 2).reshape(M, P)
+                            for i in range(tt.size)]) if tt.size else np.zeros((0, M, P))
+            big = (lam * np.abs(dtt)[:, None, None] > 700).any() if tt.size else False
+        else:
+            lam = np.ascontiguousarray(self._lam_at(0.)).reshape(1, M, P)
+            big = tt.size and (lam * np.abs(dtt).max() > 700).any()
+        if (lam < 0).any():
+            raise ValueError('negative Poisson intensity')
+        if big:
+            raise NotImplementedError(
+                'Poisson means above 700 per step are not supported by '
+                'the GPU sampler')
+        return tab, np.ascontiguousarray(lam, dtype=float)
+
     def lam_table(self, tt, dtt):
         """[slots][M][2] = (lam |dt|, exp(-lam |dt|)) in FP64."""
         M = int(np.prod(self.vshape, dtype=int))
@@ -312,14 +360,25 @@ class poisson_source(_philox_source):
         tshape = dt.shape
         M = int(np.prod(self.vshape, dtype=int))
         n = int(np.prod(tshape, dtype=int))
-        pois = self.lam_table(t.reshape(-1), dt.reshape(-1))
-        sign = np.sign(dt.reshape(-1)).astype(np.int32)
+        tf, dtf = t.reshape(-1), dt.reshape(-1)
+        sign = np.sign(dtf).astype(np.int32)
         _lib.require_gpu()
-        dpois, dsign = _lib.to_device(pois), _lib.to_device(sign)
+        dsign = _lib.to_device(sign)
         out = _lib.empty((n, M, self.paths), np.int64)
-        _lib.check(_lib.lib().sdegpu_poisson_fill(
-            _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
-            _lib.ptr(dpois), _lib.ptr(dsign), _lib.ptr(out), _lib.stream()))
+        if self.per_path(tf, dtf):
+            pois, lam = self.lam_path_tables(tf, dtf)
+            dpois = _lib.to_device(pois)
+            dlam = _lib.to_device(np.ascontiguousarray(
+                np.broadcast_to(lam, (n, M, self.paths))))
+            _lib.check(_lib.lib().sdegpu_poisson_fill_paths(
+                _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
+                _lib.ptr(dpois), _lib.ptr(dlam), _lib.ptr(dsign), _lib.ptr(out),
+                _lib.stream()))
+        else:
+            dpois = _lib.to_device(self.lam_table(tf, dtf))
+            _lib.check(_lib.lib().sdegpu_poisson_fill(
+                _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
+                _lib.ptr(dpois), _lib.ptr(dsign), _lib.ptr(out), _lib.stream()))
         res = _lib.to_host(out).astype(self.dtype, copy=False)
         return res.reshape(tshape + self.vshape + (self.paths,))
 
@@ -689,7 +748,11 @@ class cpoisson_source(_philox_source):
         M = int(np.prod(self.vshape, dtype=int))
         n = tf.size
         shape = self.vshape + (self.paths,)
-        pois = self.dn.lam_table(tf, dtf)
+        per_path = self.dn.per_path(tf, dtf)
+        if per_path:
+            pois, lam = self.dn.lam_path_tables(tf, dtf)
+        else:
+            pois = self.dn.lam_table(tf, dtf)
         sign = np.sign(dtf).astype(np.int32)
         _lib.require_gpu()
         dpois, dsign = _lib.to_device(pois), _lib.to_device(sign)
@@ -698,10 +761,18 @@ class cpoisson_source(_philox_source):
         dn = _lib.empty((n, M, self.paths), np.int64)
         epoch = self.next_epoch()
         self.dn.epoch = self.epoch
-        _lib.check(_lib.lib().sdegpu_cpoisson_fill(
-            code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
-            _lib.ptr(dpois), _lib.ptr(dsign), law, _lib.ptr(dlaw),
-            _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
+        if per_path:
+            dlam = _lib.to_device(np.ascontiguousarray(
+                np.broadcast_to(lam, (n, M, self.paths))))
+            _lib.check(_lib.lib().sdegpu_cpoisson_fill_paths(
+                code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
+                _lib.ptr(dpois), _lib.ptr(dlam), _lib.ptr(dsign), law, _lib.ptr(dlaw),
+                _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
+        else:
+            _lib.check(_lib.lib().sdegpu_cpoisson_fill(
+                code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
+                _lib.ptr(dpois), _lib.ptr(dsign), law, _lib.ptr(dlaw),
+                _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
         self.dn_value = _lib.to_host(dn).astype(self.ptype).reshape(tshape + shape)
         self.y_value = None
         return _lib.to_host(out).astype(odt, copy=False).reshape(tshape + shape)

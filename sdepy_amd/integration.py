@@ -466,7 +466,20 @@ class _gpu_sde:
                 prob.key_j[0], prob.key_j[1] = gdj.key
                 prob.epoch_j = gdj.next_epoch()
                 gdj.dn.epoch = gdj.epoch
-                pois = gdj.dn.lam_table(t0s, ds)
+                if gdj.dn.per_path(t0s, ds):
+                    # intensities that vary across paths: (|ds|, 0) rows
+                    # and each path's lam, taken to lam |ds| on the device
+                    if gdj is not dj:
+                        raise NotImplementedError(
+                            'antithetic jumps with intensities that vary '
+                            'across paths')
+                    pois, lam = gdj.dn.lam_path_tables(t0s, ds)
+                    d_lam = _lib.to_device(lam)
+                    keep.append(d_lam)
+                    prob.lam_paths = d_lam.data_ptr()
+                    prob.lam_paths_step = lam[0].size if lam.shape[0] > 1 else 0
+                else:
+                    pois = gdj.dn.lam_table(t0s, ds)
                 prob.jump_law = laws[0]
             else:
                 host_sources['dj'] = dj

@@ -286,3 +286,91 @@ def test_parameters_varying_across_paths_and_in_time(gpu, dtype):
                                x0=100., mu=.02, y0=.04, theta=.04, k=2., xi=xi)
         assert _rel(hx, rx) < 1e-10 and np.max(np.abs(hy - ry)) < 1e-12, i0
         assert np.array_equal(info['negative_y_count'], ref_info['negative_y_count'])
+
+
+class _PassW(ora.PhiloxWiener):
+    passes = 0
+
+    def begin_pass(self, tag):
+        super().begin_pass(tag)
+        self.epoch, type(self).passes = type(self).passes, type(self).passes + 1
+
+
+class _PassJ(ora.PhiloxCPoisson):
+    passes = 0
+
+    def begin_pass(self, tag):
+        super().begin_pass(tag)
+        self.epoch, type(self).passes = type(self).passes, type(self).passes + 1
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_jump_intensity_per_path(gpu, dtype):
+    """Poisson intensities broadcast against vshape + (paths,): Merton with
+    a lam per path, vshape (9,) (two grouped launches) with lam per
+    component and path, Kou with a per-path lam(t), and a backward pass: the
+    jump counts equal the oracle's exactly (lam |ds| and exp(-lam |ds|) in
+    FP64 on both sides), the paths to 1e-12 in FP64."""
+    print('per-path intensity', np.dtype(dtype).name, flush=True)
+    tol = RTOL64 if dtype == np.float64 else 5e-5
+    P, n = 1500, 41
+    g = np.random.default_rng(40)
+    tt = np.linspace(0., 1., 6)
+    cases = [
+        (sa.merton_jumpdiff_process, (), g.uniform(1., 8., P), dict(a=-.1, b=.15),
+         dict(a=-.1, b=.15), 0),
+        (sa.merton_jumpdiff_process, (9,), g.uniform(.5, 4., (9, P)), dict(a=-.1, b=.15),
+         dict(a=-.1, b=.15), 0),
+        (sa.kou_jumpdiff_process, (), None, dict(a=.2, b=.3, pa=.6),
+         dict(law='double_exp', a=.2, b=.3, pa=.6), 2),
+    ]
+    base = g.uniform(2., 6., P)
+    for cls, vs, lam, kw, law, i0 in cases:
+        if lam is None:
+            lam = lambda t: base * (1. + t)  # noqa: E731
+        info = {}
+        m = cls(paths=P, vshape=vs, dtype=dtype, steps=n, i0=i0, info=info, x0=1.,
+                mu=.03, sigma=.2, lam=lam, rng=np.random.default_rng(41), **kw)
+        run = m.prepare(tt)
+        assert all(q.lam_paths for q in run.passes[0].probs)
+        x = run.run()
+        _PassW.passes = _PassJ.passes = 0
+        ref_info = {}
+        ref = ora.integrate('jumpdiff', tt, paths=P, vshape=vs, steps=n, i0=i0,
+                            info=ref_info,
+                            sources={'dw': _PassW(m.sources['dw'].key, 0, P, vs),
+                                     'dj': _PassJ(m.sources['dj'].key, 0, P, vs, lam=lam,
+                                                  **law)},
+                            x0=1., mu=.03, sigma=.2)
+        assert np.abs(ref_info['jump_count']).sum() > 1000
+        assert np.array_equal(info['jump_count'], ref_info['jump_count']), (cls, vs)
+        assert np.allclose(info['jump_rate'], ref_info['jump_rate'], rtol=1e-12)
+        assert _rel(x, ref) < tol, (cls, vs)
+
+
+def test_sources_with_per_path_intensity(gpu):
+    """poisson_source and cpoisson_source called on their own with lam per
+    path draw what the oracle's source draws; antithetic jumps with such
+    intensities are refused."""
+    P, vs = 800, (2,)
+    g = np.random.default_rng(42)
+    lam = g.uniform(.5, 6., (2, P))
+    src = sa.cpoisson_source(paths=P, vshape=vs, dtype=np.float64, lam=lam,
+                             y=sa.norm_rv(a=.1, b=.3), rng=np.random.default_rng(43))
+    ref = ora.PhiloxCPoisson(src.key, 0, P, vs, lam=lam, a=.1, b=.3)
+    got = src(np.array([0.]), np.array([.4]))[0]
+    want = ref(0., .4)
+    assert np.abs(ref.dn_value).sum() > 500
+    assert np.array_equal(np.asarray(src.dn_value).reshape(ref.dn_value.shape), ref.dn_value)
+    np.testing.assert_allclose(got, want, rtol=RTOL64, atol=0)
+    dn = sa.poisson_source(paths=P, vshape=vs, lam=lam, rng=np.random.default_rng(44))
+    counts = dn(.2, .5)
+    twin = ora.PhiloxCPoisson(dn.key, 0, P, vs, lam=lam)
+    twin(.2, .5)
+    assert np.array_equal(np.asarray(counts).reshape(twin.dn_value.shape), twin.dn_value)
+    with pytest.raises(ValueError):       # as sdepy: the half takes P/2 paths
+        sa.even_cpoisson_source(paths=P, lam=g.uniform(1., 2., P))
+    h = g.uniform(1., 2., P // 2)
+    with pytest.raises(NotImplementedError):
+        sa.merton_jumpdiff_process(paths=P, dj=sa.even_cpoisson_source(
+            paths=P, lam=lambda t: h))((0., 1.))

@@ -416,6 +416,32 @@ def test_backward_pass_merton_b0_jump_counts(i0):
     assert info['jump_rate'][-1] == info['jump_rate'][-2]
 
 
+def test_per_path_intensity_keeps_each_paths_draws():
+    """lam broadcast against vshape + (paths,) (infrastructure.py:1631): with
+    intensities that vary across paths each path draws from its own words
+    with its own lam, so the paths whose lam is l1 get exactly the counts and
+    sizes of a constant-l1 source on the same key, and likewise for l2 (a
+    mix-up of the paths axis, or lam taken per component, fails this); and
+    the counts' means follow lam |dt|."""
+    P, vs = 3000, (2,)
+    half = np.arange(P) < P // 2
+    lam = np.where(half, 2., 7.) * np.array([[1.], [.5]])        # (2, P)
+    mixed = ora.PhiloxCPoisson((3, 4), 0, P, vs, lam=lam, a=.1, b=.2)
+    c1 = ora.PhiloxCPoisson((3, 4), 0, P, vs, lam=np.array([[2.], [1.]]), a=.1, b=.2)
+    c2 = ora.PhiloxCPoisson((3, 4), 0, P, vs, lam=np.array([[7.], [3.5]]), a=.1, b=.2)
+    for t, dt in ((0., .3), (.3, .2)):
+        zm, z1, z2 = mixed(t, dt), c1(t, dt), c2(t, dt)
+        assert np.array_equal(mixed.dn_value[:, half], c1.dn_value[:, half])
+        assert np.array_equal(mixed.dn_value[:, ~half], c2.dn_value[:, ~half])
+        assert np.array_equal(zm[:, half], z1[:, half])
+        assert np.array_equal(zm[:, ~half], z2[:, ~half])
+        for v in range(2):
+            for sel in (half, ~half):
+                n = mixed.dn_value[v, sel]
+                m = lam[v, sel][0] * dt
+                assert abs(n.mean() - m) < 4 * np.sqrt(m / sel.sum()), (v, t)
+
+
 # ---------------------------------------------------------------------------
 # SDEs packing with leading vshape axes (integration.py:1735-1783)
 # ---------------------------------------------------------------------------
