@@ -231,8 +231,24 @@ def _np_dtype(tdt):
     return _NP_OF_TORCH.get(tdt)
 
 
+def _pinned_owner(a):
+    """The pinned torch tensor whose memory holds the array ``a`` (the result
+    arrays of to_host are views of one), or None."""
+    torch = torch_mod()
+    b = a
+    while b is not None:
+        if isinstance(b, torch.Tensor):
+            return b if b.is_pinned() and b.is_contiguous() else None
+        b = getattr(b, 'base', None)
+    return None
+
+
 def to_device(a):
-    """Copy a NumPy array to the current GPU; returns the torch tensor."""
+    """Copy a NumPy array to the current GPU; returns the torch tensor.
+
+    An array in pinned memory (a view of a result that to_host returned) is
+    copied by one synchronous DMA straight from it; other large arrays go
+    through the pinned staging buffers."""
     torch = require_gpu()
     a = np.ascontiguousarray(a)
     if a.nbytes < _STAGE_MIN or np.dtype(a.dtype) not in _TORCH_DTYPES:
@@ -241,6 +257,13 @@ def to_device(a):
         return torch.from_numpy(a).to('cuda', non_blocking=False)
     out = empty(a.shape, a.dtype)
     dst = out.reshape(-1).view(torch.uint8)
+    owner = _pinned_owner(a)
+    if owner is not None:
+        off = a.__array_interface__['data'][0] - owner.data_ptr()
+        whole = owner.reshape(-1).view(torch.uint8)
+        if 0 <= off and off + a.nbytes <= whole.numel():
+            dst.copy_(whole[off:off + a.nbytes], non_blocking=False)
+            return out
     src = a.reshape(-1).view(np.uint8)
     n = src.size
     with _stage_lock:
