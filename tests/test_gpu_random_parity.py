@@ -1,7 +1,8 @@
 """Seeded random sweep of FP64 parity with the oracle (GPU tests).
 
 Each case draws a preset, a vshape (up to 12 working components, so some run
-in component groups), constant or time-dependent parameters, a correlation, an
+in component groups; the second sweep also coupled blocks of up to 24),
+constant, time-dependent or per-path parameters, a correlation, an
 output timeline, `steps` and `i0`, integrates it on the GPU and compares every
 stored value with the oracle's restatement of sdepy's loop driven by the same
 Philox keys (the epochs advance once per pass, the backward pass of i0 != 0
@@ -45,7 +46,14 @@ def _corr(k, g):
 
 
 def _param(g, shape, lo, hi, tdep):
-    """A per-component constant, or a function of t (time-dependent)."""
+    """A per-component constant, or a function of t (time-dependent); with
+    tdep 'path' or 'path_t' a value per component and path (constant, or
+    times a function of t)."""
+    if tdep in ('path', 'path_t'):
+        v = g.uniform(lo, hi, shape + (P,))
+        if tdep == 'path':
+            return v
+        return lambda t, v=v: v * (1. + .1 * np.sin(1.7 * t))
     v = g.uniform(lo, hi, shape + (1,))
     if not tdep:
         return v
@@ -57,13 +65,24 @@ def _case(seed):
     g = np.random.default_rng(1000 + seed)
     preset = ['wiener', 'lognorm', 'ou', 'hull_white', 'cir', 'full_heston',
               'heston', 'merton'][seed % 8]
+    wide = seed >= 100
     vshapes = [(), (2,), (3,), (2, 2), (11,), (2, 3)]
     if preset in ('full_heston', 'heston'):
         vshapes = [(), (2,), (2, 2), (3, 2)]
     if preset == 'hull_white':
         vshapes = [(), (2,), (5,)]
+    if wide:
+        # coupled blocks above 8 components beside small states
+        vshapes = [(), (3,), (10,), (2, 9), (12,)]
+        if preset in ('full_heston', 'heston'):
+            vshapes = [(), (2,), (5,), (2, 6)]
+        if preset == 'hull_white':
+            vshapes = [(), (3,)]
     vshape = vshapes[g.integers(len(vshapes))]
-    tdep = bool(g.integers(2))
+    if wide:
+        tdep = [False, True, 'path', 'path_t'][g.integers(4)]
+    else:
+        tdep = bool(g.integers(2))
     n_t = int(g.integers(2, 7))
     tt = np.sort(g.uniform(0., 2., n_t))
     tt = np.unique(np.round(tt, 3))
@@ -77,7 +96,7 @@ def _case(seed):
 
 def _run(seed, dtype=np.float64):
     g, preset, vshape, tdep, tt, steps, i0 = _case(seed)
-    print('case', seed, np.dtype(dtype).name, preset, vshape, 'tdep' if tdep else 'const',
+    print('case', seed, np.dtype(dtype).name, preset, vshape, tdep,
           tt, steps, i0, flush=True)
     vs = vshape
     last = vs[-1] if vs else 1
@@ -96,7 +115,7 @@ def _run(seed, dtype=np.float64):
                    k=_param(g, vs, .5, 3., tdep), sigma=_param(g, vs, .1, .5, tdep))
     elif preset == 'hull_white':
         cls = sa.hull_white_process
-        F = 2
+        F = 2 if seed < 100 else [2, 10][g.integers(2)]
         wshape = vs + (F,)
         prm = dict(x0=0., theta=0., k=_param(g, vs + (F,), .01, .3, tdep),
                    sigma=_param(g, vs + (F,), .005, .02, tdep))
@@ -120,7 +139,16 @@ def _run(seed, dtype=np.float64):
                    sigma=_param(g, vs, .1, .4, tdep))
         extra = dict(lam=float(g.uniform(.5, 6.)), a=float(g.uniform(-.2, .1)),
                      b=float(g.uniform(.05, .2)))
-    if corr is None and vs and 2 <= last <= 8 and preset not in ('hull_white',) \
+        if seed >= 100:
+            if g.integers(2):
+                # intensities per component and path
+                extra['lam'] = g.uniform(.5, 6., vs + (P,))
+            if g.integers(2):
+                cls = sa.kou_jumpdiff_process
+                extra.update(a=float(g.uniform(.1, .3)), b=float(g.uniform(.1, .3)),
+                             pa=float(g.uniform(.3, .7)))
+    max_corr = 8 if seed < 100 else 32
+    if corr is None and vs and 2 <= last <= max_corr and preset not in ('hull_white',) \
             and g.integers(2):
         corr = _corr(last, g)
     info, ref_info = {}, {}
@@ -136,8 +164,10 @@ def _run(seed, dtype=np.float64):
     sources = {'dw': dw}
     oparams = dict(prm)
     if preset == 'merton':
+        law = {} if cls is sa.merton_jumpdiff_process else \
+            dict(law='double_exp', pa=extra['pa'])
         sources['dj'] = _SeqCPoisson(p.sources['dj'].key, 0, P, wshape,
-                                     lam=extra['lam'], a=extra['a'], b=extra['b'])
+                                     lam=extra['lam'], a=extra['a'], b=extra['b'], **law)
     opreset = {'merton': 'jumpdiff'}.get(preset, preset)
     ref = ora.integrate(opreset, tt, paths=P, vshape=vshape, steps=steps, i0=i0,
                         sources=sources, **({'info': ref_info} if preset == 'merton' else {}),
@@ -174,3 +204,11 @@ def test_random_fp32_against_fp64_oracle(gpu, seed):
     """The same cases stepped in FP32 (the FP32 kernel instances) against the
     FP64 oracle on the same draws, within 2e-5 relative to the values' size."""
     _run(seed, np.float32)
+
+
+@pytest.mark.parametrize('seed', range(100, 132))
+def test_random_fp64_parity_wide_and_per_path(gpu, seed):
+    """A second sweep: states with coupled blocks of 9..24 components (the
+    wide kernel), parameters per path (constant or times a function of t),
+    Merton intensities per path and Kou's double exponential sizes."""
+    _run(seed)
