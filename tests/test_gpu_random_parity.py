@@ -212,3 +212,9 @@ def test_random_fp64_parity_wide_and_per_path(gpu, seed):
     wide kernel), parameters per path (constant or times a function of t),
     Merton intensities per path and Kou's double exponential sizes."""
     _run(seed)
+
+
+@pytest.mark.parametrize('seed', range(100, 132))
+def test_random_fp32_wide_and_per_path_against_fp64_oracle(gpu, seed):
+    """The second sweep stepped in FP32 against the FP64 oracle."""
+    _run(seed, np.float32)
