@@ -125,7 +125,12 @@ typedef struct sdegpu_euler_problem {
   int32_t nsteps;     /* Euler intervals in this pass (>= 0) */
   int32_t ncoef;      /* table row width */
   const void* coef;   /* device [nsteps][ncoef] */
-  const double* pois; /* device [nsteps][nvars][2] = (lam*|ds|, exp(-lam*|ds|)), JUMPDIFF only */
+  const double* pois; /* device [nsteps][nvars][2] = (lam*|ds|, exp(-lam*|ds|)), JUMPDIFF only;
+                         for lam*|ds| > 700 the second value is exp(-lam*|ds|/k),
+                         k = ceil(lam*|ds|/700): the count is then the sum of k counts
+                         of mean lam*|ds|/k, the first from the slot's word, part j >= 1
+                         from word (j-1)%4 of the Poisson block (slot, path,
+                         c2 | (1+(j-1)/4) << 8, epoch) */
   const int32_t* store_row; /* device [nsteps]: output row reached after step s, or -1 */
   int32_t row0;       /* output row of the initial state, or -1 */
   int32_t stream_tag; /* 0 for the forward pass, 16 for the backward pass (i0 != 0) */

@@ -233,11 +233,10 @@ class PhiloxCPoisson(_PhiloxSource):
             lam = np.broadcast_to(lam, self.vshape + (self.paths,)).reshape(
                 M, self.paths)[:, self.path_index]
         lamdt = lam * abs(dt)
-        p0 = np.exp(-lamdt)
-        w = philox.slot_words(self.key, self.epoch,
-                              self.pass_tag | philox.STREAM_POISSON,
-                              s, M, self.path_index)              # (M, P)
-        n = philox.poisson_inv(philox.u01(w), lamdt, p0)
+        c2p = self.pass_tag | philox.STREAM_POISSON
+        w = philox.slot_words(self.key, self.epoch, c2p, s, M, self.path_index)  # (M, P)
+        n = philox.poisson_draw(w, lamdt, self.key, self.epoch, c2p,
+                                s * M + np.arange(M)[:, None], self.path_index[None, :])
         def per_comp(z):
             z = z(t + dt / 2) if callable(z) else z
             return np.broadcast_to(np.asarray(z, float).reshape(-1)

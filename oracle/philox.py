@@ -155,6 +155,42 @@ def poisson_inv(u, lam, p0, cap=POISSON_CAP):
     return n
 
 
+# Poisson means above POISSON_SPLIT (poisson_draw in sdepy_amd/csrc/philox.hpp):
+# exp(-lam) underflows in FP64 from about 745, so such a count is the sum of
+# k = ceil(lam / POISSON_SPLIT) counts of mean lam / k; the first takes the
+# slot's own word, part j >= 1 word (j-1) % 4 of block (slot, path,
+# c2p | (1 + (j-1) // 4) << 8, epoch).
+POISSON_SPLIT = 700.
+
+
+def poisson_p0(lam):
+    """exp(-lam), or exp(-lam / k) for a mean split into k parts."""
+    lam = np.asarray(lam, float)
+    k = np.maximum(np.ceil(lam / POISSON_SPLIT), 1.)
+    return np.where(lam <= POISSON_SPLIT, np.exp(-lam), np.exp(-(lam / k)))
+
+
+def poisson_draw(words, lam, key, epoch, c2p, slots, paths, p0=None):
+    """Poisson counts of means lam from the slots' words (uint32), splitting
+    means above POISSON_SPLIT; slots and paths are the counters' c0 and c1
+    of each element (broadcast against words)."""
+    words, lam, slots, paths = np.broadcast_arrays(
+        np.asarray(words, np.uint32), np.asarray(lam, float),
+        np.asarray(slots, np.uint64), np.asarray(paths, np.uint64))
+    p0 = poisson_p0(lam) if p0 is None else np.broadcast_to(np.asarray(p0, float), lam.shape)
+    k = np.maximum(np.ceil(lam / POISSON_SPLIT), 1.)
+    part = np.where(lam <= POISSON_SPLIT, lam, lam / k)
+    n = poisson_inv(u01(words), part, p0)
+    for j in range(1, int(k.max()) if k.size else 1):
+        m = k > j
+        if not m.any():
+            break
+        w = philox4x32_10(slots[m].astype(np.uint32), paths[m].astype(np.uint32),
+                          c2p | ((1 + (j - 1) // 4) << 8), epoch, key[0], key[1])
+        n[m] += poisson_inv(u01(w[(j - 1) & 3]), part[m], p0[m])
+    return n
+
+
 # jump-size laws of the compound Poisson draws (SDEGPU_JUMP_* in sdegpu.h)
 JUMP_NORMAL = 0       # p0 + p1 z                  norm_rv(a, b) = (a, b, -)
 JUMP_UNIFORM = 1      # p0 + p1 u                  uniform_rv(a, b) = (a, b - a, -)

@@ -11,7 +11,8 @@
 //   c1 = path index
 //   c2 = stream id in the low byte (0 Wiener normals, 1 Poisson uniforms,
 //        2 jump-size normals; bit 4 marks the backward pass of an i0 != 0
-//        simulation) | sub-index << 8 (jump blocks)
+//        simulation) | sub-index << 8 (jump blocks; the extra parts of a
+//        Poisson mean split by poisson_draw)
 //   c3 = epoch: the number of earlier simulations drawn from the same source
 // key = the 64-bit seed drawn once from the caller's numpy rng when the
 // source is created (sdepy's `rng` argument, infrastructure.py:1334-1341).
@@ -317,6 +318,37 @@ __host__ __device__ __forceinline__ int poisson_inv(double u, double lam, double
     n += 1;
     p = p * (lam / (double)n);
     f = f + p;
+  }
+  return n;
+}
+
+// Poisson means above kPoissonSplit per slot: exp(-lam) would underflow (FP64
+// from about 745), so the count is the sum of k = ceil(lam / kPoissonSplit)
+// Poisson(lam / k) counts, each drawn by inversion.  The first takes the
+// slot's own word; part j >= 1 takes word (j-1) % 4 of the block (slot g,
+// path, c2p | (1 + (j-1) / 4) << 8, epoch), a sub-stream of the Poisson
+// stream that its slot-major blocks (sub 0) never reach.  p0 is
+// poisson_p0(lam): exp(-lam), or exp(-lam / k) when split (the host
+// computes it in double with the same operations).
+constexpr double kPoissonSplit = 700.0;
+
+__host__ __device__ __forceinline__ double poisson_p0(double lam) {
+  return lam <= kPoissonSplit ? exp(-lam) : exp(-(lam / ceil(lam / kPoissonSplit)));
+}
+
+__device__ __forceinline__ int poisson_draw(uint32_t word, double lam, double p0, uint32_t g,
+                                            uint32_t path, uint32_t c2p, uint32_t epoch,
+                                            uint32_t k0, uint32_t k1, int cap) {
+  if (lam <= kPoissonSplit) return poisson_inv(u01_f64(word), lam, p0, cap);
+  const int k = (int)ceil(lam / kPoissonSplit);
+  const double part = lam / (double)k;
+  int n = poisson_inv(u01_f64(word), part, p0, cap);
+  for (int j = 1; j < k; ++j) {
+    const ctr4 r = philox4x32_10(ctr4{g, path, c2p | ((uint32_t)(1 + (j - 1) / 4) << 8), epoch},
+                                 k0, k1);
+    const int q = (j - 1) & 3;
+    const uint32_t w = q == 0 ? r.x : q == 1 ? r.y : q == 2 ? r.z : r.w;
+    n += poisson_inv(u01_f64(w), part, p0, cap);
   }
   return n;
 }

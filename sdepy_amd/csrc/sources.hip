@@ -110,9 +110,9 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
       double lamdt = pois[2 * g], p0 = pois[2 * g + 1];
       if (lamp) {
         lamdt = lamp[g * paths + gid] * lamdt;
-        p0 = exp(-lamdt);
+        p0 = poisson_p0(lamdt);
       }
-      const int n = poisson_inv(u01_f64(bits[l]), lamdt, p0, 4096);
+      const int n = poisson_draw(bits[l], lamdt, p0, (uint32_t)g, path, c2p, epoch, k0, k1, 4096);
       const int sg = sign[s];
       if (dn_out) dn_out[g * paths + gid] = (long long)(sg * n);
       if (out) {

@@ -324,16 +324,10 @@ class poisson_source(_philox_source):
         if callable(self.lam):
             lam = np.stack([self._lam_at(tt[i] + dtt[i] / 2).reshape(M, P)
                             for i in range(tt.size)]) if tt.size else np.zeros((0, M, P))
-            big = (lam * np.abs(dtt)[:, None, None] > 700).any() if tt.size else False
         else:
             lam = np.ascontiguousarray(self._lam_at(0.)).reshape(1, M, P)
-            big = tt.size and (lam * np.abs(dtt).max() > 700).any()
         if (lam < 0).any():
             raise ValueError('negative Poisson intensity')
-        if big:
-            raise NotImplementedError(
-                'Poisson means above 700 per step are not supported by '
-                'the GPU sampler')
         return tab, np.ascontiguousarray(lam, dtype=float)
 
     def lam_table(self, tt, dtt):
@@ -346,12 +340,8 @@ class poisson_source(_philox_source):
             if (lam < 0).any():
                 raise ValueError('negative Poisson intensity')
             ld = lam * abs(dtt[i])
-            if (ld > 700).any():
-                raise NotImplementedError(
-                    'Poisson means above 700 per step are not supported by '
-                    'the GPU sampler')
             tab[i, :, 0] = ld
-            tab[i, :, 1] = np.exp(-ld)
+            tab[i, :, 1] = poisson_p0(ld)
         return tab
 
     def __call__(self, t, dt):
@@ -381,6 +371,20 @@ class poisson_source(_philox_source):
                 _lib.ptr(dpois), _lib.ptr(dsign), _lib.ptr(out), _lib.stream()))
         res = _lib.to_host(out).astype(self.dtype, copy=False)
         return res.reshape(tshape + self.vshape + (self.paths,))
+
+
+POISSON_SPLIT = 700.
+
+
+def poisson_p0(ld):
+    """The first pmf term of the kernels' Poisson inversion for means ld:
+    exp(-ld), or exp(-ld / k) for means above POISSON_SPLIT, which the
+    kernels draw as the sum of k = ceil(ld / POISSON_SPLIT) counts of mean
+    ld / k (exp(-ld) underflows from about 745).  The same FP64 operations
+    as the kernels' poisson_p0."""
+    ld = np.asarray(ld, dtype=float)
+    k = np.maximum(np.ceil(ld / POISSON_SPLIT), 1.)
+    return np.where(ld <= POISSON_SPLIT, np.exp(-ld), np.exp(-(ld / k)))
 
 
 def per_component(value, vshape, name):

@@ -374,3 +374,33 @@ def test_sources_with_per_path_intensity(gpu):
     with pytest.raises(NotImplementedError):
         sa.merton_jumpdiff_process(paths=P, dj=sa.even_cpoisson_source(
             paths=P, lam=lambda t: h))((0., 1.))
+
+
+def test_poisson_means_above_700_per_step(gpu):
+    """Intensities whose mean per step exceeds 700 (exp(-lam) would
+    underflow): each count is the sum of ceil(mean / 700) parts drawn from
+    the slot's word and its sub-stream; Merton (constant and per-path lam)
+    and a stand-alone poisson_source equal the oracle's counts exactly."""
+    print('poisson means above 700', flush=True)
+    P, n = 32, 5
+    tt = np.linspace(0., 1., 3)
+    for lam in (4000., np.random.default_rng(50).uniform(3000., 12000., P)):
+        info = {}
+        m = sa.merton_jumpdiff_process(paths=P, steps=n, info=info, x0=1., mu=.03, sigma=.2,
+                                       lam=lam, a=-1e-4, b=1e-3,
+                                       rng=np.random.default_rng(51))
+        x = m(tt)
+        ref_info = {}
+        ref = ora.integrate('jumpdiff', tt, paths=P, steps=n, info=ref_info,
+                            sources={'dw': ora.PhiloxWiener(m.sources['dw'].key, 0, P, ()),
+                                     'dj': ora.PhiloxCPoisson(m.sources['dj'].key, 0, P, (),
+                                                              lam=lam, a=-1e-4, b=1e-3)},
+                            x0=1., mu=.03, sigma=.2)
+        assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+        assert ref_info['jump_count'].mean() > 3000
+        assert _rel(x, ref) < 1e-11
+    dn = sa.poisson_source(paths=P, lam=4000., rng=np.random.default_rng(52))
+    counts = dn(0., .5)
+    twin = ora.PhiloxCPoisson(dn.key, 0, P, (), lam=4000.)
+    twin(0., .5)
+    assert np.array_equal(np.asarray(counts).reshape(twin.dn_value.shape), twin.dn_value)

@@ -115,6 +115,46 @@ def test_poisson_inversion_pmf(lam):
     assert abs(k.mean() - lam) < 5 * math.sqrt(lam / n)
 
 
+@pytest.mark.parametrize('lam', [650., 1000., 2500.])
+def test_poisson_draw_split_means_pmf(lam):
+    """Means above 700 are drawn as the sum of ceil(lam / 700) counts of
+    mean lam / k from the slot's word and its sub-stream: the sum is
+    Poisson(lam) (chi-square against scipy's pmf; mean and variance), and
+    below the split it is the plain inversion of the slot's word."""
+    n = 40_000
+    key, epoch, c2p = (11, 13), 2, philox.STREAM_POISSON
+    paths = np.arange(n)
+    w = philox.slot_words(key, epoch, c2p, 3, 1, paths)[0]
+    k = philox.poisson_draw(w, lam, key, epoch, c2p, 3, paths)
+    if lam <= philox.POISSON_SPLIT:
+        assert np.array_equal(k, philox.poisson_inv(philox.u01(w), lam, math.exp(-lam)))
+    lo, hi = int(lam - 5 * math.sqrt(lam)), int(lam + 5 * math.sqrt(lam))
+    edges = np.arange(lo, hi + 1, 8)
+    obs = np.histogram(k, bins=np.concatenate(([-1], edges, [10 ** 9])))[0]
+    cdf = scipy.stats.poisson.cdf(np.concatenate(([-1], edges, [10 ** 9])) - 1, lam)
+    exp_ = n * np.diff(cdf)
+    keep = exp_ > 5
+    chi2 = ((obs[keep] - exp_[keep]) ** 2 / exp_[keep]).sum()
+    assert scipy.stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4
+    assert abs(k.mean() - lam) < 5 * math.sqrt(lam / n)
+    assert abs(k.var() / lam - 1) < 0.05
+
+
+def test_poisson_draw_planted_split_mistakes_fail():
+    """A split that reuses the slot's word for every part (parts perfectly
+    correlated) or forgets one part fails the variance/mean pins above."""
+    n, lam = 40_000, 2100.
+    key, c2p = (11, 13), philox.STREAM_POISSON
+    paths = np.arange(n)
+    w = philox.slot_words(key, 0, c2p, 0, 1, paths)[0]
+    part = lam / 3
+    same = 3 * philox.poisson_inv(philox.u01(w), part, math.exp(-part))
+    assert not abs(same.var() / lam - 1) < 0.05
+    k = philox.poisson_draw(w, lam, key, 0, c2p, 0, paths)
+    one_missing = k - philox.poisson_inv(philox.u01(w), part, math.exp(-part))
+    assert not abs(one_missing.mean() - lam) < 5 * math.sqrt(lam / n)
+
+
 def test_poisson_inversion_breakpoints():
     lam = 1.3
     F = scipy.stats.poisson.cdf(np.arange(6), lam)
