@@ -65,7 +65,7 @@ extern "C" {
    Heston state of width D, the factors of a Hull-White value), which sdepy
    draws with one multivariate normal (infrastructure.py:1524-1553). */
 #define SDEGPU_MAX_VARS 8
-#define SDEGPU_MAX_COUPLED 32
+#define SDEGPU_MAX_COUPLED 64
 
 /* ---- compound Poisson jump-size laws (cpoisson_source's y, infrastructure.py:
    2017-2040, :1653-1796), three parameters (p0, p1, p2) per component ---- */
@@ -119,7 +119,8 @@ typedef struct sdegpu_euler_problem {
   uint32_t exp_mask;  /* bit o set: store exp(value o); must be the model's log components:
                          all nout for LOGNORM/JUMPDIFF (SDE.log, integration.py:1193-1197),
                          the log-x components for HESTON (:2441-2444; all nout bits when
-                         nout = M/2), none otherwise */
+                         nout = M/2), none otherwise; bits of values 0..31 (the model's
+                         rule decides beyond) */
   int32_t has_corr;   /* table rows carry a Cholesky factor */
   int64_t paths;      /* 1 .. 2^32-1 */
   int32_t nsteps;     /* Euler intervals in this pass (>= 0) */

@@ -25,7 +25,7 @@ MODEL_WIENER, MODEL_LOGNORM, MODEL_OU, MODEL_CIR, MODEL_HESTON, \
 OUT_IDENTITY, OUT_SUM, OUT_FIRST = range(3)
 JUMP_NORMAL, JUMP_UNIFORM, JUMP_DOUBLE_EXP = range(3)
 MAX_VARS = 8        # components of one k_euler launch, in any arrangement
-MAX_COUPLED = 32    # components of one coupled block (k_euler_wide)
+MAX_COUPLED = 64    # components of one coupled block (k_euler_wide)
 STREAM_BACKWARD = 16
 REDUCE_MIN, REDUCE_MAX, REDUCE_SUM, REDUCE_MEAN, REDUCE_VAR, REDUCE_STD = range(6)
 

@@ -35,9 +35,9 @@ static uint32_t expected_exp_mask(int model, int m, int nout, int d) {
   if (model == SDEGPU_MODEL_LOGNORM || model == SDEGPU_MODEL_JUMPDIFF)
     return nout >= 32 ? 0xFFFFFFFFu : ((1u << nout) - 1u);
   if (model != SDEGPU_MODEL_HESTON) return 0u;
-  if (nout == m / 2) return (1u << nout) - 1u;
+  if (nout == m / 2) return nout >= 32 ? 0xFFFFFFFFu : (1u << nout) - 1u;
   uint32_t mask = 0u;
-  for (int o = 0; o < m; ++o)
+  for (int o = 0; o < m && o < 32; ++o)
     if (o % (2 * d) < d) mask |= 1u << o;
   return mask;
 }

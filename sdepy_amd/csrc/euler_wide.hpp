@@ -1,12 +1,12 @@
 // Euler path kernel for one coupled block of more than 8 working components
-// (k_euler_wide): a dw correlated over 9..32 components, a Heston state of 5..16
-// (log x, y) pairs on its last axis, a Hull-White value of 9..32 factors.
+// (k_euler_wide): a dw correlated over 9..64 components, a Heston state of 5..32
+// (log x, y) pairs on its last axis, a Hull-White value of 9..64 factors.
 // sdepy draws such a block with one multivariate normal of any size
 // (wiener_source.__call__, infrastructure.py:1524-1553) and steps it with the
 // same euler_next update as any other state (integration.py:707-723).
 //
 // The block's width m is a run-time value (EulerArgs::nvars); the kernel is
-// compiled for a capacity W of 16 or 32 components, holds the state in W
+// compiled for a capacity W of 16, 32 or 64 components, holds the state in W
 // registers and guards every component loop by v < m (uniform over the
 // wave).  It does everything the general k_euler instance does (component
 // groups, recorded increments, antithetic halves, per-path parameters,
@@ -375,37 +375,29 @@ WienerFillWide<float> wiener_fill_wide_kernel<float>(int m);
 template <>
 WienerFillWide<double> wiener_fill_wide_kernel<double>(int m);
 
+#define SDEGPU_WIDE_PICK(MODEL, T)                                                        \
+  (m <= 16 ? k_euler_wide<MODEL, T, 16>                                                   \
+           : m <= 32 ? k_euler_wide<MODEL, T, 32> : k_euler_wide<MODEL, T, 64>)
+
 #define SDEGPU_EULER_WIDE_SET(T)                                                          \
   template <>                                                                             \
   EulerKernel<T> euler_wide_kernel<T>(int model, int m) {                                 \
     if (m <= SDEGPU_MAX_VARS || m > SDEGPU_MAX_COUPLED) return nullptr;                    \
-    const bool w16 = m <= 16;                                                             \
     switch (model) {                                                                      \
-      case SDEGPU_MODEL_WIENER:                                                           \
-        return w16 ? k_euler_wide<SDEGPU_MODEL_WIENER, T, 16>                             \
-                   : k_euler_wide<SDEGPU_MODEL_WIENER, T, 32>;                            \
-      case SDEGPU_MODEL_LOGNORM:                                                          \
-        return w16 ? k_euler_wide<SDEGPU_MODEL_LOGNORM, T, 16>                            \
-                   : k_euler_wide<SDEGPU_MODEL_LOGNORM, T, 32>;                           \
-      case SDEGPU_MODEL_OU:                                                               \
-        return w16 ? k_euler_wide<SDEGPU_MODEL_OU, T, 16> : k_euler_wide<SDEGPU_MODEL_OU, T, 32>; \
-      case SDEGPU_MODEL_CIR:                                                              \
-        return w16 ? k_euler_wide<SDEGPU_MODEL_CIR, T, 16>                                \
-                   : k_euler_wide<SDEGPU_MODEL_CIR, T, 32>;                               \
-      case SDEGPU_MODEL_HESTON:                                                           \
-        return w16 ? k_euler_wide<SDEGPU_MODEL_HESTON, T, 16>                             \
-                   : k_euler_wide<SDEGPU_MODEL_HESTON, T, 32>;                            \
-      case SDEGPU_MODEL_JUMPDIFF:                                                         \
-        return w16 ? k_euler_wide<SDEGPU_MODEL_JUMPDIFF, T, 16>                           \
-                   : k_euler_wide<SDEGPU_MODEL_JUMPDIFF, T, 32>;                          \
-      default:                                                                            \
-        return nullptr;                                                                   \
+      case SDEGPU_MODEL_WIENER: return SDEGPU_WIDE_PICK(SDEGPU_MODEL_WIENER, T);           \
+      case SDEGPU_MODEL_LOGNORM: return SDEGPU_WIDE_PICK(SDEGPU_MODEL_LOGNORM, T);         \
+      case SDEGPU_MODEL_OU: return SDEGPU_WIDE_PICK(SDEGPU_MODEL_OU, T);                   \
+      case SDEGPU_MODEL_CIR: return SDEGPU_WIDE_PICK(SDEGPU_MODEL_CIR, T);                 \
+      case SDEGPU_MODEL_HESTON: return SDEGPU_WIDE_PICK(SDEGPU_MODEL_HESTON, T);           \
+      case SDEGPU_MODEL_JUMPDIFF: return SDEGPU_WIDE_PICK(SDEGPU_MODEL_JUMPDIFF, T);       \
+      default: return nullptr;                                                            \
     }                                                                                     \
   }                                                                                       \
   template <>                                                                             \
   WienerFillWide<T> wiener_fill_wide_kernel<T>(int m) {                                   \
     if (m <= SDEGPU_MAX_VARS || m > SDEGPU_MAX_COUPLED) return nullptr;                    \
-    return m <= 16 ? k_wiener_fill_wide<T, 16> : k_wiener_fill_wide<T, 32>;               \
+    return m <= 16 ? k_wiener_fill_wide<T, 16>                                            \
+                   : m <= 32 ? k_wiener_fill_wide<T, 32> : k_wiener_fill_wide<T, 64>;     \
   }
 
 }  // namespace sdegpu

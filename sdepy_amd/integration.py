@@ -262,9 +262,10 @@ class _gpu_sde:
         return wtab
 
     def _output_map(self):
-        """(nout, out_mode, out_group, exp_mask)."""
+        """(nout, out_mode, out_group, exp_mask); the mask holds the bits of
+        values 0..31 (the kernels apply the model's rule beyond)."""
         M = self._nvars()
-        return M, _lib.OUT_IDENTITY, 1, ((1 << M) - 1) if self.log else 0
+        return M, _lib.OUT_IDENTITY, 1, _mask32((1 << M) - 1) if self.log else 0
 
     def _result(self, tt, xx):
         return process(t=tt, x=xx)
@@ -635,10 +636,16 @@ class _gpu_sde:
             return mg // out_group, g0 // out_group, 0
         if self.model == _lib.MODEL_HESTON:
             if nout == M // 2:
-                return mg // 2, g0 // 2, (1 << (mg // 2)) - 1
+                return mg // 2, g0 // 2, _mask32((1 << (mg // 2)) - 1)
             d = self._width()
-            return mg, g0, sum(1 << o for o in range(mg) if o % (2 * d) < d)
-        return mg, g0, ((1 << mg) - 1) if self.log else 0
+            return mg, g0, _mask32(sum(1 << o for o in range(mg) if o % (2 * d) < d))
+        return mg, g0, _mask32((1 << mg) - 1) if self.log else 0
+
+
+def _mask32(mask):
+    """The bits of values 0..31 of an exp mask (sdegpu_euler_problem's
+    exp_mask)."""
+    return mask & 0xFFFFFFFF
 
 
 class _Pass:
@@ -1111,7 +1118,7 @@ class full_heston_process(_gpu_sde):
     def _output_map(self):
         M, d = self._nvars(), self._width()
         mask = sum(1 << o for o in range(M) if o % (2 * d) < d)
-        return M, _lib.OUT_IDENTITY, 1, mask
+        return M, _lib.OUT_IDENTITY, 1, _mask32(mask)
 
     def _output_parts(self):
         M, d = self._nvars(), self._width()
@@ -1146,7 +1153,7 @@ class heston_process(full_heston_process):
 
     def _output_map(self):
         N = self._nvars() // 2
-        return N, _lib.OUT_FIRST, 1, (1 << N) - 1
+        return N, _lib.OUT_FIRST, 1, _mask32((1 << N) - 1)
 
     def _result(self, tt, xx):
         return process(tt, x=xx)

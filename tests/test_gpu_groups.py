@@ -34,7 +34,7 @@ def _corr(k, seed):
 
 def test_groups_split_as_documented(gpu):
     """Group sizes: whole blocks of the coupled axis, up to 8 components;
-    a coupled block of 9..32 on its own."""
+    a coupled block of 9..64 on its own."""
     p = sa.ornstein_uhlenbeck_process(paths=4, vshape=(5, 3), corr=_corr(3, 1))
     assert p._groups(True, 0, 1) == [(0, 6), (6, 6), (12, 3)]
     q = sa.lognorm_process(paths=4, vshape=(11,))
@@ -44,13 +44,13 @@ def test_groups_split_as_documented(gpu):
     # coupled blocks above 8 components: one launch each (the wide kernel)
     w = sa.ornstein_uhlenbeck_process(paths=4, vshape=(2, 9), corr=_corr(9, 2))
     assert w._groups(True, 0, 1) == [(0, 9), (9, 9)]
-    # above 32 coupled components: refused before any launch
+    # above 64 coupled components: refused before any launch
     with pytest.raises(NotImplementedError):
-        sa.ornstein_uhlenbeck_process(paths=4, vshape=33, corr=_corr(33, 2))((0., 1.))
+        sa.ornstein_uhlenbeck_process(paths=4, vshape=65, corr=_corr(65, 2))((0., 1.))
     with pytest.raises(NotImplementedError):
-        sa.full_heston_process(paths=4, vshape=17)((0., 1.))
+        sa.full_heston_process(paths=4, vshape=33)((0., 1.))
     with pytest.raises(NotImplementedError):
-        sa.hull_white_process(paths=4, factors=33)((0., 1.))
+        sa.hull_white_process(paths=4, factors=65)((0., 1.))
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])

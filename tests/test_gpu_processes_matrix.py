@@ -215,17 +215,17 @@ CONSTANT_CASES = [
 def test_many_components_and_coupled_limit(gpu):
     """Working states above 8 components run in groups of whole coupled
     blocks (Heston with vshape (2, 3, 1): 12 in pairs, Hull-White with 5
-    factors: 30 in fives); more than 32 coupled components (17 Heston pairs
-    on a single axis, 33 factors) are refused with NotImplementedError
+    factors: 30 in fives); more than 64 coupled components (33 Heston pairs
+    on a single axis, 65 factors) are refused with NotImplementedError
     before any launch."""
     for p in (sa.heston_process(vshape=(2, 3, 1), rho=.25, paths=11)(np.linspace(0, 1, 5)),
               sa.hull_white_process(vshape=(2, 3), factors=5, paths=11)((0., 1.))):
         assert p.vshape == (2, 3, 1) or p.vshape == (2, 3)
         assert np.isfinite(p).all()
     with pytest.raises(NotImplementedError):
-        sa.heston_process(vshape=(17,), paths=11)((0., 1.))
+        sa.heston_process(vshape=(33,), paths=11)((0., 1.))
     with pytest.raises(NotImplementedError):
-        sa.hull_white_process(vshape=(2,), factors=33, paths=11)((0., 1.))
+        sa.hull_white_process(vshape=(2,), factors=65, paths=11)((0., 1.))
 
 
 @pytest.mark.parametrize('k', range(len(CONSTANT_CASES)))

@@ -295,3 +295,74 @@ def test_sources_replay_and_time_dependent_corr(gpu):
                         sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (10,),
                                                         corr=corr)}, x0=0., mu=.1, sigma=.4)
     assert np.max(np.abs(x - ref)) < 1e-12
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_capacity_64(gpu, dtype):
+    """The third capacity (W = 64): OU over 40 correlated components, Wiener
+    over 64, full_heston with 20 pairs (40 components), Hull-White with 48
+    factors, and Merton over 36 correlated components (its FP64 instance
+    spills registers to scratch)."""
+    print('wide capacity 64', np.dtype(dtype).name, flush=True)
+    P, n, tt = 400, 12, np.linspace(0., 1., 4)
+    fp64 = dtype == np.float64
+    g = np.random.default_rng(60)
+    C = _corr(40, 61)
+    prm = dict(x0=g.uniform(-1, 1, (40, 1)), theta=g.uniform(-1, 1, (40, 1)),
+               k=g.uniform(.5, 3., (40, 1)), sigma=g.uniform(.1, .5, (40, 1)))
+    p = sa.ornstein_uhlenbeck_process(paths=P, vshape=40, dtype=dtype, steps=n, corr=C,
+                                      rng=np.random.default_rng(62), **prm)
+    x = p(tt)
+    ref = ora.integrate('ou', tt, paths=P, vshape=(40,), steps=n,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (40,),
+                                                        corr=C)}, **prm)
+    assert np.max(np.abs(x - ref)) < (1e-12 if fp64 else 3e-5)
+    C = _corr(64, 63)
+    p = sa.wiener_process(paths=P, vshape=64, dtype=dtype, steps=n, corr=C, x0=1., mu=.2,
+                          sigma=.5, rng=np.random.default_rng(64))
+    x = p(tt)
+    ref = ora.integrate('wiener', tt, paths=P, vshape=(64,), steps=n,
+                        sources={'dw': ora.PhiloxWiener(p.sources['dw'].key, 0, P, (64,),
+                                                        corr=C)}, x0=1., mu=.2, sigma=.5)
+    assert np.max(np.abs(x - ref)) < (1e-12 if fp64 else 3e-5)
+    if not fp64:
+        return
+    shp = (20, 1)
+    args = dict(x0=g.uniform(50., 150., shp), mu=g.uniform(-.05, .05, shp), sigma=1.,
+                y0=g.uniform(.01, .09, shp), theta=g.uniform(.02, .08, shp),
+                k=g.uniform(1., 3., shp), xi=g.uniform(.3, .8, shp))
+    C = _corr(40, 65)
+    info = {}
+    h = sa.full_heston_process(paths=P, vshape=20, steps=n, corr=C, info=info,
+                               rng=np.random.default_rng(66), **args)
+    hx, hy = h(tt)
+    ref_info = {}
+    rx, ry = ora.integrate('full_heston', tt, paths=P, vshape=(20,), steps=n, info=ref_info,
+                           sources={'dw': ora.PhiloxWiener(h.sources['dw'].key, 0, P, (40,),
+                                                           corr=C)}, **args)
+    assert _rel(hx, rx) < 1e-10 and np.max(np.abs(hy - ry)) < 1e-12
+    assert np.array_equal(info['negative_y_count'], ref_info['negative_y_count'])
+    hw = dict(x0=.01, theta=.02, k=g.uniform(.01, .3, (48, 1)),
+              sigma=g.uniform(.005, .02, (48, 1)))
+    q = sa.hull_white_process(paths=P, factors=48, steps=n, rho=None,
+                              rng=np.random.default_rng(67), **hw)
+    z = q(tt)
+    ref = ora.integrate('hull_white', tt, paths=P, factors=48, steps=n,
+                        sources={'dw': ora.PhiloxWiener(q.sources['dw'].key, 0, P, (48,))},
+                        **hw)
+    assert np.max(np.abs(z - ref)) < 1e-14
+    C = _corr(36, 68)
+    info = {}
+    m = sa.merton_jumpdiff_process(paths=P, vshape=36, steps=n, corr=C, info=info, x0=1.,
+                                   mu=.03, sigma=.2, lam=4., a=-.1, b=.15,
+                                   rng=np.random.default_rng(69))
+    w = m(tt)
+    ref_info = {}
+    ref = ora.integrate('jumpdiff', tt, paths=P, vshape=(36,), steps=n, info=ref_info,
+                        sources={'dw': ora.PhiloxWiener(m.sources['dw'].key, 0, P, (36,),
+                                                        corr=C),
+                                 'dj': ora.PhiloxCPoisson(m.sources['dj'].key, 0, P, (36,),
+                                                          lam=4., a=-.1, b=.15)},
+                        x0=1., mu=.03, sigma=.2)
+    assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+    assert _rel(w, ref) < RTOL64

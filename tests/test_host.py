@@ -92,7 +92,7 @@ def test_philox_host_entry_known_answer():
 def test_entry_point_validation_maps_to_exceptions():
     lib = _lib.lib()
     pr = _lib.EulerProblem()
-    pr.model, pr.dtype, pr.nvars, pr.paths = _lib.MODEL_LOGNORM, _lib.F32, 33, 10
+    pr.model, pr.dtype, pr.nvars, pr.paths = _lib.MODEL_LOGNORM, _lib.F32, 65, 10
     with pytest.raises(NotImplementedError):
         _lib.check(lib.sdegpu_euler(pr, None))
     pr.nvars, pr.nout, pr.ncoef = 1, 1, 1
