@@ -217,8 +217,9 @@ typedef struct sdegpu_euler_problem {
                                  intensities that vary across paths (poisson_source's lam
                                  broadcast to vshape + (paths,), infrastructure.py:1617-1633);
                                  pois then holds (|ds|, 0) per step and component and the
-                                 kernel takes lam |ds| and exp(-lam |ds|) in FP64.  Not with
-                                 antithetic jumps (anti_j). */
+                                 kernel takes lam |ds| and exp(-lam |ds|) in FP64.  With
+                                 antithetic jumps (anti_j = K) the trailing K paths hold
+                                 the leading K paths' intensities. */
   int64_t lam_paths_step;     /* 0, or the elements between consecutive steps' intensities:
                                  lam_paths is then [nsteps][nvars_total][paths], lam at each
                                  step's midpoint */

@@ -129,9 +129,8 @@ extern "C" int sdegpu_euler(const sdegpu_euler_problem* pr, void* stream) {
     return fail(SDEGPU_EUNSUPPORTED,
                 "a launch of %d Heston components steps one group of width %d, not %d", m, m / 2,
                 hd);
-  if (pr->lam_paths && (pr->lam_paths_step < 0 || pr->anti_j != 0 || pr->replay_dj))
-    return fail(SDEGPU_EINVAL,
-                "lam_paths: negative step, or with antithetic or recorded jumps");
+  if (pr->lam_paths && (pr->lam_paths_step < 0 || pr->replay_dj))
+    return fail(SDEGPU_EINVAL, "lam_paths: negative step, or with recorded jumps");
   if (pr->param_paths_step < 0)
     return fail(SDEGPU_EINVAL, "negative param_paths_step %lld", (long long)pr->param_paths_step);
   if (pr->info_row_slots < 0 || pr->info_row_slots > 1024)

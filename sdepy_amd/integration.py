@@ -470,11 +470,11 @@ class _gpu_sde:
                 if gdj.dn.per_path(t0s, ds):
                     # intensities that vary across paths: (|ds|, 0) rows
                     # and each path's lam, taken to lam |ds| on the device
-                    if gdj is not dj:
-                        raise NotImplementedError(
-                            'antithetic jumps with intensities that vary '
-                            'across paths')
+                    # (antithetic halves: the trailing paths repeat the
+                    # leading half's intensities, as they repeat its jumps)
                     pois, lam = gdj.dn.lam_path_tables(t0s, ds)
+                    if gdj is not dj:
+                        lam = np.concatenate((lam, lam), axis=-1)
                     d_lam = _lib.to_device(lam)
                     keep.append(d_lam)
                     prob.lam_paths = d_lam.data_ptr()

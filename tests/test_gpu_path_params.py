@@ -370,10 +370,24 @@ def test_sources_with_per_path_intensity(gpu):
     assert np.array_equal(np.asarray(counts).reshape(twin.dn_value.shape), twin.dn_value)
     with pytest.raises(ValueError):       # as sdepy: the half takes P/2 paths
         sa.even_cpoisson_source(paths=P, lam=g.uniform(1., 2., P))
-    h = g.uniform(1., 2., P // 2)
-    with pytest.raises(NotImplementedError):
-        sa.merton_jumpdiff_process(paths=P, dj=sa.even_cpoisson_source(
-            paths=P, lam=lambda t: h))((0., 1.))
+    # antithetic jumps with intensities per path of the leading half: the
+    # trailing half repeats the leading half's jumps
+    K = P // 2
+    h = g.uniform(1., 8., K)
+    info = {}
+    dj = sa.even_cpoisson_source(paths=P, lam=lambda t: h, y=sa.norm_rv(a=-.1, b=.15),
+                                 rng=np.random.default_rng(45))
+    m = sa.merton_jumpdiff_process(paths=P, steps=21, dj=dj, x0=1., mu=.03, sigma=.2,
+                                   info=info, rng=np.random.default_rng(46))
+    tt = np.linspace(0., 1., 5)
+    x = m(tt)
+    ref = ora.integrate('jumpdiff', tt, paths=P, steps=21,
+                        sources={'dw': ora.PhiloxWiener(m.sources['dw'].key, 0, P, ()),
+                                 'dj': ora.Antithetic(ora.PhiloxCPoisson(
+                                     dj.half.key, 0, K, (), lam=lambda t: h, a=-.1, b=.15),
+                                     1)},
+                        x0=1., mu=.03, sigma=.2)
+    assert _rel(x, ref) < RTOL64
 
 
 def test_poisson_means_above_700_per_step(gpu):
