@@ -127,11 +127,9 @@ typedef struct sdegpu_euler_problem {
   int32_t ncoef;      /* table row width */
   const void* coef;   /* device [nsteps][ncoef] */
   const double* pois; /* device [nsteps][nvars][2] = (lam*|ds|, exp(-lam*|ds|)), JUMPDIFF only;
-                         for lam*|ds| > 700 the second value is exp(-lam*|ds|/k),
-                         k = ceil(lam*|ds|/700): the count is then the sum of k counts
-                         of mean lam*|ds|/k, the first from the slot's word, part j >= 1
-                         from word (j-1)%4 of the Poisson block (slot, path,
-                         c2 | (1+(j-1)/4) << 8, epoch) */
+                         lam*|ds| at most 700 (exp(-lam*|ds|) underflows beyond ~745):
+                         draw a pass with larger means with sdegpu_cpoisson_fill and
+                         pass its increments as replay_dj */
   const int32_t* store_row; /* device [nsteps]: output row reached after step s, or -1 */
   int32_t row0;       /* output row of the initial state, or -1 */
   int32_t stream_tag; /* 0 for the forward pass, 16 for the backward pass (i0 != 0) */
@@ -383,6 +381,11 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  * sdegpu_poisson_fill_paths / sdegpu_cpoisson_fill_paths: the same with
  *   intensities that vary across paths, lam_paths [nslots][nvars][paths] FP64;
  *   pois then holds (|dt|, 0) per slot and component.
+ * All four take any mean: above 700 (lam*|dt| where exp(-lam*|dt|) would
+ *   underflow) the count is the sum of k = ceil(mean/700) counts of mean
+ *   mean/k, the first from the slot's word, part j >= 1 from word (j-1)%4 of
+ *   the Poisson block (slot, path, c2 | (1+(j-1)/4) << 8, epoch); the table's
+ *   second value is then exp(-mean/k).
  */
 int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
                        int32_t nvars, int64_t paths, int64_t nslots, const void* coef,

@@ -299,6 +299,12 @@ def zeros(shape, dtype):
     return t
 
 
+def cat_last(a, b):
+    """Device tensors a and b joined along their last axis."""
+    torch = require_gpu()
+    return torch.cat((a, b), dim=-1)
+
+
 def ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 

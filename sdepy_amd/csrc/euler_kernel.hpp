@@ -626,11 +626,9 @@ __global__ __launch_bounds__(kBlock, MOM ? 8 : 1) void k_euler(const EulerArgs a
             if (GEN && a.lamp) {
               // per-path intensity: pois holds |ds|
               lamds = a.lamp[(int64_t)s * a.lamp_step + (c0 + v) * P + gid] * lamds;
-              p0 = poisson_p0(lamds);
+              p0 = exp(-lamds);
             }
-            const int n = poisson_draw(upois[k * M + v], lamds, p0,
-                                       (uint32_t)((int64_t)s * mt + c0 + v), path_j, a.c2p, a.ej,
-                                       a.kj0, a.kj1, kPoissonCap);
+            const int n = poisson_inv(u01_f64(upois[k * M + v]), lamds, p0, kPoissonCap);
             T acc = (T)0;
             if (n > 0) {
               if (kDefer && !stored && njev < kJumpEvents && n < 256 && s < (1 << 20)) {

@@ -284,10 +284,9 @@ __global__ __launch_bounds__(kBlock) void k_euler_wide(const EulerArgs a,
               if (a.lamp) {
                 // per-path intensity: pois holds |ds|
                 lamds = a.lamp[(int64_t)s * a.lamp_step + (c0 + v) * P + gid] * lamds;
-                p0 = poisson_p0(lamds);
+                p0 = exp(-lamds);
               }
-              const int nj = poisson_draw(u[v], lamds, p0, (uint32_t)(g + v), path_j, a.c2p,
-                                          a.ej, a.kj0, a.kj1, kPoissonCap);
+              const int nj = poisson_inv(u01_f64(u[v]), lamds, p0, kPoissonCap);
               T acc = (T)0;
               if (nj > 0) {
                 const T p2 = a.jump_law == SDEGPU_JUMP_NORMAL ? (T)0 : p[4 * m + v];

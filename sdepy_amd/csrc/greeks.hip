@@ -479,8 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
           const T dw = z[j] * sq;
           T dj = (T)0;
           if (a.pois) {
-            const int cnt = poisson_draw(up[j], pois[2 * s], pois[2 * s + 1], (uint32_t)s, path,
-                                         kStreamPoisson, a.ej, a.kj0, a.kj1, 4096);
+            const int cnt = poisson_inv(u01_f64(up[j]), pois[2 * s], pois[2 * s + 1], 4096);
             if (cnt > 0) {
               njump += cnt;
               if (nev < kLogEvents && cnt < 256 && s < (1 << 24)) {

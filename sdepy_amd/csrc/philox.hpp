@@ -336,10 +336,9 @@ __host__ __device__ __forceinline__ double poisson_p0(double lam) {
   return lam <= kPoissonSplit ? exp(-lam) : exp(-(lam / ceil(lam / kPoissonSplit)));
 }
 
-__device__ __forceinline__ int poisson_draw(uint32_t word, double lam, double p0, uint32_t g,
-                                            uint32_t path, uint32_t c2p, uint32_t epoch,
-                                            uint32_t k0, uint32_t k1, int cap) {
-  if (lam <= kPoissonSplit) return poisson_inv(u01_f64(word), lam, p0, cap);
+__device__ __forceinline__ int poisson_split(uint32_t word, double lam, double p0, uint32_t g,
+                                             uint32_t path, uint32_t c2p, uint32_t epoch,
+                                             uint32_t k0, uint32_t k1, int cap) {
   const int k = (int)ceil(lam / kPoissonSplit);
   const double part = lam / (double)k;
   int n = poisson_inv(u01_f64(word), part, p0, cap);
@@ -351,6 +350,17 @@ __device__ __forceinline__ int poisson_draw(uint32_t word, double lam, double p0
     n += poisson_inv(u01_f64(w), part, p0, cap);
   }
   return n;
+}
+
+// A Poisson count of any mean.  Used by the stand-alone source fills; the
+// Euler kernels take means up to kPoissonSplit only (the host draws a pass
+// with larger means through the fills and replays it), so that the split
+// path costs their jump loops nothing.
+__device__ __forceinline__ int poisson_draw(uint32_t word, double lam, double p0, uint32_t g,
+                                            uint32_t path, uint32_t c2p, uint32_t epoch,
+                                            uint32_t k0, uint32_t k1, int cap) {
+  if (lam > kPoissonSplit) return poisson_split(word, lam, p0, g, path, c2p, epoch, k0, k1, cap);
+  return poisson_inv(u01_f64(word), lam, p0, cap);
 }
 
 }  // namespace sdegpu

@@ -107,12 +107,15 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
       if (g >= total) break;
       const int64_t s = g / nvars;
       // with per-path intensities pois holds |dt|: lam |dt| and its exp per path
-      double lamdt = pois[2 * g], p0 = pois[2 * g + 1];
+      int n;
       if (lamp) {
-        lamdt = lamp[g * paths + gid] * lamdt;
-        p0 = poisson_p0(lamdt);
+        const double lamdt = lamp[g * paths + gid] * pois[2 * g];
+        n = poisson_draw(bits[l], lamdt, poisson_p0(lamdt), (uint32_t)g, path, c2p, epoch, k0,
+                         k1, 4096);
+      } else {
+        n = poisson_draw(bits[l], pois[2 * g], pois[2 * g + 1], (uint32_t)g, path, c2p, epoch,
+                         k0, k1, 4096);
       }
-      const int n = poisson_draw(bits[l], lamdt, p0, (uint32_t)g, path, c2p, epoch, k0, k1, 4096);
       const int sg = sign[s];
       if (dn_out) dn_out[g * paths + gid] = (long long)(sg * n);
       if (out) {

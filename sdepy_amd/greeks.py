@@ -29,7 +29,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .infrastructure import philox_key, poisson_p0, _gpu_dtype, _get_default_rng
+from .infrastructure import philox_key, POISSON_SPLIT, _gpu_dtype, _get_default_rng
 
 HESTON_GREEKS = ('x0', 'r', 'k', 'theta', 'xi', 'rho', 'y0')
 LOG_GREEKS = ('x0', 'mu', 'sigma', 'a', 'b')
@@ -204,7 +204,11 @@ class LogGreeks:
         self.pois = None
         if lam > 0:
             ld = lam * steps_tab[:, 0]
-            self.pois = _lib.to_device(np.stack((ld, poisson_p0(ld)), axis=1))
+            if ld.max() > POISSON_SPLIT:
+                raise NotImplementedError(
+                    'the Greeks kernel takes jump intensities up to {} per '
+                    'step, not {}'.format(POISSON_SPLIT, ld.max()))
+            self.pois = _lib.to_device(np.stack((ld, np.exp(-ld)), axis=1))
             pr.pois = self.pois.data_ptr()
         pr.x0, pr.mu, pr.sigma, pr.a, pr.b = x0, mu, sigma, a, b
         pr.r, pr.strike, pr.maturity = r, strike, maturity

@@ -331,7 +331,7 @@ class poisson_source(_philox_source):
         return tab, np.ascontiguousarray(lam, dtype=float)
 
     def lam_table(self, tt, dtt):
-        """[slots][M][2] = (lam |dt|, exp(-lam |dt|)) in FP64."""
+        """[slots][M][2] = (lam |dt|, poisson_p0(lam |dt|)) in FP64."""
         M = int(np.prod(self.vshape, dtype=int))
         tab = np.zeros((tt.size, M, 2))
         for i in range(tt.size):
@@ -748,10 +748,22 @@ class cpoisson_source(_philox_source):
 
     def _gpu_draw(self, tf, dtf, laws, tshape):
         code, cdt, odt = _gpu_dtype(self.dtype)
+        shape = self.vshape + (self.paths,)
+        epoch = self.next_epoch()
+        self.dn.epoch = self.epoch
+        out, dn = self.fill_device(tf, dtf, laws, code, cdt, epoch, 0)
+        self.dn_value = _lib.to_host(dn).astype(self.ptype).reshape(tshape + shape)
+        self.y_value = None
+        return _lib.to_host(out).astype(odt, copy=False).reshape(tshape + shape)
+
+    def fill_device(self, tf, dtf, laws, code, cdt, epoch, stream_tag):
+        """Device arrays (dj [slots][M][paths] in cdt, dn int64) of the
+        slots (tf, dtf), drawn with the given epoch and stream tag: the
+        counters the Euler kernel's own draws use, so a pass drawn here and
+        replayed integrates as the kernel would have drawn it."""
         law, params = laws
         M = int(np.prod(self.vshape, dtype=int))
         n = tf.size
-        shape = self.vshape + (self.paths,)
         per_path = self.dn.per_path(tf, dtf)
         if per_path:
             pois, lam = self.dn.lam_path_tables(tf, dtf)
@@ -763,23 +775,19 @@ class cpoisson_source(_philox_source):
         dlaw = _lib.to_device(np.ascontiguousarray(params, dtype=cdt))
         out = _lib.empty((n, M, self.paths), cdt)
         dn = _lib.empty((n, M, self.paths), np.int64)
-        epoch = self.next_epoch()
-        self.dn.epoch = self.epoch
         if per_path:
             dlam = _lib.to_device(np.ascontiguousarray(
                 np.broadcast_to(lam, (n, M, self.paths))))
             _lib.check(_lib.lib().sdegpu_cpoisson_fill_paths(
-                code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
+                code, _lib.key2(self.key), epoch, stream_tag, M, self.paths, n,
                 _lib.ptr(dpois), _lib.ptr(dlam), _lib.ptr(dsign), law, _lib.ptr(dlaw),
                 _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
         else:
             _lib.check(_lib.lib().sdegpu_cpoisson_fill(
-                code, _lib.key2(self.key), epoch, 0, M, self.paths, n,
+                code, _lib.key2(self.key), epoch, stream_tag, M, self.paths, n,
                 _lib.ptr(dpois), _lib.ptr(dsign), law, _lib.ptr(dlaw),
                 _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
-        self.dn_value = _lib.to_host(dn).astype(self.ptype).reshape(tshape + shape)
-        self.y_value = None
-        return _lib.to_host(out).astype(odt, copy=False).reshape(tshape + shape)
+        return out, dn
 
 
 # --------------------------------------------------------------------------

@@ -33,7 +33,7 @@ from . import _lib
 from .infrastructure import (process, wiener_source, cpoisson_source,
                              poisson_source, norm_rv, per_component,
                              odd_wiener_source, even_cpoisson_source,
-                             double_exp_rv,
+                             double_exp_rv, POISSON_SPLIT,
                              _gpu_dtype, _shape_setup, _get_default_rng)
 
 __all__ = ['wiener_process', 'lognorm_process', 'ornstein_uhlenbeck_process',
@@ -452,7 +452,7 @@ class _gpu_sde:
                     prob.param_paths_step = path_params[0].size
         dj = self.sources.get('dj')
         self._anti_dj = False
-        pois = laws = None
+        pois = laws = dn_big = None
         if dj is not None:
             # a compound Poisson source whose law the kernel draws (the
             # leading half of an antithetic one: the trailing paths repeat
@@ -475,13 +475,36 @@ class _gpu_sde:
                     pois, lam = gdj.dn.lam_path_tables(t0s, ds)
                     if gdj is not dj:
                         lam = np.concatenate((lam, lam), axis=-1)
-                    d_lam = _lib.to_device(lam)
-                    keep.append(d_lam)
-                    prob.lam_paths = d_lam.data_ptr()
-                    prob.lam_paths_step = lam[0].size if lam.shape[0] > 1 else 0
+                    big = bool(lam.size and nsteps) and (
+                        lam.reshape(lam.shape[0], -1).max(axis=1)
+                        * np.abs(ds)).max() > POISSON_SPLIT
+                    if not big:
+                        d_lam = _lib.to_device(lam)
+                        keep.append(d_lam)
+                        prob.lam_paths = d_lam.data_ptr()
+                        prob.lam_paths_step = lam[0].size if lam.shape[0] > 1 else 0
                 else:
                     pois = gdj.dn.lam_table(t0s, ds)
+                    big = bool(pois.size) and pois[..., 0].max() > POISSON_SPLIT
                 prob.jump_law = laws[0]
+                if big:
+                    # a mean above POISSON_SPLIT, where exp(-mean) nears
+                    # underflow: the source's fill draws the pass with the
+                    # counters the kernel would use (splitting such means)
+                    # and the kernel replays the increments
+                    d_dj, d_dn = gdj.fill_device(t0s, ds, laws, code, cdt,
+                                                 prob.epoch_j, prob.stream_tag)
+                    if gdj is not dj:
+                        d_dj = _lib.cat_last(d_dj, d_dj)
+                        prob.anti_j = 0
+                    else:
+                        dn_big = _lib.to_host(d_dn)
+                    del d_dn
+                    keep.append(d_dj)
+                    prob.replay_dj = d_dj.data_ptr()
+                    # replayed increments: no law parameters in the table
+                    prob.jump_law = _lib.JUMP_NORMAL
+                    pois = laws = None
             else:
                 host_sources['dj'] = dj
         self._host_dj = 'dj' in host_sources
@@ -497,7 +520,7 @@ class _gpu_sde:
 
         # host sources, drawn in sorted id order per step (SDE.dZ)
         recorded = {k: [] for k in host_sources}
-        dn_host = []
+        dn_host = [] if dn_big is None else list(dn_big)
         for s, d in zip(t0s, ds):
             for sid in sorted(host_sources):
                 z = np.asarray(host_sources[sid](s, d), dtype=float)

@@ -413,6 +413,45 @@ def test_poisson_means_above_700_per_step(gpu):
         assert np.array_equal(info['jump_count'], ref_info['jump_count'])
         assert ref_info['jump_count'].mean() > 3000
         assert _rel(x, ref) < 1e-11
+    # antithetic halves, and a backward pass (i0 = 1: the stream tag of
+    # backward passes) with means above 700
+    K = P // 2
+    dj = sa.even_cpoisson_source(paths=P, lam=3000., y=sa.uniform_rv(a=-1e-4, b=1e-3),
+                                 rng=np.random.default_rng(53))
+    m = sa.merton_jumpdiff_process(paths=P, steps=n, dj=dj, x0=1., mu=.03, sigma=.2,
+                                   rng=np.random.default_rng(54))
+    x = m(tt)
+    ref = ora.integrate('jumpdiff', tt, paths=P, steps=n,
+                        sources={'dw': ora.PhiloxWiener(m.sources['dw'].key, 0, P, ()),
+                                 'dj': ora.Antithetic(ora.PhiloxCPoisson(
+                                     dj.half.key, 0, K, (), lam=3000., a=-1e-4, b=1e-3,
+                                     law='uniform'), 1)},
+                        x0=1., mu=.03, sigma=.2)
+    assert _rel(x, ref) < 1e-11
+    info = {}
+    m = sa.merton_jumpdiff_process(paths=P, steps=n, info=info, x0=1., mu=.03, sigma=.2,
+                                   lam=5000., a=-1e-4, b=1e-3, i0=1,
+                                   rng=np.random.default_rng(55))
+    x = m(tt)
+
+    def seq(cls):
+        # a source whose passes take epochs 0, 1, ... as the run's do
+        class _Seq(cls):
+            passes = 0
+
+            def begin_pass(self, tag):
+                super().begin_pass(tag)
+                self.epoch = self.passes
+                self.passes += 1
+        return _Seq
+    ref_info = {}
+    ref = ora.integrate('jumpdiff', tt, paths=P, steps=n, i0=1, info=ref_info,
+                        sources={'dw': seq(ora.PhiloxWiener)(m.sources['dw'].key, 0, P, ()),
+                                 'dj': seq(ora.PhiloxCPoisson)(m.sources['dj'].key, 0, P, (),
+                                                               lam=5000., a=-1e-4, b=1e-3)},
+                        x0=1., mu=.03, sigma=.2)
+    assert np.array_equal(info['jump_count'], ref_info['jump_count'])
+    assert _rel(x, ref) < 1e-11
     dn = sa.poisson_source(paths=P, lam=4000., rng=np.random.default_rng(52))
     counts = dn(0., .5)
     twin = ora.PhiloxCPoisson(dn.key, 0, P, (), lam=4000.)
