@@ -379,8 +379,10 @@ int sdegpu_payoff_moments(const void* x, int32_t dtype, int64_t paths, int32_t k
  *   p2); out [nslots][nvars][paths] dtype; dn_out optional int64 like
  *   poisson_fill.
  * sdegpu_poisson_fill_paths / sdegpu_cpoisson_fill_paths: the same with
- *   intensities that vary across paths, lam_paths [nslots][nvars][paths] FP64;
- *   pois then holds (|dt|, 0) per slot and component.
+ *   intensities that vary across paths, lam_paths FP64: slot s's
+ *   [nvars][paths] block starts at lam_paths + s * lam_paths_step
+ *   (nvars*paths for one block per slot, 0 for one block shared by every
+ *   slot); pois then holds (|dt|, 0) per slot and component.
  * All four take any mean: above 700 (lam*|dt| where exp(-lam*|dt|) would
  *   underflow) the count is the sum of k = ceil(mean/700) counts of mean
  *   mean/k, the first from the slot's word, part j >= 1 from word (j-1)%4 of
@@ -402,13 +404,14 @@ int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
  * 2017-2040) with lam broadcast to vshape + (paths,) varying across paths. */
 int sdegpu_poisson_fill_paths(const uint32_t key[2], uint32_t epoch, uint32_t stream_tag,
                               int32_t nvars, int64_t paths, int64_t nslots, const double* pois,
-                              const double* lam_paths, const int32_t* sign, long long* out,
-                              void* stream);
+                              const double* lam_paths, int64_t lam_paths_step,
+                              const int32_t* sign, long long* out, void* stream);
 int sdegpu_cpoisson_fill_paths(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                                uint32_t stream_tag, int32_t nvars, int64_t paths, int64_t nslots,
-                               const double* pois, const double* lam_paths, const int32_t* sign,
-                               int32_t jump_law, const void* law_params, void* out,
-                               long long* dn_out, void* stream);
+                               const double* pois, const double* lam_paths,
+                               int64_t lam_paths_step, const int32_t* sign, int32_t jump_law,
+                               const void* law_params, void* out, long long* dn_out,
+                               void* stream);
 
 /*
  * Pathwise Greeks by reverse-mode differentiation of the Euler scheme.

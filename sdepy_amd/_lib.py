@@ -112,10 +112,11 @@ SIGNATURES = [
       c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     ('sdegpu_poisson_fill_paths', ctypes.c_int,
      [_U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64, c_void_p,
-      c_void_p, c_void_p, c_void_p, c_void_p]),
+      c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     ('sdegpu_cpoisson_fill_paths', ctypes.c_int,
      [c_int32, _U32x2, c_uint32, c_uint32, c_int32, c_int64, c_int64,
-      c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+      c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+      c_void_p]),
     ('sdegpu_philox_host', ctypes.c_int,
      [ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
       ctypes.POINTER(c_uint32)]),

@@ -92,7 +92,8 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
                                                          const int32_t* __restrict__ sign, int law,
                                                          const T* __restrict__ lp, T* __restrict__ out,
                                                          long long* __restrict__ dn_out,
-                                                         const double* __restrict__ lamp) {
+                                                         const double* __restrict__ lamp,
+                                                         int64_t lamp_step) {
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (gid >= paths) return;
   const uint32_t path = (uint32_t)gid;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_poisson_fill(uint32_t k0, uint32_t k
       // with per-path intensities pois holds |dt|: lam |dt| and its exp per path
       int n;
       if (lamp) {
-        const double lamdt = lamp[g * paths + gid] * pois[2 * g];
+        const double lamdt = lamp[s * lamp_step + (g - s * nvars) * paths + gid] * pois[2 * g];
         n = poisson_draw(bits[l], lamdt, poisson_p0(lamdt), (uint32_t)g, path, c2p, epoch, k0,
                          k1, 4096);
       } else {
@@ -251,22 +252,28 @@ extern "C" int sdegpu_wiener_fill(int32_t dtype, const uint32_t key[2], uint32_t
 static int poisson_common(int32_t dtype, const uint32_t key[2], uint32_t epoch, uint32_t tag,
                           int32_t nvars, int64_t paths, int64_t nslots, const double* pois,
                           const int32_t* sign, int law, const void* lp, void* out,
-                          long long* dn_out, const double* lamp, void* stream) {
+                          long long* dn_out, const double* lamp, int64_t lamp_step,
+                          void* stream) {
   if (!key || nvars < 1 || paths < 1 || paths > kMaxPaths || nslots < 0)
     return fail(SDEGPU_EINVAL, "poisson_fill: nvars=%d paths=%lld slots=%lld", nvars,
                 (long long)paths, (long long)nslots);
   if (nslots == 0) return SDEGPU_OK;
   if (!pois || !sign) return fail(SDEGPU_EINVAL, "poisson_fill: null pointer");
   if (nslots * nvars > 0xFFFFFFFFll) return fail(SDEGPU_EINVAL, "poisson_fill: too many slots");
+  if (lamp && lamp_step < 0)
+    return fail(SDEGPU_EINVAL, "poisson_fill_paths: negative lam_paths_step %lld",
+                (long long)lamp_step);
   const uint32_t t = (uint32_t)tag & 0xF0u;
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(blocks_for(paths)), block(kBlock);
   if (dtype == SDEGPU_F32)
     hipLaunchKernelGGL(k_poisson_fill<float>, grid, block, 0, st, key[0], key[1], epoch, t, nvars,
-                       paths, nslots, pois, sign, law, (const float*)lp, (float*)out, dn_out, lamp);
+                       paths, nslots, pois, sign, law, (const float*)lp, (float*)out, dn_out, lamp,
+                       lamp_step);
   else if (dtype == SDEGPU_F64)
     hipLaunchKernelGGL(k_poisson_fill<double>, grid, block, 0, st, key[0], key[1], epoch, t, nvars,
-                       paths, nslots, pois, sign, law, (const double*)lp, (double*)out, dn_out, lamp);
+                       paths, nslots, pois, sign, law, (const double*)lp, (double*)out, dn_out, lamp,
+                       lamp_step);
   else
     return fail(SDEGPU_EUNSUPPORTED, "poisson_fill: dtype code %d", dtype);
   return launch_status("k_poisson_fill");
@@ -277,17 +284,18 @@ extern "C" int sdegpu_poisson_fill(const uint32_t key[2], uint32_t epoch, uint32
                                    const int32_t* sign, long long* out, void* stream) {
   if (!out) return fail(SDEGPU_EINVAL, "poisson_fill: null output");
   return poisson_common(SDEGPU_F64, key, epoch, stream_tag, nvars, paths, nslots, pois, sign,
-                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, nullptr, stream);
+                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, nullptr, 0, stream);
 }
 
 extern "C" int sdegpu_poisson_fill_paths(const uint32_t key[2], uint32_t epoch,
                                          uint32_t stream_tag, int32_t nvars, int64_t paths,
                                          int64_t nslots, const double* pois,
-                                         const double* lam_paths, const int32_t* sign,
-                                         long long* out, void* stream) {
+                                         const double* lam_paths, int64_t lam_paths_step,
+                                         const int32_t* sign, long long* out, void* stream) {
   if (!out || !lam_paths) return fail(SDEGPU_EINVAL, "poisson_fill_paths: null pointer");
   return poisson_common(SDEGPU_F64, key, epoch, stream_tag, nvars, paths, nslots, pois, sign,
-                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, lam_paths, stream);
+                        SDEGPU_JUMP_NORMAL, nullptr, nullptr, out, lam_paths, lam_paths_step,
+                        stream);
 }
 
 extern "C" int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32_t epoch,
@@ -299,19 +307,20 @@ extern "C" int sdegpu_cpoisson_fill(int32_t dtype, const uint32_t key[2], uint32
   if (jump_law < SDEGPU_JUMP_NORMAL || jump_law > SDEGPU_JUMP_DOUBLE_EXP)
     return fail(SDEGPU_EUNSUPPORTED, "cpoisson_fill: unknown jump law %d", jump_law);
   return poisson_common(dtype, key, epoch, stream_tag, nvars, paths, nslots, pois, sign, jump_law,
-                        law_params, out, dn_out, nullptr, stream);
+                        law_params, out, dn_out, nullptr, 0, stream);
 }
 
 extern "C" int sdegpu_cpoisson_fill_paths(int32_t dtype, const uint32_t key[2], uint32_t epoch,
                                           uint32_t stream_tag, int32_t nvars, int64_t paths,
                                           int64_t nslots, const double* pois,
-                                          const double* lam_paths, const int32_t* sign,
-                                          int32_t jump_law, const void* law_params, void* out,
-                                          long long* dn_out, void* stream) {
+                                          const double* lam_paths, int64_t lam_paths_step,
+                                          const int32_t* sign, int32_t jump_law,
+                                          const void* law_params, void* out, long long* dn_out,
+                                          void* stream) {
   if (!out || !law_params || !lam_paths)
     return fail(SDEGPU_EINVAL, "cpoisson_fill_paths: null pointer");
   if (jump_law < SDEGPU_JUMP_NORMAL || jump_law > SDEGPU_JUMP_DOUBLE_EXP)
     return fail(SDEGPU_EUNSUPPORTED, "cpoisson_fill_paths: unknown jump law %d", jump_law);
   return poisson_common(dtype, key, epoch, stream_tag, nvars, paths, nslots, pois, sign, jump_law,
-                        law_params, out, dn_out, lam_paths, stream);
+                        law_params, out, dn_out, lam_paths, lam_paths_step, stream);
 }

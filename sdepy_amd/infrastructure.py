@@ -357,13 +357,11 @@ class poisson_source(_philox_source):
         out = _lib.empty((n, M, self.paths), np.int64)
         if self.per_path(tf, dtf):
             pois, lam = self.lam_path_tables(tf, dtf)
-            dpois = _lib.to_device(pois)
-            dlam = _lib.to_device(np.ascontiguousarray(
-                np.broadcast_to(lam, (n, M, self.paths))))
+            dpois, dlam = _lib.to_device(pois), _lib.to_device(lam)
             _lib.check(_lib.lib().sdegpu_poisson_fill_paths(
                 _lib.key2(self.key), self.next_epoch(), 0, M, self.paths, n,
-                _lib.ptr(dpois), _lib.ptr(dlam), _lib.ptr(dsign), _lib.ptr(out),
-                _lib.stream()))
+                _lib.ptr(dpois), _lib.ptr(dlam), lam_step(lam), _lib.ptr(dsign),
+                _lib.ptr(out), _lib.stream()))
         else:
             dpois = _lib.to_device(self.lam_table(tf, dtf))
             _lib.check(_lib.lib().sdegpu_poisson_fill(
@@ -374,6 +372,12 @@ class poisson_source(_philox_source):
 
 
 POISSON_SPLIT = 700.
+
+
+def lam_step(lam):
+    """Elements between consecutive slots' intensities in a lam_path_tables
+    array: one [M][paths] block per slot, or 0 for one block shared by all."""
+    return int(lam[0].size) if lam.shape[0] > 1 else 0
 
 
 def poisson_p0(ld):
@@ -776,12 +780,11 @@ class cpoisson_source(_philox_source):
         out = _lib.empty((n, M, self.paths), cdt)
         dn = _lib.empty((n, M, self.paths), np.int64)
         if per_path:
-            dlam = _lib.to_device(np.ascontiguousarray(
-                np.broadcast_to(lam, (n, M, self.paths))))
+            dlam = _lib.to_device(lam)
             _lib.check(_lib.lib().sdegpu_cpoisson_fill_paths(
                 code, _lib.key2(self.key), epoch, stream_tag, M, self.paths, n,
-                _lib.ptr(dpois), _lib.ptr(dlam), _lib.ptr(dsign), law, _lib.ptr(dlaw),
-                _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
+                _lib.ptr(dpois), _lib.ptr(dlam), lam_step(lam), _lib.ptr(dsign), law,
+                _lib.ptr(dlaw), _lib.ptr(out), _lib.ptr(dn), _lib.stream()))
         else:
             _lib.check(_lib.lib().sdegpu_cpoisson_fill(
                 code, _lib.key2(self.key), epoch, stream_tag, M, self.paths, n,

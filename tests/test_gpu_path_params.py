@@ -368,6 +368,14 @@ def test_sources_with_per_path_intensity(gpu):
     twin = ora.PhiloxCPoisson(dn.key, 0, P, vs, lam=lam)
     twin(.2, .5)
     assert np.array_equal(np.asarray(counts).reshape(twin.dn_value.shape), twin.dn_value)
+    # several slots in one call share one block of per-path intensities
+    dn3 = sa.poisson_source(paths=P, vshape=vs, lam=lam, rng=np.random.default_rng(47))
+    t3, d3 = np.array([.2, .7, 1.]), np.array([.5, .3, -.2])
+    counts3 = np.asarray(dn3(t3, d3))
+    twin3 = ora.PhiloxCPoisson(dn3.key, 0, P, vs, lam=lam)
+    for i in range(3):
+        twin3(t3[i], d3[i])
+        assert np.array_equal(counts3[i].reshape(twin3.dn_value.shape), twin3.dn_value)
     with pytest.raises(ValueError):       # as sdepy: the half takes P/2 paths
         sa.even_cpoisson_source(paths=P, lam=g.uniform(1., 2., P))
     # antithetic jumps with intensities per path of the leading half: the
