@@ -144,6 +144,14 @@ template <typename T>
 __device__ __forceinline__ T pos_part(T v) {
   return v > (T)0 ? v : (T)0;
 }
+// FP32: one integer max of the bit pattern against 0 (a float max first
+// quiets its input, two instructions): a set sign bit is a negative integer,
+// so every negative value and -0 give +0 and the rest pass unchanged.
+template <>
+__device__ __forceinline__ float pos_part<float>(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, b > 0 ? b : 0);
+}
 
 // Running reverse-sweep state of one path: the constant log-price adjoint,
 // the variance adjoint and the four parameter sums.
@@ -190,7 +198,7 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const TV* __
           const TV vp = pos_part(v);
           const TV sv = tsqrt<TV>(vp);
           svs[j + q] = sv;
-          v = v + fma(sv * r[7], ws[j + q], fma(-r[4], vp, r[5]));
+          v = v + fma(-r[4], vp, fma(sv * r[7], ws[j + q], r[5]));
         }
       }
     }
@@ -257,8 +265,8 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
           const TV w = rho_v * (TV)z0 + rhoc_v * (TV)z[2 * j + 1];
           const TV vp = pos_part(V);
           const TV sv = tsqrt<TV>(vp);
-          X = X + fma((T)sv * r[6], z0, fma(-r[3], (T)vp, r[2]));
-          V = V + fma(sv * rv[7], w, fma(-rv[4], vp, rv[5]));
+          X = X + fma(-r[3], (T)vp, fma((T)sv * r[6], z0, r[2]));
+          V = V + fma(-rv[4], vp, fma(sv * rv[7], w, rv[5]));
         }
       }
     }
@@ -368,8 +376,8 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
           vr = fma(vr, av, cr);
           xv = fma(vv, bx, xv);
           vv = vv * av;
-          X = X + fma((T)sv * rx[6], z[2 * j], fma(-rx[3], (T)vp, rx[2]));
-          V = V + fma(sv * r[7], w, fma(-r[4], vp, r[5]));
+          X = X + fma(-rx[3], (T)vp, fma((T)sv * rx[6], z[2 * j], rx[2]));
+          V = V + fma(-r[4], vp, fma(sv * r[7], w, r[5]));
         }
       }
     }
