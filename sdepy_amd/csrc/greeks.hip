@@ -153,11 +153,13 @@ __device__ __forceinline__ float pos_part<float>(float v) {
   return __builtin_bit_cast(float, b > 0 ? b : 0);
 }
 
-// Running reverse-sweep state of one path: the constant log-price adjoint,
-// the variance adjoint and the four parameter sums.
+// Running reverse-sweep state of one path: the variance adjoint and the four
+// parameter sums, all per unit of the constant log-price adjoint Xbar (every
+// term of the variance adjoint is Xbar times a path quantity, so the sweep
+// carries U = Vbar / Xbar and the sums are scaled by Xbar once at the end).
 template <typename T>
 struct HestonRev {
-  T xbar, vbar, s1, s2, sx, sr;
+  T vbar, s1, s2, sx, sr;
 };
 
 // Waves per SIMD the Heston kernels are built for: FP32 8, FP32 with an FP64
@@ -222,7 +224,7 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const TV* __
       const TV h = trcp<TV>(sv);
       const TV c1 = fma(w, h * r[8], -r[4]);
       const TV c2 = fma(z0, h * r[9], -r[3]);
-      const TV d = fma(vb, c1, rv.xbar * c2);
+      const TV d = fma(vb, c1, c2);
       rv.vbar = sv > (TV)0 ? vb + d : vb;
     }
   }
@@ -279,11 +281,10 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
     const double pay = itm ? a.disc * intr : 0.0;
     // dP/dX_T = disc * (+-S) on the money
     const double xbar = itm ? (call ? a.disc * S : -a.disc * S) : 0.0;
-    const TV Xbar = (TV)xbar;
 
-    // ---- reverse sweep over the checkpoint segments ----
+    // ---- reverse sweep over the checkpoint segments, per unit of Xbar ----
     // Full segments run without step guards; only the last one can be short.
-    HestonRev<TV> rv{Xbar, (TV)0, (TV)0, (TV)0, (TV)0, (TV)0};
+    HestonRev<TV> rv{(TV)0, (TV)0, (TV)0, (TV)0, (TV)0};
     const int nfull = n / C;
     if (nfull * C < n)
       heston_segment<T, TV, C, false>(a, tv, ck, gid, path, nfull, n, rho_v, rhoc_v, rv);
@@ -291,14 +292,15 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
       heston_segment<T, TV, C, true>(a, tv, ck, gid, path, c, n, rho_v, rhoc_v, rv);
     const TV Vbar = rv.vbar, S1 = rv.s1, S2 = rv.s2, Sx = rv.sx, Sr = rv.sr;
 
+    // out of the money every Greek is 0, whatever the per-unit sums hold
     const double s1 = (double)S1;
     const double g[kHestonNG] = {xbar / a.x0,
                                  xbar * *sum_ds - a.maturity * pay,
-                                 a.th * s1 - (double)S2,
-                                 a.k * s1,
-                                 (double)Sx,
-                                 a.xi / (a.rhoc * a.rhoc) * (double)Sr,
-                                 (double)Vbar};
+                                 itm ? xbar * (a.th * s1 - (double)S2) : 0.0,
+                                 itm ? xbar * a.k * s1 : 0.0,
+                                 itm ? xbar * (double)Sx : 0.0,
+                                 itm ? xbar * a.xi / (a.rhoc * a.rhoc) * (double)Sr : 0.0,
+                                 itm ? xbar * (double)Vbar : 0.0};
     acc[0] = pay;
     acc[1] = pay * pay;
 #pragma unroll
