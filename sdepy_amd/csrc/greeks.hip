@@ -83,8 +83,9 @@ __device__ __forceinline__ double trcp<double>(double v) {
 // Per-step constants of the Heston step, built once per launch on the device
 // from the caller's (ds, sqrt|ds|) table, in FP64 and then rounded to T:
 //   [0] ds            [1] sq = sqrt|ds|   [2] r ds          [3] sigma^2/2 ds
-//   [4] k ds          [5] k theta ds      [6] sigma sq      [7] xi sq
-//   [8] xi sq / 2     [9] sigma sq / 2    [10] 1 - k ds
+//   [4] k ds          [5] k theta ds      [6] sigma sq zs   [7] xi sq
+//   [8] xi sq / 2     [9] sigma sq zs / 2 [10] 1 - k ds
+// (zs: the factor the kernels' normals are divided by, kGaussFold)
 // With them the forward step is
 //   X' = X + (sigma sq sv z0 + (r ds - sigma^2/2 ds y+))
 //   V' = V + (xi sq sv w    + (k theta ds - k ds y+)),   w = rho z0 + rhoc z1
@@ -92,10 +93,12 @@ __device__ __forceinline__ double trcp<double>(double v) {
 // (integration.py:2425-2433) with the increments summed before the add.
 constexpr int kHestonTab = 12;
 
+// zs multiplies the two columns that meet a normal directly (6 and 9): the
+// Greeks kernels draw their FP32 normals divided by zs = kGaussFold<float>.
 template <typename T, typename TO = T>
 __global__ __launch_bounds__(kBlock) void k_heston_table(const T* __restrict__ steps, int n,
                                                          double r, double sig, double k, double th,
-                                                         double xi, TO* __restrict__ tab,
+                                                         double xi, double zs, TO* __restrict__ tab,
                                                          double* __restrict__ sum_ds) {
   __shared__ double sh[kBlock];
   double part = 0.0;
@@ -108,10 +111,10 @@ __global__ __launch_bounds__(kBlock) void k_heston_table(const T* __restrict__ s
     row[3] = (TO)(0.5 * sig * sig * ds);
     row[4] = (TO)(k * ds);
     row[5] = (TO)(k * th * ds);
-    row[6] = (TO)(sig * sq);
+    row[6] = (TO)(sig * sq * zs);
     row[7] = (TO)(xi * sq);
     row[8] = (TO)(0.5 * xi * sq);
-    row[9] = (TO)(0.5 * sig * sq);
+    row[9] = (TO)(0.5 * sig * sq * zs);
     row[10] = (TO)(1.0 - k * ds);
     row[11] = (TO)0;
     part += ds;
@@ -172,6 +175,9 @@ constexpr int heston_waves() {
 // One checkpoint segment of the reverse sweep: recompute the segment's
 // variances and normals from its checkpoint and the Philox counters into
 // registers, then walk its steps backwards.  FULL segments hold C steps.
+// The normals are those of normals4_folded (divided by kGaussFold<T>): rho
+// and rhoc come multiplied by it, table columns 6 and 9 too, and the rho
+// sum is taken per unit of it (rho_sr = rho / kGaussFold<T>).
 // The normals are drawn in T; the variance, its adjoint and the parameter
 // sums are carried in TV (T, or FP64 with FP32 normals), with the step
 // constants of the table tv of that type.
@@ -179,7 +185,7 @@ template <typename T, typename TV, int C, bool FULL>
 __device__ __forceinline__ void heston_segment(const HestonArgs& a, const TV* __restrict__ tv,
                                                const TV* __restrict__ ck, int64_t gid,
                                                uint32_t path, int c, int n, TV rho, TV rhoc,
-                                               HestonRev<TV>& rv) {
+                                               TV rho_sr, HestonRev<TV>& rv) {
   const int s0 = c * C;
   TV svs[C], ws[C];
   T z0s[C];
@@ -189,7 +195,8 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const TV* __
     if (FULL || s0 + j < n) {
       __asm__ volatile("" ::: "memory");
       T z[4];
-      normals4<T>((uint32_t)((s0 + j) >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
+      normals4_folded<T>((uint32_t)((s0 + j) >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1,
+                         z);
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int s = s0 + j + q;
@@ -220,7 +227,7 @@ __device__ __forceinline__ void heston_segment(const HestonArgs& a, const TV* __
       rv.s2 = fma(u, sv * sv, rv.s2);
       const TV b = vb * (sv * r[1]);
       rv.sx = fma(b, w, rv.sx);
-      rv.sr = fma(b, fma(-rho, w, z0), rv.sr);
+      rv.sr = fma(b, fma(-rho_sr, w, z0), rv.sr);
       const TV h = trcp<TV>(sv);
       const TV c1 = fma(w, h * r[8], -r[4]);
       const TV c2 = fma(z0, h * r[9], -r[3]);
@@ -242,8 +249,10 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
   const uint32_t path = (uint32_t)gid;
   const int64_t P = a.paths;
   const int n = a.nsteps;
-  const T rho = (T)a.rho, rhoc = (T)a.rhoc;
-  const TV rho_v = (TV)a.rho, rhoc_v = (TV)a.rhoc;
+  // the normals come divided by zf (normals4_folded): their coefficients
+  // carry it (table columns 6 and 9 too), the rho sum is per unit of it
+  constexpr double zf = kGaussFold<T>;
+  const TV rho_v = (TV)(a.rho * zf), rhoc_v = (TV)(a.rhoc * zf), rho_sr = (TV)(a.rho / zf);
 
   double acc[kHestonNS];
 #pragma unroll
@@ -255,7 +264,7 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
     TV V = (TV)a.y0;
     for (int s0 = 0; s0 < n; s0 += 2) {
       T z[4];
-      normals4<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
+      normals4_folded<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int s = s0 + j;
@@ -287,9 +296,10 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
     HestonRev<TV> rv{(TV)0, (TV)0, (TV)0, (TV)0, (TV)0};
     const int nfull = n / C;
     if (nfull * C < n)
-      heston_segment<T, TV, C, false>(a, tv, ck, gid, path, nfull, n, rho_v, rhoc_v, rv);
+      heston_segment<T, TV, C, false>(a, tv, ck, gid, path, nfull, n, rho_v, rhoc_v, rho_sr,
+                                      rv);
     for (int c = nfull - 1; c >= 0; --c)
-      heston_segment<T, TV, C, true>(a, tv, ck, gid, path, c, n, rho_v, rhoc_v, rv);
+      heston_segment<T, TV, C, true>(a, tv, ck, gid, path, c, n, rho_v, rhoc_v, rho_sr, rv);
     const TV Vbar = rv.vbar, S1 = rv.s1, S2 = rv.s2, Sx = rv.sx, Sr = rv.sr;
 
     // out of the money every Greek is 0, whatever the per-unit sums hold
@@ -299,7 +309,7 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
                                  itm ? xbar * (a.th * s1 - (double)S2) : 0.0,
                                  itm ? xbar * a.k * s1 : 0.0,
                                  itm ? xbar * (double)Sx : 0.0,
-                                 itm ? xbar * a.xi / (a.rhoc * a.rhoc) * (double)Sr : 0.0,
+                                 itm ? xbar * a.xi / (a.rhoc * a.rhoc) * zf * (double)Sr : 0.0,
                                  itm ? xbar * (double)Vbar : 0.0};
     acc[0] = pay;
     acc[1] = pay * pay;
@@ -335,7 +345,9 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
   const bool live = gid < a.paths;
   const uint32_t path = (uint32_t)gid;
   const int n = a.nsteps;
-  const TV rho = (TV)a.rho, rhoc = (TV)a.rhoc;
+  // normals divided by zf, as in k_heston_adjoint
+  constexpr double zf = kGaussFold<T>;
+  const TV rho = (TV)(a.rho * zf), rhoc = (TV)(a.rhoc * zf), rho_r = (TV)(a.rho / zf);
 
   double acc[kHestonNS];
 #pragma unroll
@@ -349,7 +361,7 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
     TV xr = (TV)0, vr = (TV)0, xv = (TV)0, vv = (TV)1;
     for (int s0 = 0; s0 < n; s0 += 2) {
       T z[4];
-      normals4<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
+      normals4_folded<T>((uint32_t)(s0 >> 1), path, kStreamNormal, a.epoch, a.k0, a.k1, z);
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int s = s0 + j;
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
           const TV ssq = sv * r[1];
           const TV c2 = vp * r[0];
           const TV cx = ssq * w;
-          const TV cr = ssq * fma(-rho, w, z0);
+          const TV cr = ssq * fma(-rho_r, w, z0);
           x1 = fma(v1, bx, x1);
           v1 = fma(v1, av, r[0]);
           x2 = fma(v2, bx, x2);
@@ -395,7 +407,7 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
                                  xbar * (a.th * d1 - (double)x2),
                                  xbar * a.k * d1,
                                  xbar * (double)xx,
-                                 xbar * a.xi / (a.rhoc * a.rhoc) * (double)xr,
+                                 xbar * a.xi / (a.rhoc * a.rhoc) * zf * (double)xr,
                                  xbar * (double)xv};
     acc[0] = pay;
     acc[1] = pay * pay;
@@ -645,14 +657,15 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
   if (g->dtype == SDEGPU_F32) {
     float* tab = (float*)(base + lay.tab);
     hipLaunchKernelGGL(k_heston_table<float>, dim3(1), dim3(kBlock), 0, st, (const float*)g->steps,
-                       g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi, tab, sum_ds);
+                       g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi, kGaussFold<float>, tab,
+                       sum_ds);
     if (int e = launch_status("k_heston_table")) return e;
     if (v64) {
       // the variance's constants from the same FP32 (ds, sqrt ds), in FP64
       double* tabv = (double*)(base + lay.tabv);
       hipLaunchKernelGGL((k_heston_table<float, double>), dim3(1), dim3(kBlock), 0, st,
                          (const float*)g->steps, g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi,
-                         tabv, sum_ds);
+                         kGaussFold<float>, tabv, sum_ds);
       if (int e = launch_status("k_heston_table")) return e;
       if (g->method == 1)
         hipLaunchKernelGGL((k_heston_tangent<float, double>), dim3(nb), dim3(kBlock), 0, st, a,
@@ -673,7 +686,7 @@ extern "C" int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* o
     double* tab = (double*)(base + lay.tab);
     hipLaunchKernelGGL(k_heston_table<double>, dim3(1), dim3(kBlock), 0, st,
                        (const double*)g->steps, g->nsteps, g->r, g->sigma, g->k, g->theta, g->xi,
-                       tab, sum_ds);
+                       kGaussFold<double>, tab, sum_ds);
     if (int e = launch_status("k_heston_table")) return e;
     if (g->method == 1)
       hipLaunchKernelGGL((k_heston_tangent<double, double>), dim3(nb), dim3(kBlock), 0, st, a,

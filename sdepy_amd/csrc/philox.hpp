@@ -156,6 +156,31 @@ __device__ __forceinline__ void normals4(uint32_t blk, uint32_t path, uint32_t c
   Gauss<T>::pair(r.z, r.w, z[2], z[3]);
 }
 
+// The same four normals divided by kGaussFold<T>: in FP32 the pair's radius
+// is sqrt(-log2 u) with the factor sqrt(2 ln 2) left out (one multiply less
+// per pair), for callers that fold kGaussFold into the constants the normals
+// meet; in FP64 the normals themselves (kGaussFold = 1).
+template <typename T>
+constexpr double kGaussFold = sizeof(T) == 4 ? 1.1774100225154747 : 1.0;
+
+template <typename T>
+__device__ __forceinline__ void normals4_folded(uint32_t blk, uint32_t path, uint32_t c2,
+                                                uint32_t epoch, uint32_t k0, uint32_t k1, T* z) {
+  if constexpr (sizeof(T) == 4) {
+    const ctr4 r = philox4x32_10(ctr4{blk, path, c2, epoch}, k0, k1);
+    const uint32_t a[2] = {r.x, r.z}, b[2] = {r.y, r.w};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float ub = __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, b[q], 9u));
+      const float rr = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u01_f32(a[q])));
+      z[2 * q] = rr * __builtin_amdgcn_cosf(ub);
+      z[2 * q + 1] = rr * __builtin_amdgcn_sinf(ub);
+    }
+  } else {
+    normals4<T>(blk, path, c2, epoch, k0, k1, z);
+  }
+}
+
 // The normals with flat indices g .. g+M-1 of a path's stream, for any g
 // (used when a launch steps a group of components that starts at an index
 // that is not a multiple of 4): the blocks covering them are drawn and the
