@@ -58,6 +58,36 @@ def test_heston_tangent_fp64_matches_oracle(gpu, K):
         G.heston_greeks(paths=P, steps=n, method='backward', **HPRM)
 
 
+@pytest.mark.parametrize('n', [1, 2, 7, 8, 9, 17])
+def test_heston_greeks_short_and_partial_segments(gpu, n):
+    """Timelines of one step, of a partial checkpoint segment (8 steps
+    each), of exactly one and of one plus a partial one: both methods in
+    FP64 equal the oracle's adjoint on the same paths; in FP32 (normals drawn
+    without their Box-Muller radius factor, which the kernels fold into the
+    constants the normals meet) the price, x0 and r agree to FP32 rounding
+    and the variance Greeks within a fraction of their standard error."""
+    P = 2048
+    pay, g = og.heston_adjoint(KEY, 6, P, n, 1., strike=100., **HPRM)
+    ref = np.concatenate(([pay.mean()], g.mean(axis=1)))
+    se = np.concatenate(([pay.std()], g.std(axis=1))) / np.sqrt(P)
+    for method in ('adjoint', 'tangent'):
+        kw = dict(paths=P, steps=n, strike=100., key=KEY, epoch=6, method=method, **HPRM)
+        r64 = G.heston_greeks(dtype=np.float64, **kw)
+        got = np.concatenate(([r64.price], [r64.greeks[k] for k in og.HESTON_NAMES]))
+        assert np.allclose(got, ref, rtol=1e-10, atol=1e-12 * max(1., abs(ref).max())), method
+        r32 = G.heston_greeks(dtype=np.float32, **kw)
+        got = np.concatenate(([r32.price], [r32.greeks[k] for k in og.HESTON_NAMES]))
+        for i, name in enumerate(('price',) + tuple(og.HESTON_NAMES)):
+            if name in ('price', 'x0', 'r'):
+                assert abs(got[i] - ref[i]) <= 1e-4 * abs(ref[i]) + 1e-3 * se[i], (method, name)
+            else:
+                # (one step: the variance never reaches the price; two, with
+                # theta = y0: dV_1/dk = (theta - y0) ds = 0; k and theta are
+                # then exactly 0 in FP64 and an FP32 cancellation in FP32)
+                assert abs(got[i] - ref[i]) <= .05 * se[i] + 1e-3 * abs(ref[i]) + 1e-6, \
+                    (method, name)
+
+
 def test_heston_tangent_fp32_equals_adjoint_fp32(gpu):
     """FP32 forward mode and FP32 reverse mode on the same paths differ only
     by FP32 rounding of two orders of operations: far below the SE."""
