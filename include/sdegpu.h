@@ -296,7 +296,7 @@ int sdegpu_path_moments(const void* x, int32_t dtype, int64_t rows, int64_t path
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The chunk
  * partials live in a 4 MiB device array of the library that each GPU holds
  * until the process ends, which limits a call to
- * rows * ceil(paths / 65536) <= 262144 (beyond that: SDEGPU_ENOMEM, and the
+ * rows * ceil(paths / 16384) <= 262144 (beyond that: SDEGPU_ENOMEM, and the
  * workspace form serves).  Results equal sdegpu_path_moments' bit for bit.
  * All calls on one GPU share its array: calls that overlap on one GPU (on
  * two streams or from two threads) race.

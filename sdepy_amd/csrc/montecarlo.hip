@@ -25,7 +25,10 @@
 
 namespace sdegpu {
 
-constexpr int64_t kMcChunk = 65536;
+constexpr int64_t kMcChunk = 16384;  // as kChunk of the path moments
+// values per histogram block: its counts flush to global memory once per
+// block, so it keeps larger chunks (16384 measured 5% slower)
+constexpr int64_t kMcHistChunk = 65536;
 constexpr int kMcMoments = 5;      // sum d, d^2, d^3, d^4, sum v
 constexpr int kMcLdsBins = 4096;   // histograms up to this size count in LDS
 constexpr int kMcLdsCounterWords = 8192;  // LDS words for the counter copies
@@ -250,8 +253,8 @@ __global__ __launch_bounds__(kBlock) void k_mc_histogram(const T* __restrict__ x
   }
   const double lo_edge = e[0], hi_edge = e[nb];
   const double scale = hi_edge > lo_edge ? (double)nb / (hi_edge - lo_edge) : 0.0;
-  const int64_t lo = chunk * kMcChunk;
-  const int64_t hi = lo + kMcChunk < n ? lo + kMcChunk : n;
+  const int64_t lo = chunk * kMcHistChunk;
+  const int64_t hi = lo + kMcHistChunk < n ? lo + kMcHistChunk : n;
   auto visit = [&](T t) {
     const double v = (double)t;
     const int b = (v >= lo_edge && v <= hi_edge)
@@ -447,7 +450,7 @@ extern "C" int sdegpu_mc_histogram(const void* x, int32_t dtype, int64_t rows, i
   const int64_t nc = rows * (int64_t)(nbins + 1);
   hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nc)), dim3(kBlock), 0, st, counts, nc);
   if (int e = launch_status("k_zero_u64")) return e;
-  const dim3 grid((unsigned)sh.nchunks, (unsigned)rows);
+  const dim3 grid((unsigned)((sh.n + kMcHistChunk - 1) / kMcHistChunk), (unsigned)rows);
   const bool lds = nbins <= kMcLdsBins;
   // counter copies: a power of two up to 16 within kMcLdsCounterWords words
   int copies = 1;

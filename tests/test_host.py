@@ -159,7 +159,7 @@ def test_entry_point_validation_maps_to_exceptions():
     with pytest.raises(IndexError):                # negative component
         _lib.check(ax(one, 0, 1, 1, 4, 4, 4, (ctypes.c_int32 * 1)(-1), _lib.REDUCE_MAX,
                       0, 0, one, None))
-    assert lib.sdegpu_moments_workspace(3, 70000) == 3 * 2 * 2 * 8
+    assert lib.sdegpu_moments_workspace(3, 70000) == 3 * 5 * 2 * 8   # 16384-path chunks
     with pytest.raises(MemoryError):
         _lib.check(lib.sdegpu_path_moments(ctypes.c_void_p(8), 0, 3, 70000,
                                            ctypes.c_void_p(8), ctypes.c_void_p(8),

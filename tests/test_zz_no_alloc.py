@@ -15,7 +15,7 @@ from sdepy_amd import _lib
 pytestmark = pytest.mark.gpu
 
 # the array holds 4 MiB of FP64 (sum, sum of squares) pairs: one per
-# (row, 65536-path chunk)
+# (row, 16384-path chunk)
 SCRATCH_PAIRS = (4 << 20) // 16
 
 
