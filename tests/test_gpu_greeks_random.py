@@ -11,7 +11,7 @@ carries 1/sqrt(v) factors that amplify any rounding difference (the kernels'
 FP64 normals and orders of operations differ from NumPy's at the ulp).  The
 FP32 kernels of the two methods must agree with each other to a small
 fraction of the standard error (the same estimator, two orders of FP32
-operations).
+operations).  The lognormal / Merton Greeks kernel gets its own sweep.
 """
 import numpy as np
 import pytest
@@ -78,3 +78,34 @@ def test_heston_greeks_random_cases(gpu, seed, monkeypatch):
     a, t = r32['adjoint'], r32['tangent']
     for i, nm in enumerate(names):
         assert abs(a[i] - t[i]) <= .05 * se[i] + 1e-4 * abs(ref[i]) + 1e-6, (nm, a[i], t[i])
+
+
+def _loglin_case(seed):
+    g = np.random.default_rng(seed)
+    prm = dict(x0=float(g.uniform(.5, 2.)), mu=float(g.uniform(-.05, .1)),
+               sigma=float(g.uniform(.05, .6)), r=float(g.uniform(0., .06)),
+               lam=float(g.choice([0., g.uniform(.2, 8.)], p=[.25, .75])),
+               a=float(g.uniform(-.3, .2)), b=float(g.uniform(.01, .4)))
+    strike = float(prm['x0'] * g.uniform(.7, 1.3))
+    payoff = str(g.choice(['call', 'put']))
+    n = int(g.choice([1, 5, int(g.integers(2, 80))]))
+    maturity = float(g.uniform(.25, 2.))
+    return prm, strike, payoff, n, maturity
+
+
+@pytest.mark.parametrize('seed', range(300, 312))
+def test_loglin_greeks_random_cases(gpu, seed):
+    """Lognormal / Merton pathwise Greeks (a linear recursion in log x, so no
+    ill-conditioning): FP64 equals the oracle to 1e-10 relative, jump counts
+    and sizes drawn from the same Philox streams."""
+    prm, strike, payoff, n, T = _loglin_case(seed)
+    print('seed', seed, prm, strike, payoff, n, T, flush=True)
+    P = 2048
+    kw_, kj_ = (seed, 7), (11, seed)
+    pay, g = og.loglin_adjoint(kw_, kj_, 1, P, n, T, strike=strike, payoff=payoff, **prm)
+    ref = np.concatenate(([pay.mean()], g.mean(axis=1)))
+    res = G.merton_greeks(paths=P, steps=n, dtype=np.float64, key_w=kw_, key_j=kj_, epoch=1,
+                          maturity=T, strike=strike, payoff=payoff, **prm)
+    got = np.concatenate(([res.price], [res.greeks[k] for k in og.LOG_NAMES]))
+    scale = max(1e-3, float(np.abs(ref).max()))
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12 * scale), (got, ref)
