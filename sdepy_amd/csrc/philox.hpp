@@ -156,10 +156,11 @@ __device__ __forceinline__ void normals4(uint32_t blk, uint32_t path, uint32_t c
   Gauss<T>::pair(r.z, r.w, z[2], z[3]);
 }
 
-// The same four normals divided by kGaussFold<T>: in FP32 the pair's radius
-// is sqrt(-log2 u) with the factor sqrt(2 ln 2) left out (one multiply less
-// per pair), for callers that fold kGaussFold into the constants the normals
-// meet; in FP64 the normals themselves (kGaussFold = 1).
+// normals4's four normals divided by kGaussFold<T>, up to an ulp of
+// rounding: in FP32 the pair's radius is sqrt(-log2 u) with the factor
+// sqrt(2 ln 2) left out (one multiply less per pair), for callers that fold
+// kGaussFold into the constants the normals meet; in FP64 the normals
+// themselves (kGaussFold = 1).
 template <typename T>
 constexpr double kGaussFold = sizeof(T) == 4 ? 1.1774100225154747 : 1.0;
 
