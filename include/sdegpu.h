@@ -466,6 +466,8 @@ int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* out, void* w
  * exp(-r T) max(+-(x_T - strike), 0) for lognorm_process (pois = NULL) or
  * merton_jumpdiff_process with jumps a + b N(0,1); r is a fixed discount
  * rate.  pois: device [nsteps][2] = (lam*ds, exp(-lam*ds)) or NULL.
+ * maturity is the sum of the table's ds: the drift (mu - sigma^2/2) T of
+ * log x_T and the mu and sigma Greeks are formed with it.
  */
 typedef struct sdegpu_loglin_greeks {
   int32_t dtype;

@@ -447,14 +447,15 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
   const bool live = gid < a.paths;
   const uint32_t path = (uint32_t)gid;
   const int n = a.nsteps;
-  const T adt = (T)(a.mu - a.sig * a.sig / 2), sig = (T)a.sig;
   const T ja = (T)a.ja, jb = (T)a.jb;
   double acc[kLogNS];
 #pragma unroll
   for (int i = 0; i < kLogNS; ++i) acc[i] = 0.0;
   if (live) {
-    T X = (T)a.lx0;
-    T sdw = (T)0, szeta = (T)0;
+    // log x_T = log x0 + (mu - sigma^2/2) T + sigma sum dw + sum dj: only
+    // the sums of dw (which the sigma Greek needs anyway) and of the jump
+    // sizes are carried through the steps, and log x_T is formed once in FP64
+    T sdw = (T)0, szeta = (T)0, sj = (T)0;
     int njump = 0;
     // Sum of the cnt jump sizes of step s and of their normals: the jump
     // normals of (step s, jump q) are lane q % 4 of block (s, path,
@@ -497,9 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
       for (int j = 0; j < 4; ++j) {
         const int s = s0 + j;
         if (s < n) {
-          const T ds = st[2 * s], sq = st[2 * s + 1];
-          const T dw = z[j] * sq;
-          T dj = (T)0;
+          const T dw = z[j] * st[2 * s + 1];
           if (a.pois) {
             const int cnt = poisson_inv(u01_f64(up[j]), pois[2 * s], pois[2 * s + 1], 4096);
             if (cnt > 0) {
@@ -510,13 +509,13 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
                 for (int k = 0; k < kLogEvents; ++k) ev[k] = nev == k ? e : ev[k];
                 ++nev;
               } else {
-                T zsum = (T)0;
+                T dj = (T)0, zsum = (T)0;
                 jumps(s, cnt, dj, zsum);
+                sj = sj + dj;
                 szeta = szeta + zsum;
               }
             }
           }
-          X = X + ((adt * ds + sig * dw) + dj);
           sdw = sdw + dw;
         }
       }
@@ -526,11 +525,13 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
       if (k < nev) {
         T dj = (T)0, zsum = (T)0;
         jumps((int)(ev[k] >> 8), (int)(ev[k] & 255u), dj, zsum);
-        X = X + dj;
+        sj = sj + dj;
         szeta = szeta + zsum;
       }
     }
-    const double S = exp((double)X);
+    const double X = a.lx0 + (a.mu - a.sig * a.sig / 2) * a.maturity + a.sig * (double)sdw +
+                     (double)sj;
+    const double S = exp(X);
     const bool call = a.payoff == 0;
     const double intr = call ? S - a.strike : a.strike - S;
     const bool itm = intr > 0.0;
