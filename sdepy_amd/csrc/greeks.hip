@@ -90,7 +90,10 @@ __device__ __forceinline__ double trcp<double>(double v) {
 //   X' = X + (sigma sq sv z0 + (r ds - sigma^2/2 ds y+))
 //   V' = V + (xi sq sv w    + (k theta ds - k ds y+)),   w = rho z0 + rhoc z1
 // (sv = sqrt(y+), y+ = max(V, 0)), the scheme of full_heston_SDE.sde
-// (integration.py:2425-2433) with the increments summed before the add.
+// (integration.py:2425-2433).  The kernels add V's increment as one sum and
+// X's two state-dependent terms one after the other, with the sum of the
+// deterministic r ds added to log x_T after the loop (column 2 is unused
+// there); FP64 rounding apart, that is the same scheme.
 constexpr int kHestonTab = 12;
 
 // zs multiplies the two columns that meet a normal directly (6 and 9): the
@@ -276,14 +279,15 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_adjo
           const TV w = rho_v * (TV)z0 + rhoc_v * (TV)z[2 * j + 1];
           const TV vp = pos_part(V);
           const TV sv = tsqrt<TV>(vp);
-          X = X + fma(-r[3], (T)vp, fma((T)sv * r[6], z0, r[2]));
+          // the deterministic r ds terms are added once, after the loop
+          X = fma(-r[3], (T)vp, fma((T)sv * r[6], z0, X));
           V = V + fma(-rv[4], vp, fma(sv * rv[7], w, rv[5]));
         }
       }
     }
 
     // ---- payoff and terminal adjoints ----
-    const double S = exp((double)X);
+    const double S = exp((double)X + a.r * *sum_ds);
     const bool call = a.payoff == 0;
     const double intr = call ? S - a.strike : a.strike - S;
     const bool itm = intr > 0.0;
@@ -390,12 +394,12 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
           vr = fma(vr, av, cr);
           xv = fma(vv, bx, xv);
           vv = vv * av;
-          X = X + fma(-rx[3], (T)vp, fma((T)sv * rx[6], z[2 * j], rx[2]));
+          X = fma(-rx[3], (T)vp, fma((T)sv * rx[6], z[2 * j], X));
           V = V + fma(-r[4], vp, fma(sv * r[7], w, r[5]));
         }
       }
     }
-    const double S = exp((double)X);
+    const double S = exp((double)X + a.r * *sum_ds);
     const bool call = a.payoff == 0;
     const double intr = call ? S - a.strike : a.strike - S;
     const bool itm = intr > 0.0;
