@@ -495,9 +495,10 @@ int sdegpu_loglin_greeks_run(const sdegpu_loglin_greeks* g, double* out, void* w
  * reduces each terminal value with process.pmean/pvar (infrastructure.py:
  * 861-879).  Here every instance (a lognorm_process, wiener_process,
  * ornstein_uhlenbeck_process or hull_white_process with vshape ()) keeps its
- * own path count, FP32 step table (row layout of sdegpu_euler_problem.coef),
- * parameters and Philox key/epoch -- so it draws exactly what sdegpu_euler
- * would draw for it -- and the batch returns, per instance, the FP64 mean and
+ * own path count, FP32 step table (row layout of sdegpu_euler_problem.coef,
+ * but for LOGNORM and WIENER rows (ds, sqrt ds, a ds, b sqrt ds) in place of
+ * (ds, sqrt ds, a, b)), parameters and Philox key/epoch -- so it draws exactly
+ * what sdegpu_euler would draw for it -- and the batch returns, per instance, the FP64 mean and
  * sum of squared deviations of the terminal stored value (exp of the state
  * for LOGNORM, the sum of the factors for OU).
  *

@@ -115,6 +115,13 @@ def _instance(proc, timeline):
     tt, _, steps_tt = consolidate_timeline(timeline, proc.steps, 0)
     t0s, ds = steps_tt[:-1], np.diff(steps_tt)
     tab, has_corr = proc._coef_table(dw, t0s, ds)
+    if model != _lib.MODEL_OU:
+        # the kernel takes lognorm / wiener rows as (ds, sqrt ds, a ds,
+        # b sqrt ds): x + (a ds + b dw) with the products formed here in FP64
+        assert tab.shape[1] == 4 and not has_corr
+        tab = tab.copy()
+        tab[:, 2] *= tab[:, 0]
+        tab[:, 3] *= tab[:, 1]
     x0 = proc._x0_state(np.float32)
     if x0.ndim != 1:
         raise NotImplementedError('batched simulations take an initial value '

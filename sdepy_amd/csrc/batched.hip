@@ -52,7 +52,7 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
       const int s = s0 + k;
       if (s >= n) break;
       const float* c = tab + (int64_t)s * nc;
-      const float ds = c[0], sq = c[1];
+      const float sq = c[1];
       const float* p = c + poff;
       if (MODEL == SDEGPU_MODEL_OU && M > 1 && d.has_corr) {
         // rows: a = 1 - k ds, b = k theta ds; the Cholesky rows carry
@@ -66,6 +66,14 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
           for (int j = 0; j <= i; ++j) acc = fmaf(L[idx++], z[k * M + j], acc);
           x[i] = acc;
         }
+        continue;
+      }
+      if constexpr (MODEL != SDEGPU_MODEL_OU) {
+        static_assert(M == 1, "lognorm and wiener instances are scalar");
+        // lognorm / wiener (M == 1): the rows carry (a ds, b sqrt ds) in
+        // place of (a, b), so x + (a ds + b z sqrt ds) is two instructions
+        // with one scalar operand each
+        x[0] = fmaf(p[1], z[k], x[0]) + p[0];
         continue;
       }
       float dw[M];
@@ -83,14 +91,9 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
 #pragma unroll
         for (int v = 0; v < M; ++v) dw[v] = z[k * M + v] * sq;
       }
-      if constexpr (MODEL == SDEGPU_MODEL_OU) {
-        // rows: a = 1 - k ds, b = k theta ds, sigma
+      // OU rows: a = 1 - k ds, b = k theta ds, sigma
 #pragma unroll
-        for (int v = 0; v < M; ++v) x[v] = p[v] * x[v] + (p[M + v] + p[2 * M + v] * dw[v]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < M; ++v) x[v] = x[v] + (p[v] * ds + p[M + v] * dw[v]);
-      }
+      for (int v = 0; v < M; ++v) x[v] = p[v] * x[v] + (p[M + v] + p[2 * M + v] * dw[v]);
     }
   }
   float out = 0.f;
