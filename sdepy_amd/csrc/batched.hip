@@ -71,9 +71,9 @@ __device__ __forceinline__ float run_tile(const sdegpu_batch_instance& d, const 
       if constexpr (MODEL != SDEGPU_MODEL_OU) {
         static_assert(M == 1, "lognorm and wiener instances are scalar");
         // lognorm / wiener (M == 1): the rows carry (a ds, b sqrt ds) in
-        // place of (a, b), so x + (a ds + b z sqrt ds) is two instructions
-        // with one scalar operand each
-        x[0] = fmaf(p[1], z[k], x[0]) + p[0];
+        // place of (a, b); the increment is summed first, as sdepy's
+        // x + (a ds + b dw), so the small drift is not rounded against x
+        x[0] = x[0] + fmaf(p[1], z[k], p[0]);
         continue;
       }
       float dw[M];
