@@ -190,8 +190,8 @@ __device__ __forceinline__ void store_state(T* __restrict__ out, const EulerArgs
   T* o_row = row_base + gid;
   // MOM instances (VALU-bound) store through a buffer resource on the
   // block's wave-uniform row address plus the lane's byte offset, which
-  // takes the 64-bit address arithmetic off the lanes: 3% faster there, and
-  // none faster in the store-bound plain instances, which keep plain stores.
+  // takes the 64-bit address arithmetic off the lanes: 3% faster there, but
+  // no faster in the store-bound plain instances, which keep plain stores.
   T* const o_blk = row_base + (int64_t)blockIdx.x * kBlock;
   const uint32_t lane_bytes = threadIdx.x * (uint32_t)sizeof(T);
   (void)o_blk;
