@@ -23,6 +23,8 @@ cannot follow every path, and check what does not depend on the size:
   within 4 SE of Merton's (1976) Poisson-weighted series and its derivatives
   (central differences of the series, whose truncation error is far below
   the SE).
+* The 1024 heterogeneous GBM / Hull-White instances in one batch: chosen
+  instances equal their stand-alone runs, the smallest the oracle's twin.
 """
 import numpy as np
 import pytest
@@ -143,3 +145,52 @@ def test_merton_greeks_2M_x_252(gpu):
         dref = (series(**{name: prm[name] + h}) - series(**{name: prm[name] - h})) / (2 * h)
         z = abs(r64.greeks[name] - dref) / r64.greeks_se[name]
         assert z < 4., (name, r64.greeks[name], dref, z)
+
+
+def test_batched_1024_instances(gpu):
+    """The benchmark's 1024 heterogeneous GBM / 2-factor Hull-White instances
+    (7.5e10 path-steps) in one batch: chosen instances (the most paths and
+    the most steps of each kind, the fewest paths, random ones) equal their
+    stand-alone preset runs on the same Philox paths, and the smallest one
+    the oracle's FP64 twin."""
+    from sdepy_amd import batched as B
+    print('batched 1024 instances', flush=True)
+    specs = B.heterogeneous_specs(1024, seed=1234)
+    bt = B.BatchedTerminalMoments(specs, autotune=False, graph=False)
+    mean, var = bt.run()
+    assert mean.shape == var.shape == (1024,)
+    assert np.all(np.isfinite(mean)) and np.all(np.isfinite(var)) and np.all(var > 0)
+    fresh = B.heterogeneous_specs(1024, seed=1234)
+    paths = np.array([p.paths for p, _ in fresh])
+    steps = np.array([p.steps for p, _ in fresh])
+    half = len(fresh) // 2
+    pick = {int(np.argmax(paths[:half])), half + int(np.argmax(paths[half:])),
+            int(np.argmax(steps[:half])), half + int(np.argmax(steps[half:])),
+            int(np.argmin(paths))}
+    pick |= set(int(i) for i in np.random.default_rng(5).integers(0, len(fresh), 4))
+    for i in sorted(pick):
+        p, tl = fresh[i]
+        print('instance', i, p.paths, p.steps, flush=True)
+        x = p(tl)
+        m_ref = float(x.pmean(dtype=np.float64)[-1].reshape(-1)[0])
+        v_ref = float(x.pvar(dtype=np.float64)[-1].reshape(-1)[0])
+        scale = max(abs(m_ref), np.sqrt(v_ref), 1e-3)
+        assert abs(mean[i] - m_ref) < 1e-5 * scale, (i, mean[i], m_ref)
+        assert abs(var[i] - v_ref) < 1e-4 * v_ref + 1e-12, (i, var[i], v_ref)
+    # the smallest instance against the oracle's FP64 twin of its draws
+    i = int(np.argmin(paths))
+    p = fresh[i][0]
+    key, prm = p.sources['dw'].key, p._args
+    if i < half:
+        src = ora.PhiloxWiener(key, 0, p.paths, ())
+        ref = ora.integrate('lognorm', (0., 1.), paths=p.paths, steps=p.steps,
+                            sources={'dw': src}, x0=1., mu=prm['mu'], sigma=prm['sigma'])
+    else:
+        src = ora.PhiloxWiener(key, 0, p.paths, (2,), rho=prm['rho'])
+        ref = ora.integrate('hull_white', (0., 1.), paths=p.paths, steps=p.steps,
+                            sources={'dw': src}, factors=2, x0=0., theta=0.,
+                            k=prm['k'], sigma=prm['sigma'])
+    xt = ref[-1].reshape(-1)
+    sc = max(abs(xt.mean()), xt.std(), 1e-3)
+    assert abs(mean[i] - xt.mean()) < 2e-5 * sc
+    assert abs(var[i] - xt.var()) < 2e-4 * xt.var() + 1e-12
