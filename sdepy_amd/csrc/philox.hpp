@@ -334,16 +334,24 @@ __device__ __forceinline__ void words_span(uint64_t g, int cnt, uint32_t path, u
 // Poisson variate by sequential inversion of the CDF, in double precision on
 // every target so that counts are bit-identical to the host twin.
 // lam = intensity * |ds|, p0 = exp(-lam) (precomputed on the host in double).
-// The search is bounded: `cap` stops it once the pmf underflows.
+// The search is bounded: `cap` stops it once the pmf underflows.  The first
+// term is peeled: lam / 1 is lam exactly, so the division (a dozen FP64
+// instructions) runs only for counts of 2 and more, which at small means are
+// rare in a whole wave, while a count of 1 is not.
 __host__ __device__ __forceinline__ int poisson_inv(double u, double lam, double p0, int cap) {
 #pragma clang fp contract(off)
   int n = 0;
   double p = p0;
   double f = p0;
-  while (u > f && n < cap) {
-    n += 1;
-    p = p * (lam / (double)n);
+  if (u > f && cap > 0) {
+    n = 1;
+    p = p * lam;
     f = f + p;
+    while (u > f && n < cap) {
+      n += 1;
+      p = p * (lam / (double)n);
+      f = f + p;
+    }
   }
   return n;
 }
