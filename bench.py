@@ -93,7 +93,7 @@ ADJ_VALU_WAVE_INSTS = 2003390625      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
 OU_TRAFFIC_BYTES = 16024672920         # k_euler<OU, float, 4>, 1M x 1000
 BATCH_VALU_WAVE_INSTS = 31445184494    # k_batch_persistent, the 1024-instance set, 8 tiles per claim
-MERTON_VALU_WAVE_INSTS = 313663719     # k_loglin_greeks<float>, 2M x 252
+MERTON_VALU_WAVE_INSTS = 290682267     # k_loglin_greeks<float>, 2M x 252
 TANGENT_VALU_WAVE_INSTS = 1364890625   # k_heston_tangent<float, float>, 4M x 365
 
 
