@@ -465,7 +465,11 @@ int sdegpu_heston_greeks_run(const sdegpu_heston_greeks* g, double* out, void* w
  * Lognormal / Merton jump-diffusion (NG = 5): d/d(x0, mu, sigma, a, b) of
  * exp(-r T) max(+-(x_T - strike), 0) for lognorm_process (pois = NULL) or
  * merton_jumpdiff_process with jumps a + b N(0,1); r is a fixed discount
- * rate.  pois: device [nsteps][2] = (lam*ds, exp(-lam*ds)) or NULL.
+ * rate.  pois: device [nsteps][3] = (lam*ds, exp(-lam*ds), t) or NULL, t a
+ * double slot whose 64 bits hold the unsigned integer
+ * min(floor(exp(-lam*ds) * 2^32 + 0.5), 2^32 - 1): a Philox word below it
+ * maps to a uniform at or below exp(-lam*ds), a count of 0, so the kernel
+ * skips the inversion for it (any smaller t gives the same counts).
  * maturity is the sum of the table's ds: the drift (mu - sigma^2/2) T of
  * log x_T and the mu and sigma Greeks are formed with it.
  */
@@ -475,7 +479,7 @@ typedef struct sdegpu_loglin_greeks {
   int64_t paths;
   int32_t nsteps;
   const void* steps;   /* device [nsteps][2] = (ds, sqrt(ds)) */
-  const double* pois;  /* device [nsteps][2] or NULL */
+  const double* pois;  /* device [nsteps][3] or NULL */
   double x0, mu, sigma, a, b, r, strike, maturity;
   uint32_t key_w[2];
   uint32_t epoch_w;

@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kBlock, (heston_waves<T, TV>())) void k_heston_tang
 
 struct LogArgs {
   const void* steps;        // [n][2] (ds, sqrt ds)
-  const double* pois;       // [n][2] (lam ds, exp(-lam ds)) or null (no jumps)
+  const double* pois;       // [n][3] (lam ds, exp(-lam ds), zero-count bound) or null
   double* part;             // [nblocks][kLogNS]
   int64_t paths;
   int32_t nsteps, payoff;
@@ -503,8 +503,12 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
         const int s = s0 + j;
         if (s < n) {
           const T dw = z[j] * st[2 * s + 1];
-          if (a.pois) {
-            const int cnt = poisson_inv(u01_f64(up[j]), pois[2 * s], pois[2 * s + 1], 4096);
+          // a word below the step's bound (the low word of pois[s][2]) is a
+          // count of 0: one integer compare in place of forming the FP64
+          // uniform; words at or above it take the inversion, which
+          // re-checks against exp(-lam ds)
+          if (a.pois && up[j] >= reinterpret_cast<const uint32_t*>(pois)[6 * s + 4]) {
+            const int cnt = poisson_inv(u01_f64(up[j]), pois[3 * s], pois[3 * s + 1], 4096);
             if (cnt > 0) {
               njump += cnt;
               if (nev < kLogEvents && cnt < 256 && s < (1 << 24)) {

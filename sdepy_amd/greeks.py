@@ -208,7 +208,13 @@ class LogGreeks:
                 raise NotImplementedError(
                     'the Greeks kernel takes jump intensities up to {} per '
                     'step, not {}'.format(POISSON_SPLIT, ld.max()))
-            self.pois = _lib.to_device(np.stack((ld, np.exp(-ld)), axis=1))
+            p0 = np.exp(-ld)
+            # a word below min(floor(p0 2^32 + 0.5), 2^32 - 1) gives
+            # u01(word) <= p0, a count of 0: the kernel skips the inversion
+            thr = np.minimum(np.floor(p0 * 2. ** 32 + .5), 2. ** 32 - 1).astype(np.uint64)
+            tab = np.stack((ld, p0, np.zeros_like(ld)), axis=1)
+            tab[:, 2] = thr.view(np.float64)
+            self.pois = _lib.to_device(tab)
             pr.pois = self.pois.data_ptr()
         pr.x0, pr.mu, pr.sigma, pr.a, pr.b = x0, mu, sigma, a, b
         pr.r, pr.strike, pr.maturity = r, strike, maturity
