@@ -483,9 +483,9 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
     // draws its own (at lambda ds = 1/252 a wave otherwise draws a Philox
     // block at about one step in five for a single jumping lane).  Further
     // jump steps, counts above 255 and steps beyond 2^24 are drawn in place.
-    uint32_t ev[kLogEvents];
-#pragma unroll
-    for (int k = 0; k < kLogEvents; ++k) ev[k] = 0u;
+    // the events live in LDS, one column per lane: recording one is a store
+    // at a lane-computed address, not a select into each register
+    __shared__ uint32_t ev[kLogEvents][kBlock];
     int nev = 0;
     for (int s0 = 0; s0 < n; s0 += 4) {
       T z[4];
@@ -512,9 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
             if (cnt > 0) {
               njump += cnt;
               if (nev < kLogEvents && cnt < 256 && s < (1 << 24)) {
-                const uint32_t e = ((uint32_t)s << 8) | (uint32_t)cnt;
-#pragma unroll
-                for (int k = 0; k < kLogEvents; ++k) ev[k] = nev == k ? e : ev[k];
+                ev[nev][threadIdx.x] = ((uint32_t)s << 8) | (uint32_t)cnt;
                 ++nev;
               } else {
                 T dj = (T)0, zsum = (T)0;
@@ -532,7 +530,8 @@ __global__ __launch_bounds__(kBlock) void k_loglin_greeks(const LogArgs a,
     for (int k = 0; k < kLogEvents; ++k) {
       if (k < nev) {
         T dj = (T)0, zsum = (T)0;
-        jumps((int)(ev[k] >> 8), (int)(ev[k] & 255u), dj, zsum);
+        const uint32_t e = ev[k][threadIdx.x];
+        jumps((int)(e >> 8), (int)(e & 255u), dj, zsum);
         sj = sj + dj;
         szeta = szeta + zsum;
       }
