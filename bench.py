@@ -88,12 +88,12 @@ VALU_ISSUE_PEAK_PER_S = 256 * 2.4e9
 # (FETCH reads half the copied bytes: x2.0; WRITE is exact: x1.0), and the
 # kernel's VALU wave-instructions per launch (SQ_INSTS_VALU); the summaries
 # they come from are in profiles/
-ADJ_TRAFFIC_BYTES = 1383853363        # k_heston_adjoint<float, float, 8>, 4M x 365
+ADJ_TRAFFIC_BYTES = 1383912141        # k_heston_adjoint<float, float, 8>, 4M x 365
 ADJ_VALU_WAVE_INSTS = 2003390625      # of which transcendental:
 ADJ_VALU_TRANS_WAVE_INSTS = 251187500  # (issued beside the other VALU work)
-OU_TRAFFIC_BYTES = 16026335437         # k_euler<OU, float, 4>, 1M x 1000
+OU_TRAFFIC_BYTES = 16026172416         # k_euler<OU, float, 4>, 1M x 1000
 BATCH_VALU_WAVE_INSTS = 31445184494    # k_batch_persistent, the 1024-instance set, 8 tiles per claim
-MERTON_VALU_WAVE_INSTS = 261914091     # k_loglin_greeks<float>, 2M x 252
+MERTON_VALU_WAVE_INSTS = 249545552     # k_loglin_greeks<float>, 2M x 252
 TANGENT_VALU_WAVE_INSTS = 1364890625   # k_heston_tangent<float, float>, 4M x 365
 
 
